@@ -1,0 +1,197 @@
+/*
+ * ketsv.h — C ABI of the MI355X dense statevector backend for ket.
+ *
+ * One shared library (libketsv.so, gfx950) replaces ket's sparse
+ * Map<bigint, Complex> statevector path.  Every entry point is plain C:
+ * pointers, sizes and status codes, no C++ or torch types.
+ *
+ * Index convention (unchanged from ket, src/statevector.ts:4-5): bit q of a
+ * state index is qubit q (q0 = LSB).  Public q0-leftmost bitstrings are
+ * reversed by the host before they reach this ABI.
+ * Multi-qubit matrices are row-major 2^k x 2^k complex, interleaved
+ * (re, im) doubles; the first qubit operand is the MSB of the local index
+ * (src/statevector.ts:32, :104-109, :236-238).
+ *
+ * Status codes map onto the JavaScript error classes ket throws:
+ *   KSV_ERR_RANGE -> RangeError (qubit index / size / capacity,
+ *                    src/circuit.ts:1114-1117, :1469-1471)
+ *   KSV_ERR_TYPE  -> TypeError  (malformed op, tape, bitstring, control == target,
+ *                    src/circuit.ts:1118-1120, src/statevector.ts:213)
+ *   KSV_ERR_DEVICE-> Error      (HIP failure, out of device memory)
+ * ksv_last_error() returns the message of the last failure on the calling thread.
+ *
+ * Threading: one HIP stream per state handle; every call is synchronous
+ * with respect to the host (results are in caller buffers on return).
+ * Calls on one handle must not overlap.
+ */
+#ifndef KETSV_H
+#define KETSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSV_OK           0
+#define KSV_ERR_RANGE    1
+#define KSV_ERR_TYPE     2
+#define KSV_ERR_DEVICE   3
+#define KSV_ERR_HANDLE   4
+
+/* amplitude precision */
+#define KSV_F64 0   /* complex128 state, FP64 arithmetic                       */
+#define KSV_F32 1   /* complex64 state, FP32 arithmetic, FP64 accumulations    */
+
+/* op kinds: one per ket FlatOp kind (src/circuit.ts:22-47) */
+#define KSV_SINGLE      0   /* q = {q};               mat 2x2  (applySingle,   src/statevector.ts:47)  */
+#define KSV_CNOT        1   /* q = {control, target}                (applyCNOT,     src/statevector.ts:70)  */
+#define KSV_SWAP        2   /* q = {a, b}                           (applySWAP,     src/statevector.ts:85)  */
+#define KSV_TWO         3   /* q = {a, b};            mat 4x4  (applyTwo,      src/statevector.ts:110) */
+#define KSV_CONTROLLED  4   /* q = {control, target}; mat 2x2  (applyControlled, src/statevector.ts:212) */
+#define KSV_TOFFOLI     5   /* q = {c1, c2, target}                 (applyToffoli,  src/statevector.ts:138) */
+#define KSV_CSWAP       6   /* q = {control, a, b}                  (applyCSwap,    src/statevector.ts:153) */
+#define KSV_CSRSWAP     7   /* q = {control, a, b}                  (applyCsrSwap,  src/statevector.ts:177) */
+#define KSV_UNITARY     8   /* q = {qs[0] (MSB) ...}; mat 2^k x 2^k (applyUnitary, src/statevector.ts:240) */
+#define KSV_BARRIER     9   /* no effect on the state (src/circuit.ts:155)                                 */
+
+#define KSV_MAX_OPERANDS 12  /* widest KSV_UNITARY accepted */
+
+/* One op of the tape.  `mat` is an offset, in doubles, into the tape's
+ * matrix array (interleaved re/im), or -1 for kinds without a matrix. */
+typedef struct ksv_op {
+    int32_t kind;
+    int32_t nq;
+    int32_t q[KSV_MAX_OPERANDS];
+    int64_t mat;
+} ksv_op;
+
+/* Gradient bookkeeping for ksv_adjoint: op `op` of the tape depends on
+ * parameter `param` with d(op matrix)/d(param) = coeff * dmat, where `dmat`
+ * is an offset (doubles) into the matrix array holding the derivative of the
+ * op's matrix with respect to its own angle.  Several records may name the
+ * same op (u3) or the same parameter (qaoa's gamma on every edge). */
+typedef struct ksv_param {
+    int32_t op;
+    int32_t param;
+    double  coeff;
+    int64_t dmat;
+} ksv_param;
+
+typedef struct ksv_state   ksv_state;
+typedef struct ksv_program ksv_program;
+
+/* Options for ksv_set_option. */
+#define KSV_OPT_PRUNE        0  /* 1: drop |a|^2 < 1e-14 after every gate like ket's accumulate
+                                   (src/statevector.ts:19-27, src/complex.ts:24); 0: exact dense  */
+#define KSV_OPT_TILE_QUBITS  1  /* qubits held per LDS tile in fused passes (0 = autotuned)      */
+#define KSV_OPT_TIMING       2  /* 1: time every pass launch with HIP events on the state stream  */
+#define KSV_OPT_FUSE         3  /* 1: multiply runs of gates on the same qubits into one matrix
+                                   (exact mode only; ket's per-gate pruning forbids it)            */
+
+/* Per-state launch statistics (KSV_OPT_TIMING). */
+typedef struct ksv_stats {
+    uint64_t pass_launches;     /* fused LDS pass kernels launched          */
+    double   pass_ms;           /* summed HIP-event time of those launches  */
+    double   pass_bytes;        /* algorithmic HBM bytes of those launches  */
+    uint64_t other_launches;    /* every other kernel                       */
+    double   other_ms;
+    double   other_bytes;
+    uint64_t gates_applied;     /* tape ops applied                         */
+} ksv_stats;
+
+/* Program summary (ksv_program_info). */
+typedef struct ksv_program_summary {
+    uint64_t ops;               /* tape ops                                 */
+    uint64_t passes;            /* HBM passes the schedule makes            */
+    uint64_t device_ops;        /* ops after fusion                         */
+    int32_t  tile_qubits;
+    int32_t  reserved;
+} ksv_program_summary;
+
+/* ---- library ------------------------------------------------------------ */
+int ksv_last_error(char* buf, size_t len);
+int ksv_device_count(int* count);
+
+/* ---- state lifetime ------------------------------------------------------ *
+ * Replaces `zero(n)` / `svFromBitstring` (src/statevector.ts:16,
+ * src/circuit.ts:136-140): ksv_create allocates 2^n amplitudes in HBM and
+ * sets |0...0>; ksv_init_basis sets the basis state |index>.            */
+int ksv_create(int n, int precision, ksv_state** out);
+int ksv_destroy(ksv_state* s);
+int ksv_set_option(ksv_state* s, int option, double value);
+int ksv_init_basis(ksv_state* s, uint64_t index);
+int ksv_copy_state(ksv_state* dst, const ksv_state* src);
+
+/* ---- gate application ------------------------------------------------------ *
+ * Replaces simulatePure's dispatch loop (src/circuit.ts:143-161) and the
+ * apply* functions (src/statevector.ts:47-270).                        */
+int ksv_apply(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats);
+int ksv_program_create(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats,
+                       size_t nmats, ksv_program** out);
+int ksv_program_run(ksv_state* s, ksv_program* p);
+int ksv_program_info(const ksv_program* p, ksv_program_summary* out);
+int ksv_program_destroy(ksv_program* p);
+
+/* ---- read-out ---------------------------------------------------------------- */
+/* Dense amplitudes [offset, offset+count) as interleaved doubles (statevector(),
+ * src/circuit.ts:1431-1441). */
+int ksv_get_amplitudes(ksv_state* s, uint64_t offset, uint64_t count, double* out);
+int ksv_set_amplitudes(ksv_state* s, uint64_t offset, uint64_t count, const double* in);
+/* One amplitude (amplitude(), src/circuit.ts:1491-1495). */
+int ksv_amplitude(ksv_state* s, uint64_t index, double out[2]);
+/* sum |a|^2 in FP64 */
+int ksv_norm2(ksv_state* s, double* out);
+/* All (index, p) with p > thresh in ascending index (probabilities(),
+ * src/statevector.ts:273-280; exactProbs, src/circuit.ts:3700-3711).
+ * Writes min(count, cap) entries; *count receives the full count. */
+int ksv_probs_filtered(ksv_state* s, double thresh, uint64_t* idx, double* p, size_t cap,
+                       size_t* count);
+/* The k most probable indices, p descending, ties by ascending index. */
+int ksv_topk(ksv_state* s, size_t k, uint64_t* idx, double* p);
+/* P(qubit q = 1) for every q (marginals(), src/circuit.ts:1508-1518). */
+int ksv_marginals(ksv_state* s, double* out);
+/* rho00, rho11, Re rho01, Im rho01 of qubit q (blochAngles(), src/circuit.ts:4102-4129). */
+int ksv_reduce_1q(ksv_state* s, int q, double out[4]);
+/* Shot sampling of run()'s fast path (src/circuit.ts:3425-3450): probabilities
+ * p > 1e-14 in ascending index order, inclusive CDF with its last entry
+ * forced to 1.0, and for each r the first index with cdf >= r.  The host
+ * supplies the xorshift32 draws r (src/circuit.ts:813-819) in ket's order. */
+int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out);
+
+/* Pauli expectations.  Term t is coeff[t] * prod_q P_q with
+ * P_q = X if xmask bit q only, Z if zmask bit q only, Y if both.
+ * mode 0: exact <psi|P|psi>.
+ * mode 1: ket's method (Circuit.expectation src/circuit.ts:3728-3753,
+ *         vqe src/algorithms.ts:301-336): rotate a copy (X->H, Y->Rx(pi/2)),
+ *         keep p > 1e-14, sum +-p by parity; identity terms give coeff.
+ * out[t] = coeff[t] * <P_t>;  *total (may be NULL) = sum_t out[t]. */
+int ksv_expect_pauli(ksv_state* s, const uint64_t* xmask, const uint64_t* zmask,
+                     const double* coeff, size_t nterms, int mode, double* out, double* total);
+
+/* Adjoint differentiation of E = <psi|H|psi>, psi = tape|init>, H = sum of
+ * Pauli terms, w.r.t. the parameters named in `params` (replaces the
+ * parameter-shift loop of gradient(), src/algorithms.ts:536-547).
+ * `s` holds the initial state on entry; the forward pass runs on it and the
+ * backward sweep un-computes it, so it holds the initial state again on
+ * return (up to rounding).  Runs in exact mode (no pruning).
+ * grad[0..nparam) receives dE/dparam. */
+int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                const ksv_param* params, size_t nrec, size_t nparam,
+                const uint64_t* xmask, const uint64_t* zmask, const double* coeff, size_t nterms,
+                double* grad, double* energy);
+
+/* ---- measurement ---------------------------------------------------------------- */
+/* collapseQubit (src/circuit.ts:55-70): outcome = r < p1; project and scale
+ * by 1/sqrt(p_outcome) in FP64. */
+int ksv_collapse(ksv_state* s, int q, double r, int* outcome);
+
+/* ---- instrumentation -------------------------------------------------------------- */
+int ksv_get_stats(ksv_state* s, ksv_stats* out);
+int ksv_reset_stats(ksv_state* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KETSV_H */

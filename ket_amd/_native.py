@@ -1,0 +1,295 @@
+"""ctypes binding of libketsv.so (the C ABI in include/ketsv.h).
+
+The library is the only compute path: if it is missing, importing a
+simulation entry point raises at once (there is no CPU fallback).
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_size_t, c_uint64, c_void_p
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libketsv.so")
+
+KSV_OK, KSV_ERR_RANGE, KSV_ERR_TYPE, KSV_ERR_DEVICE, KSV_ERR_HANDLE = 0, 1, 2, 3, 4
+KSV_F64, KSV_F32 = 0, 1
+(KSV_SINGLE, KSV_CNOT, KSV_SWAP, KSV_TWO, KSV_CONTROLLED, KSV_TOFFOLI, KSV_CSWAP, KSV_CSRSWAP,
+ KSV_UNITARY, KSV_BARRIER) = range(10)
+KSV_MAX_OPERANDS = 12
+KSV_OPT_PRUNE, KSV_OPT_TILE_QUBITS, KSV_OPT_TIMING, KSV_OPT_FUSE = 0, 1, 2, 3
+
+OP_DTYPE = np.dtype([("kind", "<i4"), ("nq", "<i4"), ("q", "<i4", (KSV_MAX_OPERANDS,)), ("mat", "<i8")])
+PARAM_DTYPE = np.dtype([("op", "<i4"), ("param", "<i4"), ("coeff", "<f8"), ("dmat", "<i8")])
+assert OP_DTYPE.itemsize == 64 and PARAM_DTYPE.itemsize == 24
+
+
+class RangeError(ValueError):
+    """ket's RangeError (qubit index, size or capacity out of range)."""
+
+
+class KsvStats(ctypes.Structure):
+    _fields_ = [("pass_launches", c_uint64), ("pass_ms", c_double), ("pass_bytes", c_double),
+                ("other_launches", c_uint64), ("other_ms", c_double), ("other_bytes", c_double),
+                ("gates_applied", c_uint64)]
+
+
+class KsvProgramSummary(ctypes.Structure):
+    _fields_ = [("ops", c_uint64), ("passes", c_uint64), ("device_ops", c_uint64),
+                ("tile_qubits", c_int32), ("reserved", c_int32)]
+
+
+# (name, restype, argtypes) of every entry point in include/ketsv.h
+_P = c_void_p
+_DP = POINTER(c_double)
+_U64P = POINTER(c_uint64)
+SIGNATURES = [
+    ("ksv_last_error", c_int, [ctypes.c_char_p, c_size_t]),
+    ("ksv_device_count", c_int, [POINTER(c_int)]),
+    ("ksv_create", c_int, [c_int, c_int, POINTER(c_void_p)]),
+    ("ksv_destroy", c_int, [_P]),
+    ("ksv_set_option", c_int, [_P, c_int, c_double]),
+    ("ksv_init_basis", c_int, [_P, c_uint64]),
+    ("ksv_copy_state", c_int, [_P, _P]),
+    ("ksv_apply", c_int, [_P, _P, c_size_t, _P, c_size_t]),
+    ("ksv_program_create", c_int, [_P, _P, c_size_t, _P, c_size_t, POINTER(c_void_p)]),
+    ("ksv_program_run", c_int, [_P, _P]),
+    ("ksv_program_info", c_int, [_P, POINTER(KsvProgramSummary)]),
+    ("ksv_program_destroy", c_int, [_P]),
+    ("ksv_get_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
+    ("ksv_set_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
+    ("ksv_amplitude", c_int, [_P, c_uint64, _P]),
+    ("ksv_norm2", c_int, [_P, _DP]),
+    ("ksv_probs_filtered", c_int, [_P, c_double, _P, _P, c_size_t, POINTER(c_size_t)]),
+    ("ksv_topk", c_int, [_P, c_size_t, _P, _P]),
+    ("ksv_marginals", c_int, [_P, _P]),
+    ("ksv_reduce_1q", c_int, [_P, c_int, _P]),
+    ("ksv_sample", c_int, [_P, _P, c_size_t, _P]),
+    ("ksv_expect_pauli", c_int, [_P, _P, _P, _P, c_size_t, c_int, _P, _DP]),
+    ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
+    ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
+    ("ksv_get_stats", c_int, [_P, POINTER(KsvStats)]),
+    ("ksv_reset_stats", c_int, [_P]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libketsv.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"ket_amd: HIP library {LIB_PATH} is not built; run __graft_entry__.build() "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    lib().ksv_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc):
+    """Map a status code to ket's error classes."""
+    if rc == KSV_OK:
+        return
+    msg = last_error()
+    if rc == KSV_ERR_RANGE:
+        raise RangeError(msg)
+    if rc == KSV_ERR_TYPE:
+        raise TypeError(msg)
+    raise RuntimeError(f"libketsv: {msg}")
+
+
+def device_count():
+    c = c_int(0)
+    lib().ksv_device_count(ctypes.byref(c))
+    return c.value
+
+
+def ptr(a):
+    """Data pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return c_void_p(a.ctypes.data)
+
+
+class State:
+    """Owning handle of one device statevector (2^n amplitudes in HBM)."""
+
+    def __init__(self, n, precision="f64", prune=True, tile=0, fuse=False, timing=False):
+        self.n = int(n)
+        self.precision = precision
+        prec = KSV_F64 if precision == "f64" else KSV_F32
+        h = c_void_p()
+        check(lib().ksv_create(self.n, prec, ctypes.byref(h)))
+        self._h = h
+        self.set_option(KSV_OPT_PRUNE, 1 if prune else 0)
+        if tile:
+            self.set_option(KSV_OPT_TILE_QUBITS, tile)
+        if fuse:
+            self.set_option(KSV_OPT_FUSE, 1)
+        if timing:
+            self.set_option(KSV_OPT_TIMING, 1)
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("state has been freed")
+        return self._h
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ksv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_option(self, opt, value):
+        check(lib().ksv_set_option(self.handle, opt, float(value)))
+
+    def init_basis(self, index=0):
+        check(lib().ksv_init_basis(self.handle, int(index)))
+
+    def apply(self, tape):
+        ops, mats = tape
+        check(lib().ksv_apply(self.handle, ptr(ops), len(ops), ptr(mats), len(mats)))
+
+    def program(self, tape):
+        return Program(self, tape)
+
+    def amplitudes(self, offset=0, count=None):
+        if count is None:
+            count = (1 << self.n) - offset
+        out = np.empty(2 * count, dtype=np.float64)
+        check(lib().ksv_get_amplitudes(self.handle, offset, count, ptr(out)))
+        return out.view(np.complex128)
+
+    def set_amplitudes(self, values, offset=0):
+        v = np.ascontiguousarray(values, dtype=np.complex128)
+        check(lib().ksv_set_amplitudes(self.handle, offset, len(v), ptr(v.view(np.float64))))
+
+    def amplitude(self, index):
+        out = np.empty(2, dtype=np.float64)
+        check(lib().ksv_amplitude(self.handle, int(index), ptr(out)))
+        return complex(out[0], out[1])
+
+    def norm2(self):
+        v = c_double()
+        check(lib().ksv_norm2(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def probs_filtered(self, thresh=1e-14, cap=None):
+        if cap is None:
+            cnt = c_size_t()
+            check(lib().ksv_probs_filtered(self.handle, thresh, None, None, 0, ctypes.byref(cnt)))
+            cap = cnt.value
+        idx = np.empty(max(cap, 1), dtype=np.uint64)
+        p = np.empty(max(cap, 1), dtype=np.float64)
+        cnt = c_size_t()
+        check(lib().ksv_probs_filtered(self.handle, thresh, ptr(idx), ptr(p), cap, ctypes.byref(cnt)))
+        m = min(cnt.value, cap)
+        return idx[:m], p[:m]
+
+    def topk(self, k):
+        idx = np.empty(k, dtype=np.uint64)
+        p = np.empty(k, dtype=np.float64)
+        check(lib().ksv_topk(self.handle, k, ptr(idx), ptr(p)))
+        return idx, p
+
+    def marginals(self):
+        out = np.empty(self.n, dtype=np.float64)
+        check(lib().ksv_marginals(self.handle, ptr(out)))
+        return out
+
+    def reduce_1q(self, q):
+        out = np.empty(4, dtype=np.float64)
+        check(lib().ksv_reduce_1q(self.handle, int(q), ptr(out)))
+        return out
+
+    def sample(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.empty(len(r), dtype=np.uint64)
+        check(lib().ksv_sample(self.handle, ptr(r), len(r), ptr(out)))
+        return out
+
+    def expect_pauli(self, xmask, zmask, coeff, mode=0):
+        x = np.ascontiguousarray(xmask, dtype=np.uint64)
+        z = np.ascontiguousarray(zmask, dtype=np.uint64)
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        out = np.empty(max(len(c), 1), dtype=np.float64)
+        tot = c_double()
+        check(lib().ksv_expect_pauli(self.handle, ptr(x), ptr(z), ptr(c), len(c), int(mode), ptr(out),
+                                     ctypes.byref(tot)))
+        return out[:len(c)], tot.value
+
+    def adjoint(self, tape, records, nparam, xmask, zmask, coeff):
+        ops, mats = tape
+        rec = np.ascontiguousarray(records, dtype=PARAM_DTYPE)
+        x = np.ascontiguousarray(xmask, dtype=np.uint64)
+        z = np.ascontiguousarray(zmask, dtype=np.uint64)
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        grad = np.zeros(max(nparam, 1), dtype=np.float64)
+        e = c_double()
+        check(lib().ksv_adjoint(self.handle, ptr(ops), len(ops), ptr(mats), len(mats), ptr(rec), len(rec), nparam,
+                                ptr(x), ptr(z), ptr(c), len(c), ptr(grad), ctypes.byref(e)))
+        return grad[:nparam], e.value
+
+    def collapse(self, q, r):
+        o = c_int()
+        check(lib().ksv_collapse(self.handle, int(q), float(r), ctypes.byref(o)))
+        return o.value
+
+    def copy_from(self, other):
+        check(lib().ksv_copy_state(self.handle, other.handle))
+
+    def stats(self):
+        s = KsvStats()
+        check(lib().ksv_get_stats(self.handle, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in KsvStats._fields_}
+
+    def reset_stats(self):
+        check(lib().ksv_reset_stats(self.handle))
+
+
+class Program:
+    """A scheduled tape resident on the device (ksv_program_*)."""
+
+    def __init__(self, state, tape):
+        ops, mats = tape
+        h = c_void_p()
+        check(lib().ksv_program_create(state.handle, ptr(ops), len(ops), ptr(mats), len(mats), ctypes.byref(h)))
+        self._h = h
+        self.state = state
+
+    def run(self, state=None):
+        st = state or self.state
+        check(lib().ksv_program_run(st.handle, self._h))
+
+    def info(self):
+        s = KsvProgramSummary()
+        check(lib().ksv_program_info(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in KsvProgramSummary._fields_}
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ksv_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

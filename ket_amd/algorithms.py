@@ -1,0 +1,305 @@
+"""Algorithm library on the hot path (src/algorithms.ts): QFT, QAOA / Max-Cut,
+hardware-efficient ansätze, and the VQE energy, gradient and minimiser.
+
+vqe() simulates the ansatz once on the GPU and reads every Pauli term off
+the device state; ket re-simulates the circuit once per term
+(src/algorithms.ts:301-336).  gradient() computes the adjoint gradient on the
+GPU when ket's parameter-shift rule is exact for the ansatz (then the two
+agree to rounding), and otherwise runs ket's parameter-shift formula on the
+GPU so the numbers stay ket's; method='adjoint' / 'psr' force either.
+"""
+import math
+
+import numpy as np
+
+from . import gates as G
+from .circuit import OPTIONS, Circuit, Sym, _TraceCtx, _flatten, pauli_masks
+from ._native import PARAM_DTYPE, RangeError, State
+from .tape import pack
+
+
+# ---- QFT (src/algorithms.ts:16-39) --------------------------------------------
+def qft(n):
+    c = Circuit(n)
+    for j in range(n - 1, -1, -1):
+        c = c.h(j)
+        for k in range(j - 1, -1, -1):
+            c = c.cu1(math.pi / 2 ** (j - k), k, j)
+    for i in range(n // 2):
+        c = c.swap(i, n - 1 - i)
+    return c
+
+
+def iqft(n):
+    c = Circuit(n)
+    for i in range(n // 2):
+        c = c.swap(i, n - 1 - i)
+    for j in range(n):
+        for k in range(j - 1, -1, -1):
+            c = c.cu1(-math.pi / 2 ** (j - k), k, j)
+        c = c.h(j)
+    return c
+
+
+# ---- QAOA / Max-Cut (src/algorithms.ts:231-276) ----------------------------------
+def maxCutHamiltonian(n, edges):
+    if len(edges) == 0:
+        return []
+    terms = [{"coeff": len(edges) / 2, "ops": "I" * n}]
+    for u, v in edges:
+        arr = ["I"] * n
+        arr[n - 1 - u] = "Z"
+        arr[n - 1 - v] = "Z"
+        terms.append({"coeff": -0.5, "ops": "".join(arr)})
+    return terms
+
+
+def qaoa(n, edges, gamma, beta):
+    if len(gamma) != len(beta):
+        raise TypeError(f"gamma and beta must have equal length (got {len(gamma)} vs {len(beta)})")
+    c = Circuit(n)
+    for q in range(n):
+        c = c.h(q)
+    for l in range(len(gamma)):
+        for u, v in edges:
+            c = c.cnot(u, v).rz(-gamma[l], v).cnot(u, v)
+        for q in range(n):
+            c = c.rx(2 * beta[l], q)
+    return c
+
+
+# ---- ansätze (src/algorithms.ts:362-399) ----------------------------------------
+class AnsatzFn:
+    """An ansatz function with its parameter count (src/algorithms.ts:340-348)."""
+
+    def __init__(self, param_count, fn):
+        self.paramCount = param_count
+        self._fn = fn
+
+    def __call__(self, params):
+        return self._fn(params)
+
+
+def realAmplitudes(n, reps=3):
+    count = n * (reps + 1)
+
+    def build(params):
+        if len(params) != count:
+            raise RangeError(f"realAmplitudes({n}, {reps}) needs {count} params, got {len(params)}")
+        c, p = Circuit(n), 0
+        for _ in range(reps):
+            for q in range(n):
+                c = c.ry(params[p], q)
+                p += 1
+            for q in range(n - 1):
+                c = c.cnot(q, q + 1)
+        for q in range(n):
+            c = c.ry(params[p], q)
+            p += 1
+        return c
+
+    return AnsatzFn(count, build)
+
+
+def efficientSU2(n, reps=3):
+    count = 2 * n * (reps + 1)
+
+    def build(params):
+        if len(params) != count:
+            raise RangeError(f"efficientSU2({n}, {reps}) needs {count} params, got {len(params)}")
+        c, p = Circuit(n), 0
+        for _ in range(reps):
+            for q in range(n):
+                c = c.ry(params[p], q)
+                c = c.rz(params[p + 1], q)
+                p += 2
+            for q in range(n - 1):
+                c = c.cnot(q, q + 1)
+        for q in range(n):
+            c = c.ry(params[p], q)
+            c = c.rz(params[p + 1], q)
+            p += 2
+        return c
+
+    return AnsatzFn(count, build)
+
+
+# ---- Hamiltonians -----------------------------------------------------------------
+def _terms(hamiltonian, n):
+    """PauliTerms -> (xmask, zmask, coeff) arrays; ops[0] acts on qubit n-1."""
+    xs, zs, cs = [], [], []
+    for t in hamiltonian:
+        coeff = t["coeff"] if isinstance(t, dict) else t.coeff
+        ops = t["ops"] if isinstance(t, dict) else t.ops
+        if len(ops) != n:
+            raise TypeError(f"ops '{ops}' length must equal ansatz.qubits ({n})")
+        up = ops.upper()
+        if any(ch not in "IXYZ" for ch in up):
+            raise TypeError(f"ops '{ops}' must contain only I, X, Y, Z")
+        x, z = pauli_masks(up, q0_left=False)
+        xs.append(x)
+        zs.append(z)
+        cs.append(float(coeff))
+    return np.array(xs, dtype=np.uint64), np.array(zs, dtype=np.uint64), np.array(cs, dtype=np.float64)
+
+
+def vqe(ansatz, hamiltonian):
+    """<ansatz|H|ansatz> (src/algorithms.ts:301-336)."""
+    n = ansatz.qubits
+    x, z, c = _terms(hamiltonian, n)
+    if len(c) == 0:
+        return 0.0
+    st = ansatz.simulate()
+    _, tot = st.expect_pauli(x, z, c, mode=1 if OPTIONS["prune"] else 0)
+    st.free()
+    return tot
+
+
+# ---- gradients --------------------------------------------------------------------
+# Parametrised gates whose angles enter as exp(-i theta P / 2)-like factors with
+# unit frequency: ket's two-point shift by pi/2 is exact for them.
+_PSR_EXACT = {"rx", "ry", "rz", "vz", "u1", "p", "u2", "u3", "xx", "yy", "zz", "cu1"}
+
+
+def _dmat(name, vals, which):
+    """Derivative of a gate's 2x2 / 4x4 matrix w.r.t. its `which`-th angle."""
+    if name in ("rx", "crx"):
+        return G.dRx(vals[0])
+    if name in ("ry", "cry"):
+        return G.dRy(vals[0])
+    if name in ("rz", "vz", "crz"):
+        return G.dRz(vals[0])
+    if name in ("u1", "p", "cu1"):
+        return G.dU1(vals[0])
+    if name in ("u2", "cu2"):
+        return G.dU2(vals[0], vals[1], which)
+    if name in ("u3", "cu3"):
+        return G.dU3(vals[0], vals[1], vals[2], which)
+    if name == "gpi":
+        return G.dGpi(vals[0])
+    if name == "gpi2":
+        return G.dGpi2(vals[0])
+    if name == "xx":
+        return G.dXx(vals[0])
+    if name == "yy":
+        return G.dYy(vals[0])
+    if name == "zz":
+        return G.dZz(vals[0])
+    if name == "xy":
+        return G.dXy(vals[0])
+    if name == "ms":
+        return G.dMs(vals[0], vals[1], which)
+    return None
+
+
+def trace(ansatz, params):
+    """Call `ansatz` with symbolic parameters.
+
+    Returns (flat_ops, records, psr_exact) where records lists
+    (op index, param index, coeff, derivative matrix), or None when the
+    ansatz does something other than affine arithmetic with its parameters.
+    """
+    ctx = _TraceCtx()
+    syms = [Sym({i: 1.0}, 0.0, float(v), ctx) for i, v in enumerate(params)]
+    try:
+        circ = ansatz(syms)
+    except (TypeError, ValueError, AttributeError):
+        return None
+    if ctx.escaped or not isinstance(circ, Circuit):
+        return None
+    flat = _flatten(circ._ops())
+    records = []
+    uses = {}
+    exact = True
+    for i, op in enumerate(flat):
+        meta = op[-1] if op[0] in ("single", "two", "controlled") else None
+        if not meta or not isinstance(meta, tuple):
+            continue
+        name, ps = meta
+        vals = [p.value if isinstance(p, Sym) else float(p) for p in ps]
+        for which, p in enumerate(ps):
+            if not isinstance(p, Sym):
+                continue
+            for pi, coeff in p.terms.items():
+                if coeff == 0:
+                    continue
+                d = _dmat(name, vals, which)
+                if d is None:
+                    return None
+                records.append((i, pi, coeff, d))
+                uses[pi] = uses.get(pi, 0) + 1
+                if name not in _PSR_EXACT or abs(coeff) != 1.0:
+                    exact = False
+    if any(v != 1 for v in uses.values()):
+        exact = False
+    return flat, records, exact, circ.qubits
+
+
+def _adjoint(flat, records, nparam, n, hamiltonian):
+    ops, mats = pack(flat)
+    mats = list(mats)
+    rec = np.zeros(len(records), dtype=PARAM_DTYPE)
+    for j, (oi, pi, coeff, d) in enumerate(records):
+        rec[j] = (oi, pi, coeff, len(mats))
+        for v in d:
+            mats.append(complex(v).real)
+            mats.append(complex(v).imag)
+    mats = np.asarray(mats, dtype=np.float64)
+    x, z, c = _terms(hamiltonian, n)
+    st = State(n, OPTIONS["precision"], prune=False, tile=OPTIONS["tile"])
+    grad, energy = st.adjoint((ops, mats), rec, nparam, x, z, c)
+    st.free()
+    return grad, energy
+
+
+def gradient(ansatz, hamiltonian, params, method="auto"):
+    """dE/dparams (src/algorithms.ts:536-547).
+
+    method: 'auto' (adjoint when ket's shift rule is exact, else ket's shift
+    rule), 'adjoint' (exact gradient by adjoint differentiation) or 'psr'
+    (ket's parameter-shift formula, each energy simulated on the GPU).
+    """
+    params = [float(p) for p in params]
+    if method not in ("auto", "adjoint", "psr"):
+        raise TypeError(f"unknown gradient method '{method}'")
+    if method != "psr":
+        tr = trace(ansatz, params)
+        if tr is not None:
+            flat, records, exact, n = tr
+            if method == "adjoint" or exact:
+                grad, _ = _adjoint(flat, records, len(params), n, hamiltonian)
+                return [float(g) for g in grad]
+        if method == "adjoint":
+            raise TypeError("adjoint gradient needs an ansatz built from affine functions of its parameters")
+    shift = math.pi / 2
+    out = []
+    for i in range(len(params)):
+        plus = [p + shift if j == i else p for j, p in enumerate(params)]
+        minus = [p - shift if j == i else p for j, p in enumerate(params)]
+        out.append(0.5 * (vqe(ansatz(plus), hamiltonian) - vqe(ansatz(minus), hamiltonian)))
+    return out
+
+
+def energy_and_gradient(ansatz, hamiltonian, params):
+    """(E, dE/dparams) from one adjoint sweep (exact, no pruning)."""
+    params = [float(p) for p in params]
+    tr = trace(ansatz, params)
+    if tr is None:
+        raise TypeError("adjoint gradient needs an ansatz built from affine functions of its parameters")
+    flat, records, _, n = tr
+    grad, e = _adjoint(flat, records, len(params), n, hamiltonian)
+    return float(e), [float(g) for g in grad]
+
+
+def minimize(ansatz, hamiltonian, initialParams, lr=0.1, steps=200, tol=1e-6):
+    """Gradient descent (src/algorithms.ts:591-609)."""
+    params = [float(p) for p in initialParams]
+    for step in range(steps):
+        grad = gradient(ansatz, hamiltonian, params)
+        gnorm = math.sqrt(sum(g * g for g in grad))
+        if gnorm < tol:
+            return {"params": params, "energy": vqe(ansatz(params), hamiltonian), "steps": step,
+                    "converged": True, "truncated": False}
+        params = [p - lr * g for p, g in zip(params, grad)]
+    return {"params": params, "energy": vqe(ansatz(params), hamiltonian), "steps": steps, "converged": False,
+            "truncated": False}

@@ -1,0 +1,642 @@
+// C ABI of libketsv: state handles, programs, read-out and measurement.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "ksv_internal.h"
+
+namespace ksv {
+hipError_t chunk_counts_pred(int n, int prec, const void* psi, int mode, uint64_t kv, uint32_t* d_cnt,
+                             hipStream_t st);
+hipError_t chunk_compact_pred(int n, int prec, const void* psi, int mode, uint64_t kv, const uint64_t* d_off,
+                              uint64_t* d_idx, double* d_p, uint64_t cap, hipStream_t st);
+}  // namespace ksv
+
+using namespace ksv;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(KSV_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int NBLK = 1024;          // reduction grid
+constexpr int MAXN = 34;            // 2^34 complex64 = 128 GiB
+
+struct ksv_state {
+    int n = 0;
+    int prec = KSV_F64;
+    int dev = 0;
+    hipStream_t st = nullptr;
+    void* psi = nullptr;
+    void* psi2 = nullptr;           // second state buffer, allocated on first use
+    double* d_part = nullptr;       // NBLK x 64 partials
+    double* d_small = nullptr;      // 64 results
+    int prune = 1;
+    int tile = 0;                   // 0 = default
+    int timing = 0;
+    int fuse = 0;
+    ksv_stats stats;
+    std::vector<hipEvent_t> ev;
+};
+
+struct ksv_program : public Program {};
+
+static size_t amp_bytes(const ksv_state* s) { return s->prec == KSV_F64 ? 16 : 8; }
+static size_t state_bytes(const ksv_state* s) { return amp_bytes(s) << s->n; }
+
+static int default_tile(const ksv_state* s) {
+    if (s->tile > 0) return s->tile;
+    return s->prec == KSV_F64 ? 12 : 13;
+}
+
+extern "C" {
+
+int ksv_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        size_t m = std::min(len - 1, g_err.size());
+        std::memcpy(buf, g_err.data(), m);
+        buf[m] = 0;
+    }
+    return (int)g_err.size();
+}
+
+int ksv_device_count(int* count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    if (count) *count = c;
+    return KSV_OK;
+}
+
+int ksv_create(int n, int precision, ksv_state** out) {
+    if (!out) return fail(KSV_ERR_TYPE, "ksv_create: null output pointer");
+    *out = nullptr;
+    if (n < 1 || n > MAXN) return fail(KSV_ERR_RANGE, "qubit count " + std::to_string(n) + " is out of range (1.." + std::to_string(MAXN) + ")");
+    if (precision != KSV_F64 && precision != KSV_F32) return fail(KSV_ERR_TYPE, "unknown precision");
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1) return fail(KSV_ERR_DEVICE, "no HIP device available");
+    std::unique_ptr<ksv_state> s(new ksv_state());
+    s->n = n;
+    s->prec = precision;
+    std::memset(&s->stats, 0, sizeof s->stats);
+    HIPCHK(hipGetDevice(&s->dev));
+    HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    hipError_t e = hipMalloc(&s->psi, state_bytes(s.get()));
+    if (e != hipSuccess) {
+        hipStreamDestroy(s->st);
+        return fail(KSV_ERR_DEVICE, "out of device memory allocating a " + std::to_string(n) + "-qubit state");
+    }
+    HIPCHK(hipMalloc(&s->d_part, sizeof(double) * NBLK * 64));
+    HIPCHK(hipMalloc(&s->d_small, sizeof(double) * 64));
+    HIPCHK(k_init_basis(n, precision, s->psi, 0, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    *out = s.release();
+    return KSV_OK;
+}
+
+int ksv_destroy(ksv_state* s) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    hipStreamSynchronize(s->st);
+    hipFree(s->psi);
+    if (s->psi2) hipFree(s->psi2);
+    hipFree(s->d_part);
+    hipFree(s->d_small);
+    for (auto e : s->ev) hipEventDestroy(e);
+    hipStreamDestroy(s->st);
+    delete s;
+    return KSV_OK;
+}
+
+int ksv_set_option(ksv_state* s, int option, double value) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    switch (option) {
+    case KSV_OPT_PRUNE: s->prune = value != 0; return KSV_OK;
+    case KSV_OPT_TILE_QUBITS: {
+        int t = (int)value;
+        int tmax = s->prec == KSV_F64 ? KMAX_TILE : KMAX_TILE + 1;
+        if (t != 0 && (t < 8 || t > tmax)) return fail(KSV_ERR_RANGE, "tile qubits must be 0 or 8.." + std::to_string(tmax));
+        s->tile = t;
+        return KSV_OK;
+    }
+    case KSV_OPT_TIMING: s->timing = value != 0; return KSV_OK;
+    case KSV_OPT_FUSE: s->fuse = value != 0; return KSV_OK;
+    default: return fail(KSV_ERR_TYPE, "unknown option " + std::to_string(option));
+    }
+}
+
+int ksv_init_basis(ksv_state* s, uint64_t index) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (index >> s->n) return fail(KSV_ERR_RANGE, "basis index out of range");
+    HIPCHK(k_init_basis(s->n, s->prec, s->psi, index, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+int ksv_copy_state(ksv_state* dst, const ksv_state* src) {
+    if (!dst || !src) return fail(KSV_ERR_HANDLE, "null state");
+    if (dst->n != src->n || dst->prec != src->prec) return fail(KSV_ERR_TYPE, "state shapes differ");
+    HIPCHK(hipStreamSynchronize(src->st));
+    HIPCHK(hipMemcpyAsync(dst->psi, src->psi, state_bytes(dst), hipMemcpyDeviceToDevice, dst->st));
+    HIPCHK(hipStreamSynchronize(dst->st));
+    return KSV_OK;
+}
+
+// --------------------------------------------------------------------------
+// programs
+// --------------------------------------------------------------------------
+static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                         int prune, bool fuse, ksv_program** out) {
+    if (nops && !ops) return fail(KSV_ERR_TYPE, "null op tape");
+    std::vector<LOp> lops;
+    std::string err;
+    int rc = lower_tape(s->n, ops, nops, mats, nmats, lops, err);
+    if (rc) return fail(rc, err);
+    std::unique_ptr<ksv_program> p(new ksv_program());
+    p->n = s->n;
+    p->precision = s->prec;
+    p->prune = prune;
+    p->tile = default_tile(s);
+    p->tape_ops = nops;
+    schedule(s->n, p->tile, fuse && !prune, lops, *p);
+    p->dev = s->dev;
+    if (!p->dops.empty()) {
+        HIPCHK(hipMalloc(&p->d_ops, sizeof(DOp) * p->dops.size()));
+        HIPCHK(hipMemcpy(p->d_ops, p->dops.data(), sizeof(DOp) * p->dops.size(), hipMemcpyHostToDevice));
+    }
+    if (!p->mats.empty()) {
+        if (s->prec == KSV_F64) {
+            HIPCHK(hipMalloc(&p->d_mats, 16 * p->mats.size()));
+            HIPCHK(hipMemcpy(p->d_mats, p->mats.data(), 16 * p->mats.size(), hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(2 * p->mats.size());
+            for (size_t i = 0; i < p->mats.size(); i++) {
+                f[2 * i] = (float)p->mats[i].real();
+                f[2 * i + 1] = (float)p->mats[i].imag();
+            }
+            HIPCHK(hipMalloc(&p->d_mats, 8 * p->mats.size()));
+            HIPCHK(hipMemcpy(p->d_mats, f.data(), 8 * p->mats.size(), hipMemcpyHostToDevice));
+        }
+    }
+    *out = p.release();
+    return KSV_OK;
+}
+
+static int run_program_on(ksv_state* s, ksv_program* p, void* psi) {
+    const bool timing = s->timing != 0;
+    const size_t nst = p->steps.size();
+    if (timing) {
+        while (s->ev.size() < nst + 1) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            s->ev.push_back(e);
+        }
+        HIPCHK(hipEventRecord(s->ev[0], s->st));
+    }
+    for (size_t i = 0; i < nst; i++) {
+        const Step& st = p->steps[i];
+        if (st.big) HIPCHK(launch_big(*p, st, psi, p->prune, s->st));
+        else HIPCHK(launch_pass(*p, st, psi, s->st));
+        if (timing) HIPCHK(hipEventRecord(s->ev[i + 1], s->st));
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    const double bytes = 2.0 * (double)state_bytes(s);
+    for (size_t i = 0; i < nst; i++) {
+        const Step& st = p->steps[i];
+        float ms = 0;
+        if (timing) HIPCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+        if (st.big) {
+            s->stats.other_launches++;
+            s->stats.other_ms += ms;
+            s->stats.other_bytes += bytes;
+        } else {
+            s->stats.pass_launches++;
+            s->stats.pass_ms += ms;
+            s->stats.pass_bytes += bytes;
+        }
+    }
+    s->stats.gates_applied += p->tape_ops;
+    return KSV_OK;
+}
+
+int ksv_program_create(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                       ksv_program** out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (!out) return fail(KSV_ERR_TYPE, "null output pointer");
+    return build_program(s, ops, nops, mats, nmats, s->prune, s->fuse != 0, out);
+}
+
+int ksv_program_run(ksv_state* s, ksv_program* p) {
+    if (!s || !p) return fail(KSV_ERR_HANDLE, "null state or program");
+    if (p->n != s->n || p->precision != s->prec) return fail(KSV_ERR_TYPE, "program was built for another state shape");
+    return run_program_on(s, p, s->psi);
+}
+
+int ksv_program_info(const ksv_program* p, ksv_program_summary* out) {
+    if (!p || !out) return fail(KSV_ERR_HANDLE, "null program");
+    out->ops = p->tape_ops;
+    out->passes = p->steps.size();
+    out->device_ops = p->dops.size() + p->big_ops.size();
+    out->tile_qubits = p->tile;
+    out->reserved = 0;
+    return KSV_OK;
+}
+
+int ksv_program_destroy(ksv_program* p) {
+    if (!p) return fail(KSV_ERR_HANDLE, "null program");
+    if (p->d_ops) hipFree(p->d_ops);
+    if (p->d_mats) hipFree(p->d_mats);
+    delete p;
+    return KSV_OK;
+}
+
+int ksv_apply(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    ksv_program* p = nullptr;
+    int rc = build_program(s, ops, nops, mats, nmats, s->prune, s->fuse != 0, &p);
+    if (rc) return rc;
+    rc = run_program_on(s, p, s->psi);
+    ksv_program_destroy(p);
+    return rc;
+}
+
+// --------------------------------------------------------------------------
+// read-out
+// --------------------------------------------------------------------------
+int ksv_get_amplitudes(ksv_state* s, uint64_t offset, uint64_t count, double* out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (offset > (1ull << s->n) || count > (1ull << s->n) - offset) return fail(KSV_ERR_RANGE, "amplitude range out of bounds");
+    if (!count) return KSV_OK;
+    if (!out) return fail(KSV_ERR_TYPE, "null output buffer");
+    if (s->prec == KSV_F64) {
+        HIPCHK(hipMemcpyAsync(out, (char*)s->psi + 16 * offset, 16 * count, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    } else {
+        std::vector<float> f(2 * count);
+        HIPCHK(hipMemcpyAsync(f.data(), (char*)s->psi + 8 * offset, 8 * count, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        for (uint64_t i = 0; i < 2 * count; i++) out[i] = f[i];
+    }
+    return KSV_OK;
+}
+
+int ksv_set_amplitudes(ksv_state* s, uint64_t offset, uint64_t count, const double* in) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (offset > (1ull << s->n) || count > (1ull << s->n) - offset) return fail(KSV_ERR_RANGE, "amplitude range out of bounds");
+    if (!count) return KSV_OK;
+    if (!in) return fail(KSV_ERR_TYPE, "null input buffer");
+    if (s->prec == KSV_F64) {
+        HIPCHK(hipMemcpyAsync((char*)s->psi + 16 * offset, in, 16 * count, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    } else {
+        std::vector<float> f(2 * count);
+        for (uint64_t i = 0; i < 2 * count; i++) f[i] = (float)in[i];
+        HIPCHK(hipMemcpyAsync((char*)s->psi + 8 * offset, f.data(), 8 * count, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    }
+    return KSV_OK;
+}
+
+int ksv_amplitude(ksv_state* s, uint64_t index, double out[2]) {
+    return ksv_get_amplitudes(s, index, 1, out);
+}
+
+static int fetch_small(ksv_state* s, double* out, int m) {
+    HIPCHK(hipMemcpyAsync(out, s->d_small, sizeof(double) * m, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+int ksv_norm2(ksv_state* s, double* out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    HIPCHK(red_norm2(s->n, s->prec, s->psi, s->d_small, s->d_part, NBLK, s->st));
+    return fetch_small(s, out, 1);
+}
+
+int ksv_marginals(ksv_state* s, double* out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    HIPCHK(red_marginals(s->n, s->prec, s->psi, s->d_small, s->d_part, NBLK, s->st));
+    return fetch_small(s, out, s->n);
+}
+
+int ksv_reduce_1q(ksv_state* s, int q, double out[4]) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (q < 0 || q >= s->n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(s->n) + "-qubit circuit");
+    HIPCHK(red_1q(s->n, s->prec, s->psi, q, s->d_small, s->d_part, NBLK, s->st));
+    return fetch_small(s, out, 4);
+}
+
+// Ordered compaction of amplitudes satisfying a predicate into caller buffers.
+static int compact(ksv_state* s, int mode, double thresh, uint64_t kv, uint64_t* idx, double* p, uint64_t cap,
+                   uint64_t* total) {
+    const uint64_t nch = ((1ull << s->n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG;
+    uint32_t* d_cnt = nullptr;
+    uint64_t* d_off = nullptr;
+    HIPCHK(hipMalloc(&d_cnt, 4 * nch));
+    HIPCHK(hipMalloc(&d_off, 8 * nch));
+    if (mode == 0) HIPCHK(chunk_counts(s->n, s->prec, s->psi, thresh, d_cnt, s->st));
+    else HIPCHK(chunk_counts_pred(s->n, s->prec, s->psi, mode, kv, d_cnt, s->st));
+    std::vector<uint32_t> cnt(nch);
+    std::vector<uint64_t> off(nch);
+    HIPCHK(hipMemcpyAsync(cnt.data(), d_cnt, 4 * nch, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    uint64_t acc = 0;
+    for (uint64_t c = 0; c < nch; c++) { off[c] = acc; acc += cnt[c]; }
+    *total = acc;
+    const uint64_t m = std::min(acc, cap);
+    if (m) {
+        uint64_t* d_idx = nullptr;
+        double* d_p = nullptr;
+        HIPCHK(hipMemcpyAsync(d_off, off.data(), 8 * nch, hipMemcpyHostToDevice, s->st));
+        HIPCHK(hipMalloc(&d_idx, 8 * m));
+        HIPCHK(hipMalloc(&d_p, 8 * m));
+        if (mode == 0) HIPCHK(chunk_compact(s->n, s->prec, s->psi, thresh, d_off, d_idx, d_p, m, s->st));
+        else HIPCHK(chunk_compact_pred(s->n, s->prec, s->psi, mode, kv, d_off, d_idx, d_p, m, s->st));
+        if (idx) HIPCHK(hipMemcpyAsync(idx, d_idx, 8 * m, hipMemcpyDeviceToHost, s->st));
+        if (p) HIPCHK(hipMemcpyAsync(p, d_p, 8 * m, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        hipFree(d_idx);
+        hipFree(d_p);
+    }
+    hipFree(d_cnt);
+    hipFree(d_off);
+    return KSV_OK;
+}
+
+int ksv_probs_filtered(ksv_state* s, double thresh, uint64_t* idx, double* p, size_t cap, size_t* count) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    uint64_t total = 0;
+    int rc = compact(s, 0, thresh, 0, idx, p, cap, &total);
+    if (rc) return rc;
+    if (count) *count = (size_t)total;
+    return KSV_OK;
+}
+
+int ksv_topk(ksv_state* s, size_t k, uint64_t* idx, double* p) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    const uint64_t N = 1ull << s->n;
+    if (k == 0) return KSV_OK;
+    if (k > N) return fail(KSV_ERR_RANGE, "k exceeds the number of amplitudes");
+    if (!idx || !p) return fail(KSV_ERR_TYPE, "null output buffer");
+    // radix-select the k-th largest key (bit pattern of p) with 12-bit digits
+    uint32_t* d_hist = nullptr;
+    HIPCHK(hipMalloc(&d_hist, 4 * 4096));
+    std::vector<uint32_t> h(4096);
+    uint64_t prefix = 0, above = 0;   // `above` = keys strictly greater than the current prefix range
+    uint64_t want = k;                // rank (1-based) of the target within the prefix range
+    const int rounds[6][2] = {{64, 52}, {52, 40}, {40, 28}, {28, 16}, {16, 4}, {4, 0}};
+    for (auto& rd : rounds) {
+        const int hi = rd[0], lo = rd[1];
+        HIPCHK(hipMemsetAsync(d_hist, 0, 4 * 4096, s->st));
+        HIPCHK(key_histogram(s->n, s->prec, s->psi, prefix, hi, lo, d_hist, s->st));
+        HIPCHK(hipMemcpyAsync(h.data(), d_hist, 4 * 4096, hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        const int nb = 1 << std::min(12, hi - lo);
+        uint64_t cum = 0;
+        int d = nb - 1;
+        for (; d >= 0; d--) {
+            if (cum + h[d] >= want) break;
+            cum += h[d];
+        }
+        if (d < 0) d = 0;
+        above += cum;
+        want -= cum;
+        prefix = (prefix << (hi - lo)) | (uint64_t)d;
+    }
+    hipFree(d_hist);
+    const uint64_t kth = prefix;      // full 64-bit key of the k-th largest
+    // keys > kth: `above` of them, all wanted; keys == kth: the first (k - above) by index
+    std::vector<uint64_t> ig(above + 1), ie(k - above + 1);
+    std::vector<double> pg(above + 1), pe(k - above + 1);
+    uint64_t tg = 0, te = 0;
+    int rc = compact(s, 2, 0, kth, ig.data(), pg.data(), above, &tg);
+    if (rc) return rc;
+    rc = compact(s, 1, 0, kth, ie.data(), pe.data(), k - above, &te);
+    if (rc) return rc;
+    std::vector<std::pair<double, uint64_t>> all;
+    all.reserve(k);
+    for (uint64_t i = 0; i < std::min(tg, above); i++) all.push_back({pg[i], ig[i]});
+    for (uint64_t i = 0; i < std::min(te, (uint64_t)(k - above)); i++) all.push_back({pe[i], ie[i]});
+    std::sort(all.begin(), all.end(), [](const std::pair<double, uint64_t>& a, const std::pair<double, uint64_t>& b) {
+        if (a.first != b.first) return a.first > b.first;
+        return a.second < b.second;
+    });
+    for (size_t i = 0; i < k; i++) {
+        idx[i] = i < all.size() ? all[i].second : 0;
+        p[i] = i < all.size() ? all[i].first : 0;
+    }
+    return KSV_OK;
+}
+
+int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (!shots) return KSV_OK;
+    if (!r || !out) return fail(KSV_ERR_TYPE, "null buffer");
+    const double thresh = 1e-14;
+    const int64_t nch = (int64_t)(((1ull << s->n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG);
+    double *d_sum = nullptr, *d_r = nullptr;
+    uint64_t *d_out = nullptr, *d_last = nullptr;
+    HIPCHK(hipMalloc(&d_sum, 8 * nch));
+    HIPCHK(chunk_sums(s->n, s->prec, s->psi, thresh, d_sum, s->st));
+    std::vector<double> cum(nch);
+    HIPCHK(hipMemcpyAsync(cum.data(), d_sum, 8 * nch, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    int64_t lastc = -1;
+    double acc = 0;
+    for (int64_t c = 0; c < nch; c++) {
+        if (cum[c] > 0) lastc = c;
+        acc += cum[c];
+        cum[c] = acc;
+    }
+    if (lastc < 0) {   // no probability above the filter: ket returns index 0
+        for (size_t i = 0; i < shots; i++) out[i] = 0;
+        hipFree(d_sum);
+        return KSV_OK;
+    }
+    HIPCHK(hipMalloc(&d_last, 8));
+    HIPCHK(hipMemsetAsync(d_last, 0, 8, s->st));
+    HIPCHK(last_nonzero_chunk(s->n, s->prec, s->psi, thresh, lastc, d_last, s->st));
+    uint64_t last_idx = 0;
+    HIPCHK(hipMemcpyAsync(&last_idx, d_last, 8, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(d_sum, cum.data(), 8 * nch, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMalloc(&d_r, 8 * shots));
+    HIPCHK(hipMalloc(&d_out, 8 * shots));
+    HIPCHK(hipMemcpyAsync(d_r, r, 8 * shots, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(sample_search(s->n, s->prec, s->psi, thresh, d_sum, nch, d_r, shots, last_idx, d_out, s->st));
+    HIPCHK(hipMemcpyAsync(out, d_out, 8 * shots, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    hipFree(d_sum);
+    hipFree(d_r);
+    hipFree(d_out);
+    hipFree(d_last);
+    return KSV_OK;
+}
+
+static int ensure_psi2(ksv_state* s) {
+    if (s->psi2) return KSV_OK;
+    hipError_t e = hipMalloc(&s->psi2, state_bytes(s));
+    if (e != hipSuccess) { s->psi2 = nullptr; return fail(KSV_ERR_DEVICE, "out of device memory for a second state"); }
+    return KSV_OK;
+}
+
+// Rotation into the Z basis of one Pauli term (ket: X -> h, Y -> rx(pi/2)).
+static void rotation_tape(int n, uint64_t x, uint64_t z, std::vector<ksv_op>& ops, std::vector<double>& mats) {
+    const double sq2 = 1 / std::sqrt(2.0);
+    const double h[8] = {sq2, 0, sq2, 0, sq2, 0, -sq2, 0};
+    const double co = std::cos(M_PI / 2 / 2), si = std::sin(M_PI / 2 / 2);
+    const double rx[8] = {co, 0, 0, -si, 0, -si, co, 0};
+    mats.assign(h, h + 8);
+    mats.insert(mats.end(), rx, rx + 8);
+    ops.clear();
+    for (int q = 0; q < n; q++) {
+        bool bx = (x >> q) & 1, bz = (z >> q) & 1;
+        if (!bx) continue;
+        ksv_op o;
+        std::memset(&o, 0, sizeof o);
+        o.kind = KSV_SINGLE;
+        o.nq = 1;
+        o.q[0] = q;
+        o.mat = bz ? 8 : 0;
+        ops.push_back(o);
+    }
+}
+
+int ksv_expect_pauli(ksv_state* s, const uint64_t* xmask, const uint64_t* zmask, const double* coeff, size_t nterms,
+                     int mode, double* out, double* total) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (nterms && (!xmask || !zmask || !coeff || !out)) return fail(KSV_ERR_TYPE, "null term buffer");
+    if (mode != 0 && mode != 1) return fail(KSV_ERR_TYPE, "unknown expectation mode");
+    const uint64_t all = (1ull << s->n) - 1;
+    for (size_t t = 0; t < nterms; t++)
+        if ((xmask[t] | zmask[t]) & ~all) return fail(KSV_ERR_RANGE, "Pauli term acts outside the register");
+    double tot = 0;
+    if (mode == 0) {
+        uint64_t *d_x = nullptr, *d_z = nullptr;
+        if (nterms) {
+            HIPCHK(hipMalloc(&d_x, 8 * nterms));
+            HIPCHK(hipMalloc(&d_z, 8 * nterms));
+            HIPCHK(hipMemcpyAsync(d_x, xmask, 8 * nterms, hipMemcpyHostToDevice, s->st));
+            HIPCHK(hipMemcpyAsync(d_z, zmask, 8 * nterms, hipMemcpyHostToDevice, s->st));
+        }
+        for (size_t t0 = 0; t0 < nterms; t0 += 8) {
+            int m = (int)std::min<size_t>(8, nterms - t0);
+            HIPCHK(red_pauli(s->n, s->prec, s->psi, d_x + t0, d_z + t0, m, s->d_small, s->d_part, NBLK, s->st));
+            double v[16];
+            int rc = fetch_small(s, v, 16);
+            if (rc) return rc;
+            for (int j = 0; j < m; j++) {
+                size_t t = t0 + j;
+                int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+                double re = v[2 * j], im = v[2 * j + 1], val;
+                switch (ny) {
+                case 0: val = re; break;
+                case 1: val = -im; break;
+                case 2: val = -re; break;
+                default: val = im; break;
+                }
+                out[t] = coeff[t] * val;
+                tot += out[t];
+            }
+        }
+        if (d_x) hipFree(d_x);
+        if (d_z) hipFree(d_z);
+    } else {
+        std::vector<ksv_op> ops;
+        std::vector<double> mats;
+        for (size_t t = 0; t < nterms; t++) {
+            if (std::fabs(coeff[t]) < 1e-15) { out[t] = 0; continue; }
+            if (!(xmask[t] | zmask[t])) { out[t] = coeff[t]; tot += out[t]; continue; }
+            int rc = ensure_psi2(s);
+            if (rc) return rc;
+            HIPCHK(k_copy(s->n, s->prec, s->psi2, s->psi, s->st));
+            rotation_tape(s->n, xmask[t], zmask[t], ops, mats);
+            if (!ops.empty()) {
+                ksv_program* p = nullptr;
+                rc = build_program(s, ops.data(), ops.size(), mats.data(), mats.size(), 1, false, &p);
+                if (rc) return rc;
+                rc = run_program_on(s, p, s->psi2);
+                ksv_program_destroy(p);
+                if (rc) return rc;
+            }
+            HIPCHK(red_parity(s->n, s->prec, s->psi2, xmask[t] | zmask[t], 1e-14, s->d_small, s->d_part, NBLK, s->st));
+            double v;
+            rc = fetch_small(s, &v, 1);
+            if (rc) return rc;
+            out[t] = coeff[t] * v;
+            tot += out[t];
+        }
+    }
+    if (total) *total = tot;
+    return KSV_OK;
+}
+
+int ksv_collapse(ksv_state* s, int q, double r, int* outcome) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (q < 0 || q >= s->n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(s->n) + "-qubit circuit");
+    HIPCHK(red_bit1(s->n, s->prec, s->psi, q, s->d_small, s->d_part, NBLK, s->st));
+    double p1 = 0;
+    int rc = fetch_small(s, &p1, 1);
+    if (rc) return rc;
+    int o = r < p1 ? 1 : 0;
+    double scale = 1 / std::sqrt(o ? p1 : 1 - p1);
+    HIPCHK(k_project(s->n, s->prec, s->psi, q, o, scale, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    if (outcome) *outcome = o;
+    return KSV_OK;
+}
+
+
+int ksv_get_stats(ksv_state* s, ksv_stats* out) {
+    if (!s || !out) return fail(KSV_ERR_HANDLE, "null state");
+    *out = s->stats;
+    return KSV_OK;
+}
+
+int ksv_reset_stats(ksv_state* s) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    std::memset(&s->stats, 0, sizeof s->stats);
+    return KSV_OK;
+}
+
+}  // extern "C"
+
+// Helpers for the adjoint translation unit.
+namespace ksv {
+int adj_fail(int code, const std::string& msg) { return fail(code, msg); }
+
+int adj_context(ksv_state* s, int* n, int* prec, void** psi, void** psi2, hipStream_t* st, double** part,
+                int* prune, int* tile) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    int rc = ensure_psi2(s);
+    if (rc) return rc;
+    *n = s->n;
+    *prec = s->prec;
+    *psi = s->psi;
+    *psi2 = s->psi2;
+    *st = s->st;
+    *part = s->d_part;
+    *prune = s->prune;
+    *tile = default_tile(s);
+    return KSV_OK;
+}
+
+int adj_run_tape(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int prune,
+                 void* psi) {
+    ksv_program* p = nullptr;
+    int rc = build_program(s, ops, nops, mats, nmats, prune, false, &p);
+    if (rc) return rc;
+    rc = run_program_on(s, p, psi);
+    ksv_program_destroy(p);
+    return rc;
+}
+}  // namespace ksv

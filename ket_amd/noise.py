@@ -1,0 +1,148 @@
+"""run()'s per-shot path: noise, mid-circuit measurement, reset and `if`
+(src/circuit.ts:169-207, :209-304, :3455-3481), driven from the host over
+the device state.
+
+Each shot re-simulates the circuit, consuming the seeded xorshift32 stream
+in ket's order: one draw per 1-qubit gate when p1 > 0, one per cnot /
+controlled / swap / two gate when p2 > 0, one per measure (a second when
+pMeas > 0), one per reset, one for the final sample, then one per qubit
+when pMeas > 0.  Gates between two draws that need the state (measure,
+reset) are applied as one tape.
+"""
+import numpy as np
+
+from . import gates as G
+from ._native import State
+from .tape import pack
+
+# Published device error rates (src/circuit.ts:249-261).
+DEVICES = {
+    "aria-1": {"qubits": 25, "nativeGates": ("gpi", "gpi2", "ms", "vz"), "noise": {"p1": 0.0003, "p2": 0.005, "pMeas": 0.004}},
+    "forte-1": {"qubits": 36, "nativeGates": ("gpi", "gpi2", "ms", "vz", "zz"), "noise": {"p1": 0.0001, "p2": 0.002, "pMeas": 0.002}},
+    "harmony": {"qubits": 11, "nativeGates": ("gpi", "gpi2", "ms", "vz"), "noise": {"p1": 0.001, "p2": 0.015, "pMeas": 0.01}},
+    "ibm_sherbrooke": {"qubits": 127, "noise": {"p1": 2.4e-4, "p2": 7.4e-3, "pMeas": 1.35e-2}},
+    "ibm_brisbane": {"qubits": 127, "noise": {"p1": 2.4e-4, "p2": 7.6e-3, "pMeas": 1.35e-2}},
+    "ibm_torino": {"qubits": 133, "noise": {"p1": 2.0e-4, "p2": 3.0e-3, "pMeas": 1.0e-2}},
+    "h1-1": {"qubits": 20, "noise": {"p1": 1.8e-5, "p2": 9.7e-4, "pMeas": 2.3e-3}},
+    "h2-1": {"qubits": 56, "noise": {"p1": 1.9e-5, "p2": 1.1e-3, "pMeas": 1.0e-3}},
+}
+
+# 15 non-identity two-qubit Paulis (src/circuit.ts:273-278)
+_TWO_PAULI = [(None, G.X), (None, G.Y), (None, G.Z),
+              (G.X, None), (G.X, G.X), (G.X, G.Y), (G.X, G.Z),
+              (G.Y, None), (G.Y, G.X), (G.Y, G.Y), (G.Y, G.Z),
+              (G.Z, None), (G.Z, G.X), (G.Z, G.Y), (G.Z, G.Z)]
+
+
+def resolve_noise(noise):
+    """Device name -> NoiseParams dict, dict as-is, None -> None (src/circuit.ts:3411-3417)."""
+    if noise is None:
+        return None
+    if isinstance(noise, str):
+        d = DEVICES.get(noise)
+        if d is None:
+            raise TypeError(f"Unknown device profile '{noise}'. Known: {', '.join(DEVICES)}")
+        return dict(d["noise"])
+    return dict(noise)
+
+
+def _creg_value(shot_cregs, name):
+    bits = shot_cregs.get(name, [])
+    v = 0
+    for i, b in enumerate(bits):
+        if b:
+            v |= 1 << i
+    return v
+
+
+class _Shot:
+    def __init__(self, st, rng, noise):
+        self.st = st
+        self.rng = rng
+        self.p1 = (noise or {}).get("p1", 0) or 0
+        self.p2 = (noise or {}).get("p2", 0) or 0
+        self.pm = (noise or {}).get("pMeas", 0) or 0
+        self.pending = []
+
+    def flush(self):
+        if self.pending:
+            self.st.apply(pack(self.pending))
+            self.pending = []
+
+    def dep1(self, q):
+        rand = self.rng()
+        if rand >= self.p1:
+            return
+        r = rand / self.p1
+        g = G.X if r < 1 / 3 else (G.Y if r < 2 / 3 else G.Z)
+        self.pending.append(("single", q, g, None))
+
+    def dep2(self, a, b):
+        rand = self.rng()
+        if rand >= self.p2:
+            return
+        pa, pb = _TWO_PAULI[min(int(rand / self.p2 * 15), 14)]
+        if pa is not None:
+            self.pending.append(("single", a, pa, None))
+        if pb is not None:
+            self.pending.append(("single", b, pb, None))
+
+    def run_ops(self, ops, shot_cregs):
+        from .circuit import _flatten
+        for op in _flatten(ops):
+            k = op[0]
+            if k == "single":
+                self.pending.append(op)
+                if self.p1:
+                    self.dep1(op[1])
+            elif k in ("cnot", "controlled", "swap", "two"):
+                self.pending.append(op)
+                if self.p2:
+                    self.dep2(op[1], op[2])
+            elif k in ("toffoli", "cswap", "csrswap", "unitary"):
+                self.pending.append(op)
+            elif k == "measure":
+                self.flush()
+                outcome = self.st.collapse(op[1], self.rng())
+                reported = (1 - outcome) if (self.pm and self.rng() < self.pm) else outcome
+                reg = shot_cregs.get(op[2])
+                if reg is not None:
+                    reg[op[3]] = reported == 1
+            elif k == "reset":
+                self.flush()
+                outcome = self.st.collapse(op[1], self.rng())
+                if outcome == 1:
+                    self.pending.append(("single", op[1], G.X, None))
+            elif k == "if":
+                self.flush()
+                if _creg_value(shot_cregs, op[1]) == op[2]:
+                    self.run_ops(op[3], shot_cregs)
+            # barrier: nothing
+        self.flush()
+
+
+def run_per_shot(circ, shots, rng, noise, init, creg_counts):
+    """Per-shot simulation; returns (counts {index: n}, creg_counts)."""
+    from .circuit import OPTIONS
+    st = State(circ.qubits, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+    counts = {}
+    pm = (noise or {}).get("pMeas", 0) or 0
+    ops = circ._ops()
+    for _ in range(shots):
+        shot_cregs = {name: [False] * size for name, size in circ._cregs.items()}
+        st.init_basis(init)
+        sh = _Shot(st, rng, noise)
+        sh.run_ops(ops, shot_cregs)
+        idx = int(st.sample(np.array([rng()]))[0])
+        if pm:
+            for q in range(circ.qubits):
+                if rng() < pm:
+                    idx ^= 1 << q
+        counts[idx] = counts.get(idx, 0) + 1
+        for name, bits in shot_cregs.items():
+            acc = creg_counts[name]
+            for j, b in enumerate(bits):
+                if b:
+                    acc[j] += 1
+    st.free()
+    return counts, creg_counts

@@ -1,0 +1,186 @@
+"""ctypes binding of the CPU oracle (libketsv_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker — never by the ket_amd package.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_char_p, c_double, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libketsv_oracle.so")
+MAXQ = 12
+
+
+class OrcGate(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 12), ("nq", ctypes.c_int32), ("q", ctypes.c_int32 * MAXQ),
+                ("p", c_double * 3), ("mat", ctypes.c_int64)]
+
+
+def build():
+    subprocess.run(["make", "-B", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_gate_matrix.argtypes = [c_char_p, POINTER(c_double), POINTER(c_double)]
+        L.orc_gate_matrix.restype = c_int
+        L.orc_apply.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_int]
+        L.orc_apply.restype = c_int
+        L.orc_apply_named.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_int]
+        L.orc_apply_named.restype = c_int
+        L.orc_init_basis.argtypes = [c_int, c_void_p, c_uint64]
+        L.orc_probabilities.argtypes = [c_int, c_void_p, c_double, c_void_p, c_void_p]
+        L.orc_probabilities.restype = c_size_t
+        L.orc_prng.argtypes = [c_uint32, c_size_t, c_void_p]
+        L.orc_sample.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p]
+        L.orc_sample.restype = c_int
+        L.orc_marginals.argtypes = [c_int, c_void_p, c_void_p]
+        L.orc_bloch.argtypes = [c_int, c_void_p, c_int, POINTER(c_double), POINTER(c_double)]
+        L.orc_expectation.argtypes = [c_int, c_void_p, c_char_p]
+        L.orc_expectation.restype = c_double
+        L.orc_vqe.argtypes = [c_int, c_void_p, POINTER(c_char_p), c_void_p, c_size_t]
+        L.orc_vqe.restype = c_double
+        L.orc_expect_direct.argtypes = [c_int, c_void_p, c_uint64, c_uint64]
+        L.orc_expect_direct.restype = c_double
+        L.orc_collapse.argtypes = [c_int, c_void_p, c_int, c_double]
+        L.orc_collapse.restype = c_int
+        L.orc_gate_struct_size.restype = c_size_t
+        L.orc_op_struct_size.restype = c_size_t
+        assert L.orc_gate_struct_size() == ctypes.sizeof(OrcGate)
+        assert L.orc_op_struct_size() == 64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return c_void_p(a.ctypes.data)
+
+
+def gate_matrix(name, params=()):
+    ps = (c_double * 3)(*(list(params) + [0.0] * (3 - len(params))))
+    out = (c_double * 32)()
+    dim = lib().orc_gate_matrix(name.encode(), ps, out)
+    if dim < 0:
+        raise KeyError(name)
+    v = np.array(out[: 2 * dim * dim])
+    return (v[0::2] + 1j * v[1::2]).reshape(dim, dim)
+
+
+def zero_state(n, index=0):
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[index] = 1
+    return psi
+
+
+# method name -> number of qubit operands (the rest of the arguments are angles)
+NQ = {}
+for _m in ("id h x y z s si sdg t ti tdg v vi srn srndg rx ry rz vz r2 r4 r8 u1 p u2 u3 gpi gpi2").split():
+    NQ[_m] = 1
+for _m in ("cnot cx swap xx yy zz xy iswap srswap ms cy cz ch crx cry crz cr2 cr4 cr8 cu1 cu2 cu3 cs ct csdg ctdg "
+           "csrn").split():
+    NQ[_m] = 2
+for _m in ("ccx cswap csrswap").split():
+    NQ[_m] = 3
+
+
+def apply_named(n, psi, calls, prune=True):
+    """Apply a list of (method, args) in ket's Circuit-method form to psi (in place).
+
+    ('unitary', (matrix, q0, q1, ...)) carries its matrix first.
+    """
+    gates = (OrcGate * max(len(calls), 1))()
+    mats = []
+    for i, (name, args) in enumerate(calls):
+        g = gates[i]
+        g.name = name.encode()
+        g.mat = -1
+        if name == "unitary":
+            m = np.asarray(args[0], dtype=np.complex128)
+            qs = args[1:]
+            g.mat = len(mats)
+            for v in m.reshape(-1):
+                mats.extend([v.real, v.imag])
+        elif name == "barrier":
+            qs = args
+        else:
+            k = NQ[name]
+            qs = args[len(args) - k:]
+            for j, v in enumerate(args[: len(args) - k]):
+                g.p[j] = float(v)
+        g.nq = len(qs)
+        for j, q in enumerate(qs):
+            g.q[j] = int(q)
+    m = np.asarray(mats if mats else [0.0], dtype=np.float64)
+    rc = lib().orc_apply_named(n, _p(psi), ctypes.addressof(gates), len(calls), _p(m), 1 if prune else 0)
+    if rc:
+        raise RuntimeError(f"oracle rejected the circuit (code {rc})")
+    return psi
+
+
+def apply_tape(n, psi, ops, mats, prune=True):
+    m = mats if len(mats) else np.zeros(1)
+    rc = lib().orc_apply(n, _p(psi), _p(ops), len(ops), _p(m), 1 if prune else 0)
+    if rc:
+        raise RuntimeError(f"oracle rejected the tape (code {rc})")
+    return psi
+
+
+def probabilities(n, psi, thresh=1e-14):
+    idx = np.empty(1 << n, dtype=np.uint64)
+    p = np.empty(1 << n, dtype=np.float64)
+    k = lib().orc_probabilities(n, _p(psi), thresh, _p(idx), _p(p))
+    return idx[:k], p[:k]
+
+
+def prng(seed, count):
+    out = np.empty(count, dtype=np.float64)
+    lib().orc_prng(seed & 0xFFFFFFFF, count, _p(out))
+    return out
+
+
+def sample(n, psi, r):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.empty(len(r), dtype=np.uint64)
+    lib().orc_sample(n, _p(psi), _p(r), len(r), _p(out))
+    return out
+
+
+def marginals(n, psi):
+    out = np.empty(n, dtype=np.float64)
+    lib().orc_marginals(n, _p(psi), _p(out))
+    return out
+
+
+def bloch(n, psi, q):
+    th, ph = c_double(), c_double()
+    lib().orc_bloch(n, _p(psi), q, ctypes.byref(th), ctypes.byref(ph))
+    return th.value, ph.value
+
+
+def expectation(n, psi, pauli):
+    return lib().orc_expectation(n, _p(psi), pauli.upper().encode())
+
+
+def vqe(n, psi, terms):
+    ops = (c_char_p * len(terms))(*[t["ops"].upper().encode() for t in terms])
+    c = np.array([t["coeff"] for t in terms], dtype=np.float64)
+    return lib().orc_vqe(n, _p(psi), ops, _p(c), len(terms))
+
+
+def expect_direct(n, psi, x, z):
+    return lib().orc_expect_direct(n, _p(psi), x, z)
+
+
+def collapse(n, psi, q, r):
+    return lib().orc_collapse(n, _p(psi), q, r)

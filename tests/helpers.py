@@ -1,0 +1,71 @@
+"""Shared test helpers: random circuits in ket's Circuit-method form."""
+import math
+import random
+
+import numpy as np
+
+import ket_amd
+
+SINGLE0 = "id h x y z s si sdg t ti tdg v vi srn srndg r2 r4 r8".split()
+SINGLE1 = "rx ry rz vz u1 p gpi gpi2".split()
+TWO0 = "cnot cx swap iswap srswap cy cz ch cr2 cr4 cr8 cs ct csdg ctdg csrn".split()
+TWO1 = "xx yy zz xy crx cry crz cu1".split()
+THREE = "ccx cswap csrswap".split()
+
+
+def random_calls(n, count, seed, kinds="all", with_unitary=True):
+    """A seeded random circuit over every gate method of ket's Circuit."""
+    rnd = random.Random(seed)
+    calls = []
+    for _ in range(count):
+        r = rnd.random()
+        ang = lambda: rnd.uniform(-2 * math.pi, 2 * math.pi)
+        if r < 0.25 or n < 2:
+            q = rnd.randrange(n)
+            if rnd.random() < 0.5:
+                calls.append((rnd.choice(SINGLE0), (q,)))
+            else:
+                name = rnd.choice(SINGLE1 + ["u2", "u3"])
+                k = {"u2": 2, "u3": 3}.get(name, 1)
+                calls.append((name, tuple(ang() for _ in range(k)) + (q,)))
+        elif r < 0.65 or n < 3:
+            a, b = rnd.sample(range(n), 2)
+            if rnd.random() < 0.5:
+                calls.append((rnd.choice(TWO0), (a, b)))
+            else:
+                name = rnd.choice(TWO1 + ["ms", "cu2", "cu3"])
+                k = {"ms": 2, "cu2": 2, "cu3": 3}.get(name, 1)
+                calls.append((name, tuple(ang() for _ in range(k)) + (a, b)))
+        elif r < 0.9 or not with_unitary:
+            a, b, c = rnd.sample(range(n), 3)
+            calls.append((rnd.choice(THREE), (a, b, c)))
+        else:
+            k = rnd.choice([1, 2, 3]) if n >= 3 else 1
+            qs = rnd.sample(range(n), k)
+            calls.append(("unitary", (random_unitary(1 << k, rnd),) + tuple(qs)))
+    return calls
+
+
+def random_unitary(dim, rnd):
+    rs = np.random.RandomState(rnd.randrange(1 << 30))
+    z = rs.normal(size=(dim, dim)) + 1j * rs.normal(size=(dim, dim))
+    q, r = np.linalg.qr(z)
+    q = q * (np.diag(r) / np.abs(np.diag(r)))
+    return [[complex(v) for v in row] for row in q]
+
+
+def build(n, calls):
+    c = ket_amd.Circuit(n)
+    for name, args in calls:
+        if name == "unitary":
+            c = c.unitary(args[0], *args[1:])
+        else:
+            c = getattr(c, name)(*args)
+    return c
+
+
+def dense_ref(n, calls, prune=True, init=0):
+    import ketsv_oracle as O
+    psi = O.zero_state(n, init)
+    O.apply_named(n, psi, calls, prune=prune)
+    return psi

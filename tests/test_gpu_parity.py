@@ -1,0 +1,305 @@
+"""GPU parity: libketsv (through the C ABI) against the CPU oracle.
+
+Bars: amplitudes and probabilities within 1e-12 (FP64), seeded shot
+indices identical, gradients within 1e-8 of the parameter-shift rule.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import ket_amd
+from ket_amd import Circuit, workloads
+from ket_amd._native import State
+
+import ketsv_oracle as O
+from helpers import build, dense_ref, random_calls
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False):
+    c = build(n, calls)
+    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse)
+    if init:
+        st.init_basis(init)
+    st.apply(c.tape())
+    a = st.amplitudes()
+    st.free()
+    return a
+
+
+def test_config1_probs_and_shots():
+    """12 qubits, 20 layers (830 gates): exactProbs and 10 000 seeded shots."""
+    n = 12
+    calls = workloads.random_layer_calls(n, 20, seed=42)
+    assert len(calls) == 830
+    ref = dense_ref(n, calls, prune=True)
+    c = build(n, calls)
+    got = c.statevectorDense()
+    assert np.max(np.abs(got - ref)) < TOL
+    probs = c.exactProbs()
+    idx, p = O.probabilities(n, ref)
+    assert len(probs) == len(idx)
+    for i, v in zip(idx.tolist(), p.tolist()):
+        bs = format(i, "b").zfill(n)[::-1]
+        assert abs(probs[bs] - v) < TOL
+    dist = c.run(shots=10000, seed=42)
+    r = O.prng(42, 10000)
+    want = O.sample(n, ref, r)
+    u, cnt = np.unique(want, return_counts=True)
+    assert len(dist.probs) == len(u)
+    for i, k in zip(u.tolist(), cnt.tolist()):
+        assert dist.histogram[str(i)] == k / 10000
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12])
+@pytest.mark.parametrize("prune", [True, False])
+def test_random_circuits_every_op(n, prune):
+    calls = random_calls(n, 150, seed=1000 + n)
+    ref = dense_ref(n, calls, prune=prune)
+    got = gpu_state(n, calls, prune=prune)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("n,tile", [(14, 8), (15, 10), (16, 12), (17, 11), (18, 13), (18, 12)])
+def test_multi_pass_tiles(n, tile):
+    """Many tiles and passes: global controls, diagonal gates on global qubits."""
+    calls = random_calls(n, 120, seed=7 * n + tile, with_unitary=False)
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, tile=tile)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+def test_fused_one_qubit_runs_exact_mode():
+    n = 14
+    calls = workloads.random_layer_calls(n, 6, seed=3)
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, fuse=True, tile=10)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+def test_wide_unitary():
+    import random as _r
+    from helpers import random_unitary
+    rnd = _r.Random(5)
+    n = 9
+    calls = [("h", (q,)) for q in range(n)]
+    calls.append(("unitary", (random_unitary(32, rnd), 0, 4, 2, 7, 8)))
+    calls.append(("unitary", (random_unitary(64, rnd), 8, 1, 3, 5, 6, 0)))
+    calls.append(("unitary", (random_unitary(16, rnd), 3, 1, 8, 2)))
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+def test_qft_closed_form():
+    n = 10
+    c, bs, x = workloads.qft_input(n, seed=7)
+    got = c.statevectorDense(initialState=bs)
+    k = np.arange(1 << n)
+    want = np.exp(2j * np.pi * x * k / (1 << n)) / math.sqrt(1 << n)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_initial_state_and_amplitude():
+    c = Circuit(3).x(0).cnot(0, 2)
+    assert c.exactProbs() == {"101": 1.0}
+    assert abs(c.amplitude("101") - 1) < TOL
+    assert c.amplitude("000") == 0
+    assert c.exactProbs(initialState="100") == {"000": 1.0}
+    assert c.exactProbs(initialState="001") == {"100": 1.0}
+    b = Circuit(2).h(0).cnot(0, 1)
+    assert b.stateAsString() == "0.7071|00⟩ + 0.7071|11⟩"
+    assert abs(b.probability("11") - 0.5) < TOL
+
+
+def test_readouts_against_oracle():
+    n = 9
+    calls = random_calls(n, 120, seed=99)
+    ref = dense_ref(n, calls, prune=True)
+    c = build(n, calls)
+    m = c.marginals()
+    assert np.max(np.abs(np.array(m) - O.marginals(n, ref))) < TOL
+    for q in (0, 4, 8):
+        th, ph = O.bloch(n, ref, q)
+        b = c.blochAngles(q)
+        assert abs(b["theta"] - th) < 1e-9 and (abs(b["phi"] - ph) < 1e-9 or abs(abs(b["phi"] - ph) - 2 * math.pi) < 1e-9)
+    for pauli in ("XIZYIIXZY", "ZZZZZZZZZ", "IIIIXIIII", "YYIIIIIII"):
+        assert abs(c.expectation(pauli) - O.expectation(n, ref, pauli)) < TOL
+    H = workloads.tfim_hamiltonian(n)
+    assert abs(ket_amd.vqe(c, H) - O.vqe(n, ref, H)) < TOL
+
+
+def test_exact_mode_expectation_direct():
+    n = 8
+    calls = random_calls(n, 100, seed=11)
+    ket_amd.configure(prune=False)
+    ref = dense_ref(n, calls, prune=False)
+    c = build(n, calls)
+    for pauli in ("XYZIXYZI", "ZIIIIIIZ", "YIIIIIII"):
+        x = z = 0
+        for q, ch in enumerate(pauli):
+            if ch in "XY":
+                x |= 1 << q
+            if ch in "YZ":
+                z |= 1 << q
+        assert abs(c.expectation(pauli) - O.expect_direct(n, ref, x, z)) < TOL
+
+
+def test_topk_and_norm():
+    n = 16
+    calls = workloads.random_layer_calls(n, 4, seed=5)
+    ket_amd.configure(prune=False)
+    ref = dense_ref(n, calls, prune=False)
+    st = build(n, calls).simulate()
+    assert abs(st.norm2() - 1) < 1e-12
+    idx, p = st.topk(1024)
+    pr = np.abs(ref) ** 2
+    order = np.lexsort((np.arange(len(pr)), -pr))[:1024]
+    assert np.array_equal(idx.astype(np.int64), order)
+    assert np.max(np.abs(p - pr[order])) < TOL
+    st.free()
+
+
+def test_topk_ties_qft():
+    """All 2^n probabilities equal: top-k by ascending index among ties."""
+    n = 14
+    c, bs, x = workloads.qft_input(n, seed=7)
+    ket_amd.configure(prune=False)
+    st = c.simulate(initialState=bs)
+    idx, p = st.topk(100)
+    assert np.max(np.abs(p * (1 << n) - 1)) < 1e-9
+    st.free()
+
+
+@pytest.mark.parametrize("n", [4, 13, 17])
+def test_sampling_matches_oracle(n):
+    calls = workloads.random_layer_calls(n, 3, seed=21 + n)
+    ref = dense_ref(n, calls, prune=True)
+    st = build(n, calls).simulate()
+    r = O.prng(123, 4000)
+    got = st.sample(r)
+    want = O.sample(n, ref, r)
+    st.free()
+    assert np.array_equal(got, want)
+
+
+def test_sampling_sparse_state_last_entry():
+    """r above the last CDF value lands on the last non-zero index (cdf[-1] := 1)."""
+    c = Circuit(14).h(3).cnot(3, 13)
+    st = c.simulate()
+    # p = (1/sqrt 2)^2 = 0.4999999999999999 each; the CDF total stays below 1
+    got = st.sample(np.array([0.2, 0.3, 0.7, 0.9999999999999999]))
+    st.free()
+    assert got.tolist() == [0, 0, 8 | 8192, 8 | 8192]
+
+
+def test_collapse_and_per_shot_measure():
+    c = Circuit(2).h(0).measure(0, "c", 0).if_("c", 1, lambda k: k.x(1))
+    d = c.run(shots=200, seed=9)
+    # q1 always equals q0 after the conditional flip
+    assert set(d.probs) <= {"00", "11"}
+    assert abs(d.cregs["c"][0] - d.probs.get("11", 0)) < 1e-12
+
+
+def test_noise_run_reproducible():
+    c = Circuit(3).h(0).cnot(0, 1).cnot(1, 2)
+    a = c.run(shots=100, seed=5, noise="harmony")
+    b = c.run(shots=100, seed=5, noise="harmony")
+    assert a.probs == b.probs
+
+
+def test_errors_map_to_ket_classes():
+    with pytest.raises(ket_amd.RangeError):
+        Circuit(2).h(2)
+    with pytest.raises(TypeError):
+        Circuit(2).cnot(1, 1)
+    with pytest.raises(TypeError):
+        Circuit(3).unitary([[1, 0], [0, 1], [0, 0]], 0)
+    with pytest.raises(TypeError):
+        Circuit(3).xx(0.3, 1, 1).statevector()
+    with pytest.raises(TypeError):
+        Circuit(1).rx("theta", 0).statevector()
+
+
+def _psr_oracle(ansatz, H, params, n):
+    out = []
+    for i in range(len(params)):
+        e = []
+        for s in (math.pi / 2, -math.pi / 2):
+            p = list(params)
+            p[i] += s
+            calls = [(op[1][0], op[1][1]) for op in _calls_of(ansatz(p))]
+            psi = dense_ref(n, calls, prune=True)
+            e.append(O.vqe(n, psi, H))
+        out.append(0.5 * (e[0] - e[1]))
+    return out
+
+
+def _calls_of(circ):
+    """Recover (method, args) calls from a Circuit built by efficientSU2/realAmplitudes."""
+    out = []
+    for op in circ.ops:
+        if op[0] == "single":
+            name, ps = op[3]
+            out.append(("c", (name, tuple(float(v) for v in ps) + (op[1],))))
+        elif op[0] == "cnot":
+            out.append(("c", ("cnot", (op[1], op[2]))))
+        else:
+            raise AssertionError(op[0])
+    return out
+
+
+@pytest.mark.parametrize("which", ["su2", "real"])
+def test_adjoint_matches_parameter_shift(which):
+    n = 6
+    ans = ket_amd.efficientSU2(n, 2) if which == "su2" else ket_amd.realAmplitudes(n, 3)
+    params = workloads.ansatz_params(ans.paramCount, seed=1234)
+    H = workloads.tfim_hamiltonian(n)
+    g = ket_amd.gradient(ans, H, params)
+    want = _psr_oracle(ans, H, params, n)
+    assert np.max(np.abs(np.array(g) - np.array(want))) < 1e-8
+    gp = ket_amd.gradient(ans, H, params, method="psr")
+    assert np.max(np.abs(np.array(gp) - np.array(want))) < 1e-10
+
+
+def test_adjoint_qaoa_against_finite_differences():
+    n = 6
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (0, 5), (1, 4)]
+    H = ket_amd.maxCutHamiltonian(n, edges)
+
+    def ans(p):
+        return ket_amd.qaoa(n, edges, p[:2], p[2:])
+
+    params = [0.7, 0.4, 0.3, 1.1]
+    ket_amd.configure(prune=False)
+    g = ket_amd.gradient(ans, H, params, method="adjoint")
+    h = 1e-6
+    fd = []
+    for i in range(4):
+        pp = list(params); pp[i] += h
+        pm = list(params); pm[i] -= h
+        fd.append((ket_amd.vqe(ans(pp), H) - ket_amd.vqe(ans(pm), H)) / (2 * h))
+    assert np.max(np.abs(np.array(g) - np.array(fd))) < 1e-6
+
+
+def test_program_reuse_and_stats():
+    n = 16
+    c = workloads.random_layers(n, 4, seed=1)
+    st = State(n, "f64", prune=False, timing=True)
+    prog = st.program(c.tape())
+    info = prog.info()
+    assert info["passes"] >= 1
+    prog.run()
+    a1 = st.amplitudes()
+    st.init_basis(0)
+    prog.run()
+    a2 = st.amplitudes()
+    assert np.array_equal(a1, a2)
+    s = st.stats()
+    assert s["pass_launches"] == 2 * info["passes"] and s["pass_ms"] > 0
+    prog.free()
+    st.free()

@@ -1,0 +1,163 @@
+"""Host-side logic without a GPU: gate matrices, tape lowering, builder
+errors, PRNG, Distribution, gradient tracing, and the C ABI exports."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ket_amd
+from ket_amd import Circuit, gates as G, workloads
+from ket_amd._native import LIB_PATH, OP_DTYPE, SIGNATURES, RangeError
+from ket_amd.circuit import Distribution, make_prng, to_uint32
+
+import ketsv_oracle as O
+from helpers import build, random_calls
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HOST = {"id": G.Id, "h": G.H, "x": G.X, "y": G.Y, "z": G.Z, "s": G.S, "si": G.Si, "t": G.T, "ti": G.Ti,
+        "v": G.V, "vi": G.Vi, "r2": G.R2, "r4": G.R4, "r8": G.R8, "iswap": G.ISwap, "srswap": G.SrSwap}
+HOSTP = {"rx": G.Rx, "ry": G.Ry, "rz": G.Rz, "u1": G.U1, "u2": G.U2, "u3": G.U3, "gpi": G.Gpi, "gpi2": G.Gpi2,
+         "xx": G.Xx, "yy": G.Yy, "zz": G.Zz, "xy": G.Xy, "ms": G.Ms}
+NPAR = {"u2": 2, "u3": 3, "ms": 2}
+
+
+@pytest.mark.parametrize("name", list(HOST) + list(HOSTP))
+def test_host_matrices_bit_identical_to_oracle(name):
+    for trial in range(5):
+        ps = [1.3 * trial - 2.1, 0.77 * trial + 0.1, -0.4 * trial][:NPAR.get(name, 1 if name in HOSTP else 0)]
+        host = np.array(HOSTP[name](*ps) if name in HOSTP else HOST[name], dtype=complex)
+        orc = O.gate_matrix(name, ps).reshape(-1)
+        assert np.array_equal(host, orc), name
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("prune", [True, False])
+def test_tape_lowering_matches_named_semantics(seed, prune):
+    """Circuit -> tape (kinds + matrices) applied by the oracle equals the
+    oracle's own restatement of every Circuit method, bit for bit."""
+    n = 5
+    calls = random_calls(n, 80, seed=seed)
+    ops, mats = build(n, calls).tape()
+    a = O.zero_state(n)
+    O.apply_tape(n, a, ops, mats, prune=prune)
+    b = O.zero_state(n)
+    O.apply_named(n, b, calls, prune=prune)
+    assert np.array_equal(a, b)
+
+
+def test_tape_layout():
+    c = Circuit(4).h(0).cnot(0, 3).swap(1, 2).ccx(0, 1, 2).barrier().h(1)
+    ops, mats = c.tape()
+    assert ops.dtype == OP_DTYPE and ops.dtype.itemsize == 64
+    assert ops["kind"].tolist() == [0, 1, 2, 5, 9, 0]
+    assert ops["q"][1, :2].tolist() == [0, 3]
+    assert ops["mat"][0] == ops["mat"][5] == 0          # shared H matrix
+    assert ops["mat"][1] == -1
+    assert len(mats) == 8
+
+
+def test_builder_errors_and_immutability():
+    c = Circuit(3)
+    with pytest.raises(RangeError, match="qubit index 3 is out of range for a 3-qubit circuit"):
+        c.h(3)
+    with pytest.raises(RangeError):
+        c.cnot(-1, 0)
+    with pytest.raises(TypeError, match="control and target qubits must differ"):
+        c.crz(0.1, 2, 2)
+    with pytest.raises(TypeError, match="must be 4×4"):
+        c.unitary([[1, 0], [0, 1]], 0, 1)
+    d = c.h(0)
+    assert len(c) == 0 and len(d) == 1
+    assert len(d.x(1)) == 2 and len(d) == 1
+    with pytest.raises(TypeError, match="initialState"):
+        Circuit(2).h(0).simulate(initialState="012")
+
+
+def test_params_bind_compose_decompose():
+    a = Circuit(2).rx("theta", 0).ry("phi", 1).cu1("lam", 0, 1)
+    assert a.params == ["lam", "phi", "theta"]
+    with pytest.raises(TypeError, match="unbound"):
+        a.tape()
+    b = a.bind({"theta": 0.3, "phi": 0.2, "lam": 1.1})
+    assert b.params == []
+    with pytest.raises(TypeError, match="bind: unbound parameter 'phi'"):
+        a.bind({"theta": 0.3, "lam": 1.0})
+    ops, _ = b.tape()
+    assert ops["kind"].tolist() == [0, 0, 4]
+    bell = Circuit(2).h(0).cnot(0, 1)
+    c = Circuit(4).defineGate("bell", bell).gate("bell", 2, 3).gate("bell", 0, 1)
+    flat = c.decompose().ops
+    assert [op[0] for op in flat] == ["single", "cnot", "single", "cnot"]
+    assert flat[1][1:] == (2, 3) and flat[3][1:] == (0, 1)
+    with pytest.raises(TypeError, match="expects 2 qubit"):
+        c.gate("bell", 0)
+    with pytest.raises(TypeError, match="compose: qubit count mismatch"):
+        bell.compose(Circuit(3))
+    assert len(bell.compose(bell)) == 4
+
+
+def test_prng_and_seed_conversion():
+    for seed in (42, 1, 7, 123456789, 2 ** 31 + 5):
+        r = make_prng(seed)
+        assert [r() for _ in range(20)] == O.prng(seed, 20).tolist()
+    assert to_uint32(-1) == 2 ** 32 - 1
+    assert to_uint32(2.9) == 2
+    assert to_uint32(float("nan")) == 0
+    r0, r1 = make_prng(0), make_prng(1)
+    assert r0() == r1()
+
+
+def test_distribution_keys():
+    d = Distribution(3, 4, {1: 3, 6: 1})
+    assert d.probs == {"100": 0.75, "011": 0.25}
+    assert d.histogram == {"1": 0.75, "6": 0.25}
+    assert d.most == "100"
+    assert abs(d.entropy - (-(0.75 * math.log2(0.75) + 0.25 * math.log2(0.25)))) < 1e-15
+
+
+def test_workload_shapes():
+    assert len(workloads.random_layer_calls(12, 20)) == 830
+    assert len(workloads.random_layer_calls(30, 40)) == 4180
+    assert len(ket_amd.qft(30)) == 480
+    ans = ket_amd.efficientSU2(24, 24)
+    assert ans.paramCount == 1200
+    c = ans(workloads.ansatz_params(1200))
+    assert len(c) == 1200 + 24 * 23
+    assert len(workloads.tfim_hamiltonian(24)) == 47
+    inst = workloads.qaoa_instances(8)
+    assert all(14 <= n <= 22 for n, *_ in inst)
+
+
+def test_gradient_tracing():
+    from ket_amd.algorithms import trace
+    ra = ket_amd.realAmplitudes(4, 2)
+    flat, rec, exact, n = trace(ra, [0.1] * ra.paramCount)
+    assert exact and len(rec) == ra.paramCount and n == 4
+    edges = [(0, 1), (1, 2)]
+    qa = lambda p: ket_amd.qaoa(3, edges, p[:1], p[1:])
+    flat, rec, exact, n = trace(qa, [0.3, 0.2])
+    assert not exact                          # gamma on every edge, beta doubled
+    assert sorted({(r[1], r[2]) for r in rec}) == [(0, -1.0), (1, 2.0)]
+    bad = lambda p: Circuit(1).rx(math.cos(p[0]), 0)
+    assert trace(bad, [0.3]) is None
+
+
+def test_c_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "ketsv.h")).read()
+    declared = set(re.findall(r"^int (ksv_\w+)\(", hdr, re.M))
+    assert len(declared) >= 25
+    assert declared == {name for name, _, _ in SIGNATURES}
+    lib = ctypes.CDLL(LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_product_path_fails_loudly_without_device():
+    if ket_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        Circuit(2).h(0).statevector()

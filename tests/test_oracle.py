@@ -1,0 +1,320 @@
+"""The CPU oracle pinned against closed forms, invariants, brute force and
+the known answers of ket's own tests (src/gates.test.ts,
+src/circuit.test.ts).  Runs without a GPU."""
+import cmath
+import math
+
+import numpy as np
+import pytest
+
+import ketsv_oracle as O
+from helpers import random_calls
+
+S2 = 1 / math.sqrt(2)
+ALL1 = "id h x y z s si t ti v vi r2 r4 r8".split()
+PAR1 = {"rx": 1, "ry": 1, "rz": 1, "u1": 1, "u2": 2, "u3": 3, "gpi": 1, "gpi2": 1}
+PAR2 = {"xx": 1, "yy": 1, "zz": 1, "xy": 1, "ms": 2}
+
+
+def M(name, *p):
+    return O.gate_matrix(name, p)
+
+
+def close(a, b, tol=1e-11):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) < tol
+
+
+# ---- gate matrices (src/gates.test.ts:105-455) -----------------------------------
+@pytest.mark.parametrize("name", ALL1 + ["iswap", "srswap"])
+def test_static_gates_unitary(name):
+    m = M(name)
+    assert close(m.conj().T @ m, np.eye(len(m)))
+
+
+@pytest.mark.parametrize("name", list(PAR1) + list(PAR2))
+def test_parametric_gates_unitary(name):
+    k = PAR1.get(name, PAR2.get(name))
+    m = M(name, *[1.23, -0.7, 2.9][:k])
+    assert close(m.conj().T @ m, np.eye(len(m)))
+
+
+def test_gate_identities():
+    X, Y, Z, H = M("x"), M("y"), M("z"), M("h")
+    assert close(M("u3", math.pi, 0, math.pi), X)
+    assert close(M("u3", math.pi / 2, 0, math.pi), H)
+    assert close(M("u3", 0.81, 0, 0), M("ry", 0.81))
+    assert close(M("u3", 0, 0, 0.37), M("u1", 0.37))
+    assert close(M("u2", 0, math.pi), H)
+    assert close(M("u1", math.pi / 2), M("s"))
+    assert close(M("u1", math.pi / 4), M("t"))
+    assert close(M("u1", math.pi), Z)
+    assert close(M("v") @ M("v"), X)
+    assert close(M("s") @ M("s"), Z)
+    assert close(M("t") @ M("t"), M("s"))
+    assert close(M("gpi", 0), X)
+    assert close(M("gpi", math.pi / 2), Y)
+    assert close(M("gpi2", 0), M("rx", math.pi / 2))
+    assert close(M("gpi2", math.pi / 2), M("ry", math.pi / 2))
+    assert close(M("ms", 0, 0), M("xx", math.pi / 2))
+    assert close(M("ms", math.pi / 2, math.pi / 2), M("yy", math.pi / 2))
+    assert close(M("iswap"), np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]]))
+    assert close(M("srswap") @ M("srswap"), M("iswap"))
+    # Rz(theta) = e^{-i theta/2} U1(theta); R2 = Rz(pi/2)
+    assert close(M("rz", 0.9), cmath.exp(-0.45j) * M("u1", 0.9))
+    assert close(M("r2"), M("rz", math.pi / 2))
+    # ZZ(theta) = CNOT (I (x) Rz(theta)) CNOT, first qubit = MSB
+    cn = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]])
+    assert close(M("zz", 0.77), cn @ np.kron(np.eye(2), M("rz", 0.77)) @ cn)
+    # XX(theta) = exp(-i theta/2 X(x)X)
+    xx = np.kron(X, X)
+    assert close(M("xx", 0.6), math.cos(0.3) * np.eye(4) - 1j * math.sin(0.3) * xx)
+    yy = np.kron(Y, Y)
+    assert close(M("yy", 0.6), math.cos(0.3) * np.eye(4) - 1j * math.sin(0.3) * yy)
+
+
+def test_gate_exact_entries():
+    """Exact values quoted by src/gates.test.ts."""
+    h = M("h")
+    assert h[0, 0] == S2 and h[1, 1] == -S2
+    assert M("t")[1, 1] == complex(S2, S2)
+    rx = M("rx", math.pi)
+    assert abs(rx[0, 1] - (-1j)) < 1e-15 and abs(rx[0, 0]) < 1e-15
+
+
+# ---- brute force: full 2^n matrices built independently with numpy ---------------
+def embed(n, m, qs):
+    """Full operator of a 2^k matrix on qubits qs (qs[0] = MSB of the local index)."""
+    N = 1 << n
+    k = len(qs)
+    U = np.zeros((N, N), dtype=complex)
+    for col in range(N):
+        lc = 0
+        for b, q in enumerate(qs):
+            lc |= ((col >> q) & 1) << (k - 1 - b)
+        base = col
+        for q in qs:
+            base &= ~(1 << q)
+        for lr in range(1 << k):
+            row = base
+            for b, q in enumerate(qs):
+                if (lr >> (k - 1 - b)) & 1:
+                    row |= 1 << q
+            U[row, col] += m[lr, lc]
+    return U
+
+
+def controlled(m):
+    out = np.eye(2 * len(m), dtype=complex)
+    out[len(m):, len(m):] = m
+    return out
+
+
+CTRL = {"cy": "y", "cz": "z", "ch": "h", "crx": "rx", "cry": "ry", "crz": "rz", "cr2": "r2", "cr4": "r4",
+        "cr8": "r8", "cu1": "u1", "cu2": "u2", "cu3": "u3", "cs": "s", "ct": "t", "csdg": "si", "ctdg": "ti",
+        "csrn": "v"}
+ALIAS = {"sdg": "si", "tdg": "ti", "srn": "v", "srndg": "vi", "vz": "rz", "p": "u1"}
+
+
+def full_op(n, name, args):
+    X = M("x")
+    if name == "unitary":
+        return embed(n, np.asarray(args[0]), args[1:])
+    k = O.NQ[name]
+    ps, qs = args[:len(args) - k], args[len(args) - k:]
+    if name in ("cnot", "cx"):
+        return embed(n, controlled(X), qs)
+    if name == "swap":
+        return embed(n, np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]]), qs)
+    if name == "ccx":
+        return embed(n, controlled(controlled(X)), qs)
+    if name == "cswap":
+        return embed(n, controlled(np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]])), qs)
+    if name == "csrswap":
+        s = 1 / math.sqrt(2)
+        return embed(n, controlled(np.array([[1, 0, 0, 0], [0, s, 1j * s, 0], [0, 1j * s, s, 0], [0, 0, 0, 1]])), qs)
+    if name in CTRL:
+        return embed(n, controlled(M(CTRL[name], *ps)), qs)
+    return embed(n, M(ALIAS.get(name, name), *ps), qs)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_brute_force(seed):
+    n = 4
+    calls = random_calls(n, 40, seed=seed)
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, calls, prune=False)
+    want = np.zeros(1 << n, dtype=complex)
+    want[0] = 1
+    for name, args in calls:
+        want = full_op(n, name, args) @ want
+    assert close(psi, want, 1e-12)
+
+
+# ---- closed forms --------------------------------------------------------------
+def qft_calls(n):
+    calls = []
+    for j in range(n - 1, -1, -1):
+        calls.append(("h", (j,)))
+        for k in range(j - 1, -1, -1):
+            calls.append(("cu1", (math.pi / 2 ** (j - k), k, j)))
+    for i in range(n // 2):
+        calls.append(("swap", (i, n - 1 - i)))
+    return calls
+
+
+@pytest.mark.parametrize("n", [3, 4, 6])
+def test_qft_is_scaled_inverse_fft(n):
+    rs = np.random.RandomState(n)
+    psi0 = rs.normal(size=1 << n) + 1j * rs.normal(size=1 << n)
+    psi0 /= np.linalg.norm(psi0)
+    psi = psi0.copy()
+    O.apply_named(n, psi, qft_calls(n), prune=False)
+    assert close(psi, math.sqrt(1 << n) * np.fft.ifft(psi0), 1e-12)
+
+
+def test_qft3_phases():
+    """src/circuit.test.ts:3917-3929: QFT|1> amplitudes e^{2 pi i k / 8}/sqrt 8."""
+    psi = O.zero_state(3, 1)
+    O.apply_named(3, psi, qft_calls(3), prune=False)
+    for k in range(8):
+        assert abs(psi[k] - cmath.exp(2j * math.pi * k / 8) / math.sqrt(8)) < 1e-12
+
+
+def test_bell_and_ghz():
+    psi = O.zero_state(2)
+    O.apply_named(2, psi, [("h", (0,)), ("cnot", (0, 1))])
+    idx, p = O.probabilities(2, psi)
+    assert idx.tolist() == [0, 3] and close(p, [0.5, 0.5], 1e-15)
+    n = 20
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, [("h", (0,))] + [("cnot", (i, i + 1)) for i in range(n - 1)])
+    idx, p = O.probabilities(n, psi)
+    assert idx.tolist() == [0, (1 << n) - 1]
+
+
+def test_prune_semantics():
+    """accumulate() drops |a|^2 < 1e-14 (src/statevector.ts:19-27)."""
+    psi = O.zero_state(1)
+    O.apply_named(1, psi, [("rx", (1e-8, 0))], prune=True)
+    assert psi[1] == 0
+    psi = O.zero_state(1)
+    O.apply_named(1, psi, [("rx", (1e-8, 0))], prune=False)
+    assert abs(psi[1]) > 0
+    psi = O.zero_state(1)
+    O.apply_named(1, psi, [("rx", (1e-6, 0))], prune=True)
+    assert abs(psi[1]) > 0
+
+
+# ---- seeded sampling --------------------------------------------------------------
+def test_prng_is_xorshift32():
+    def ref(seed, k):
+        s = (seed & 0xFFFFFFFF) or 1
+        out = []
+        for _ in range(k):
+            s ^= (s << 13) & 0xFFFFFFFF
+            s ^= s >> 17
+            s ^= (s << 5) & 0xFFFFFFFF
+            out.append(s / 2 ** 32)
+        return out
+
+    for seed in (1, 42, 7, 2 ** 32 - 1, 0):
+        assert O.prng(seed, 50).tolist() == ref(seed, 50)
+    # seed 0 behaves as seed 1 ((seed >>> 0) || 1)
+    assert O.prng(0, 5).tolist() == O.prng(1, 5).tolist()
+
+
+def test_sample_cdf_rules():
+    psi = np.zeros(8, dtype=complex)
+    psi[1] = math.sqrt(0.25)
+    psi[6] = math.sqrt(0.75)
+    r = np.array([0.1, 0.25, 0.2500001, 0.9, 0.9999999])
+    assert O.sample(3, psi, r).tolist() == [1, 1, 6, 6, 6]
+    # total below 1: r beyond it lands on the last entry
+    psi[6] = math.sqrt(0.5)
+    assert O.sample(3, psi, np.array([0.9]))[0] == 6
+
+
+def test_sample_histogram_matches_probabilities():
+    n = 5
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, random_calls(n, 30, seed=3))
+    r = O.prng(42, 200000)
+    s = O.sample(n, psi, r)
+    counts = np.bincount(s.astype(np.int64), minlength=1 << n)
+    p = np.abs(psi) ** 2
+    exp = p * len(r)
+    m = exp > 5
+    chi2 = np.sum((counts[m] - exp[m]) ** 2 / exp[m])
+    assert chi2 < 3 * m.sum()
+
+
+# ---- read-outs --------------------------------------------------------------------
+def test_expectation_known_answers():
+    """src/circuit.ts:3720-3726 examples and src/circuit.test.ts:7064-7097."""
+    psi = O.zero_state(1)
+    assert O.expectation(1, psi, "Z") == 1
+    psi = O.zero_state(1, 1)
+    assert O.expectation(1, psi, "Z") == -1
+    psi = O.zero_state(1)
+    O.apply_named(1, psi, [("h", (0,))])
+    assert abs(O.expectation(1, psi, "X") - 1) < 1e-12
+    bell = O.zero_state(2)
+    O.apply_named(2, bell, [("h", (0,)), ("cnot", (0, 1))])
+    assert abs(O.expectation(2, bell, "ZZ") - 1) < 1e-12
+    assert abs(O.expectation(2, bell, "XX") - 1) < 1e-12
+    assert abs(O.expectation(2, bell, "YY") + 1) < 1e-12
+    for th in (0.3, 1.7, -2.2):
+        psi = O.zero_state(1)
+        O.apply_named(1, psi, [("ry", (th, 0))])
+        assert abs(O.expectation(1, psi, "Z") - math.cos(th)) < 1e-12
+        assert abs(O.expectation(1, psi, "X") - math.sin(th)) < 1e-12
+        assert abs(O.expect_direct(1, psi, 1, 0) - math.sin(th)) < 1e-12
+
+
+def test_direct_expectation_equals_rotation_method():
+    n = 5
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, random_calls(n, 40, seed=8), prune=False)
+    for pauli in ("XYZIX", "ZZIII", "YYYYY", "IXIXI"):
+        x = sum(1 << q for q, ch in enumerate(pauli) if ch in "XY")
+        z = sum(1 << q for q, ch in enumerate(pauli) if ch in "YZ")
+        P = np.eye(1, dtype=complex)
+        for ch in reversed(pauli):
+            P = np.kron(P, {"I": np.eye(2), "X": M("x"), "Y": M("y"), "Z": M("z")}[ch])
+        want = np.vdot(psi, P @ psi).real
+        assert abs(O.expect_direct(n, psi, x, z) - want) < 1e-12
+        assert abs(O.expectation(n, psi, pauli) - want) < 1e-10
+
+
+def test_vqe_convention_and_gradient_closed_forms():
+    """vqe's ops[0] acts on qubit n-1; d<Z>/dtheta = -sin(theta) for ry
+    (src/circuit.test.ts:4165-4235)."""
+    n = 2
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, [("x", (0,))])
+    assert O.vqe(n, psi, [{"coeff": 1.0, "ops": "IZ"}]) == -1
+    assert O.vqe(n, psi, [{"coeff": 1.0, "ops": "ZI"}]) == 1
+    assert O.vqe(n, psi, [{"coeff": 2.5, "ops": "II"}, {"coeff": 1e-16, "ops": "ZZ"}]) == 2.5
+    for th in (0.4, 1.3):
+        def e(t):
+            s = O.zero_state(1)
+            O.apply_named(1, s, [("ry", (t, 0))])
+            return O.vqe(1, s, [{"coeff": 1.0, "ops": "Z"}])
+        psr = 0.5 * (e(th + math.pi / 2) - e(th - math.pi / 2))
+        assert abs(psr + math.sin(th)) < 1e-12
+
+
+def test_marginals_bloch_collapse():
+    n = 3
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, [("h", (0,)), ("x", (2,))])
+    assert close(O.marginals(n, psi), [0.5, 0, 1], 1e-12)
+    th, ph = O.bloch(n, psi, 0)
+    assert abs(th - math.pi / 2) < 1e-12 and abs(ph) < 1e-12
+    th, ph = O.bloch(n, psi, 2)
+    # rz = -0.9999999999999998 here; acos amplifies that rounding to ~2e-8
+    assert abs(th - math.pi) < 1e-7
+    bell = O.zero_state(2)
+    O.apply_named(2, bell, [("h", (0,)), ("cnot", (0, 1))])
+    out = O.collapse(2, bell, 0, 0.3)     # p1 = 0.5 > 0.3 -> outcome 1
+    assert out == 1 and abs(bell[3] - 1) < 1e-12 and bell[0] == 0
