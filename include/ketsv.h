@@ -133,6 +133,11 @@ int ksv_program_create(ksv_state* s, const ksv_op* ops, size_t nops, const doubl
 int ksv_program_run(ksv_state* s, ksv_program* p);
 int ksv_program_info(const ksv_program* p, ksv_program_summary* out);
 int ksv_program_destroy(ksv_program* p);
+/* Schedule a tape without a device (dry run): the pass count and op count
+ * ksv_program_create would produce for a state of n qubits with the given
+ * tile width (0 = default), fusion and pruning.  Used by the autotuner. */
+int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+             int tile, int fuse, int prune, ksv_program_summary* out);
 
 /* ---- read-out ---------------------------------------------------------------- */
 /* Dense amplitudes [offset, offset+count) as interleaved doubles (statevector(),

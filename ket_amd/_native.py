@@ -55,6 +55,7 @@ SIGNATURES = [
     ("ksv_program_run", c_int, [_P, _P]),
     ("ksv_program_info", c_int, [_P, POINTER(KsvProgramSummary)]),
     ("ksv_program_destroy", c_int, [_P]),
+    ("ksv_plan", c_int, [c_int, c_int, _P, c_size_t, _P, c_size_t, c_int, c_int, c_int, POINTER(KsvProgramSummary)]),
     ("ksv_get_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
     ("ksv_set_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
     ("ksv_amplitude", c_int, [_P, c_uint64, _P]),
@@ -113,6 +114,15 @@ def device_count():
     c = c_int(0)
     lib().ksv_device_count(ctypes.byref(c))
     return c.value
+
+
+def plan(n, tape, precision="f64", tile=0, fuse=False, prune=True):
+    """Dry-run schedule of a tape (no device needed)."""
+    ops, mats = tape
+    s = KsvProgramSummary()
+    check(lib().ksv_plan(int(n), KSV_F64 if precision == "f64" else KSV_F32, ptr(ops), len(ops), ptr(mats),
+                         len(mats), int(tile), 1 if fuse else 0, 1 if prune else 0, ctypes.byref(s)))
+    return {f: getattr(s, f) for f, _ in KsvProgramSummary._fields_}
 
 
 def ptr(a):

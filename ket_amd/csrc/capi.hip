@@ -259,6 +259,29 @@ int ksv_program_destroy(ksv_program* p) {
     return KSV_OK;
 }
 
+int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int tile,
+             int fuse, int prune, ksv_program_summary* out) {
+    if (!out) return fail(KSV_ERR_TYPE, "null output pointer");
+    if (n < 1 || n > MAXN) return fail(KSV_ERR_RANGE, "qubit count out of range");
+    if (nops && !ops) return fail(KSV_ERR_TYPE, "null op tape");
+    std::vector<LOp> lops;
+    std::string err;
+    int rc = lower_tape(n, ops, nops, mats, nmats, lops, err);
+    if (rc) return fail(rc, err);
+    Program p;
+    p.n = n;
+    p.precision = precision;
+    p.prune = prune;
+    p.tile = tile > 0 ? tile : (precision == KSV_F64 ? 12 : 13);
+    schedule(n, p.tile, fuse && !prune, lops, p);
+    out->ops = nops;
+    out->passes = p.steps.size();
+    out->device_ops = p.dops.size() + p.big_ops.size();
+    out->tile_qubits = p.tile;
+    out->reserved = 0;
+    return KSV_OK;
+}
+
 int ksv_apply(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats) {
     if (!s) return fail(KSV_ERR_HANDLE, "null state");
     ksv_program* p = nullptr;
