@@ -9,35 +9,9 @@
 // All loops over per-op bit lists are unrolled to their static maximum with
 // a guard, so kernel arguments and op fields stay in scalar registers
 // (a dynamically indexed argument array would be spilled to scratch).
-#include "ksv_internal.h"
+#include "kcommon.h"
 
 namespace ksv {
-
-template <typename V>
-__device__ __forceinline__ V cmul(V a, V b) {
-    V r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
-    return r;
-}
-template <typename V>
-__device__ __forceinline__ V cmadd2(V m0, V x0, V m1, V x1) {
-    V r;
-    r.x = (m0.x * x0.x - m0.y * x0.y) + (m1.x * x1.x - m1.y * x1.y);
-    r.y = (m0.x * x0.y + m0.y * x0.x) + (m1.x * x1.y + m1.y * x1.x);
-    return r;
-}
-template <typename V>
-__device__ __forceinline__ V prune_amp(V v, int prune) {
-    if (prune && (v.x * v.x + v.y * v.y) < 1e-14) { v.x = 0; v.y = 0; }
-    return v;
-}
-__device__ __forceinline__ uint32_t ins0(uint32_t p, int q) {
-    return ((p >> q) << (q + 1)) | (p & ((1u << q) - 1));
-}
-__device__ __forceinline__ uint64_t ins0_64(uint64_t p, int q) {
-    return ((p >> q) << (q + 1)) | (p & ((1ull << q) - 1));
-}
 
 // Element index of the e-th element of an op: insert zero bits at the op's
 // ascending positions, then set the local control bits.

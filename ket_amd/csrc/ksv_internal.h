@@ -44,6 +44,51 @@ constexpr int MK_MAX = 4;          // widest dense in-pass matrix
 constexpr int DIAG_MAX = 6;        // widest in-pass diagonal
 constexpr int BIG_MAX = 12;        // widest standalone unitary
 
+// ---------------------------------------------------------------------------
+// Register-tile ("wave") pass: one wavefront owns 2^(6+r) amplitudes, lane l
+// register j holding the amplitude whose tile-local lane bits are l and
+// register-slot bits are j.  Ops act on register slots; a qubit on a lane
+// bit is brought into a slot by W_LSWAP (cross-lane exchange).
+// ---------------------------------------------------------------------------
+enum WKind : uint8_t {
+    W_M1 = 0,     // 2x2 on slot a
+    W_M2 = 1,     // 4x4 on slots a (MSB), b
+    W_PHASE = 2,  // multiply by mats[mat] where every control bit is 1
+    W_DIAG = 3,   // diagonal over k qubits at loc[]
+    W_PERMX = 4,  // X on slot a (controlled)
+    W_SWAPR = 5,  // exchange slots a, b (controlled)
+    W_LSWAP = 6,  // exchange lane bit a with slot b
+};
+
+struct WOp {
+    uint8_t kind;
+    uint8_t a, b;
+    uint8_t k;
+    uint8_t loc[6];    // W_DIAG: 0..5 lane bit, 8+s slot, 16+g global bit g; loc[0] = MSB of the row
+    uint16_t pad;
+    uint32_t lctl;     // lane-bit controls (mask of lane index bits)
+    uint32_t rctl;     // slot controls (mask of register index bits)
+    int32_t mat;       // complex offset into the program matrix array
+    uint64_t gctl;     // global controls
+};
+static_assert(sizeof(WOp) == 32, "WOp layout");
+
+constexpr int WLANE_BITS = 6;
+
+// Byte i of a packed word holds the state bit of lane bit / register slot /
+// tile bit i (arrays inside a by-value kernel argument would be copied to
+// scratch memory, packed words stay in scalar registers).
+struct WaveArgs {
+    void* psi;
+    const WOp* ops;
+    const void* mats;
+    int nops;
+    int k;               // tile qubits = 6 + r
+    uint64_t lane_in, lane_out;   // state bit of each lane bit at load / store
+    uint64_t reg_in, reg_out;     // state bit of each register slot at load / store
+    uint64_t tq_lo, tq_hi;        // tile state bits ascending (to build the base)
+};
+
 struct PassArgs {
     void* psi;
     const DOp* ops;
@@ -73,10 +118,14 @@ struct LOp {
 
 struct Step {
     bool big = false;        // standalone unitary step
+    bool wave = false;       // register-tile pass (WOps) instead of an LDS pass (DOps)
     int k = 0;               // tile qubits
     int8_t tq[16] = {0};
-    int op_begin = 0, op_count = 0;   // DOps of this pass
+    int op_begin = 0, op_count = 0;   // DOps (LDS pass) or WOps (wave pass) of this step
     int big_op = -1;         // index into Program::big_ops
+    int rlog = 0;            // wave pass: log2 registers per lane
+    int8_t lane_in[6] = {0}, lane_out[6] = {0};
+    int8_t reg_in[8] = {0}, reg_out[8] = {0};
 };
 
 struct BigOp {
@@ -92,7 +141,10 @@ struct Program {
     uint64_t tape_ops = 0;
     std::vector<Step> steps;
     std::vector<DOp> dops;
+    std::vector<WOp> wops;
     std::vector<BigOp> big_ops;
+    int rlog = 0;            // > 0: wave passes with 2^rlog registers per lane
+    WOp* d_wops = nullptr;
     std::vector<cd> mats;
     // device copies
     DOp* d_ops = nullptr;
@@ -104,10 +156,12 @@ struct Program {
 int lower_tape(int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                std::vector<LOp>& out, std::string& err);
 void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog);
+void schedule_wave(int n, int rlog, bool fuse, std::vector<LOp>& lops, Program& prog);
 
 // kernels (k_pass.hip / k_reduce.hip)
 hipError_t launch_pass(const Program& p, const Step& s, void* psi, hipStream_t st);
 hipError_t launch_big(const Program& p, const Step& s, void* psi, int prune, hipStream_t st);
+hipError_t launch_wave(const Program& p, const Step& s, void* psi, hipStream_t st);
 
 struct Scratch {
     double* partial = nullptr;     // reduction partials

@@ -342,8 +342,67 @@ static void emit_dop(const LOp& o, const int* pos, Program& prog) {
 
 static int popc(uint64_t x) { return __builtin_popcountll(x); }
 
-void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog) {
-    if (fuse) fuse_1q(n, lops);
+// Choose the ops of one pass: scan the remaining ops in order and take every
+// op whose qubits no earlier skipped op touches (so program order between
+// dependent ops is kept) and whose must-be-local qubits fit the tile.
+// With `strict` (ket's per-gate pruning) the pass ends at the first op that
+// does not fit: pruning after every gate makes even commuting gates
+// order-sensitive, so no op may move ahead of another.
+// Returns the tile mask (forced bits included, not yet padded).
+template <typename Allowed>
+static uint64_t pick_pass(const std::vector<LOp>& lops, std::vector<int>& rem, int K, uint64_t forced,
+                          uint64_t all, Allowed allowed, std::vector<int>& chosen, std::vector<int>& rest,
+                          bool strict) {
+    uint64_t T = forced, blocked = 0;
+    chosen.clear();
+    rest.clear();
+    size_t i = 0;
+    for (; i < rem.size(); i++) {
+        const LOp& o = lops[rem[i]];
+        if (!allowed(o) || (o.touch & blocked)) { blocked |= o.touch; rest.push_back(rem[i]); }
+        else {
+            uint64_t nt = T | o.need;
+            if (popc(nt) <= K) { T = nt; chosen.push_back(rem[i]); }
+            else { blocked |= o.touch; rest.push_back(rem[i]); }
+        }
+        if (strict && blocked) blocked = all;
+        if ((blocked & all) == all) { i++; break; }
+    }
+    for (; i < rem.size(); i++) rest.push_back(rem[i]);
+    return T;
+}
+
+static void emit_big(const LOp& o, Program& prog) {
+    Step s;
+    s.big = true;
+    s.big_op = (int)prog.big_ops.size();
+    BigOp b;
+    b.tg = o.tg;
+    b.mat = (int)prog.mats.size();
+    prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+    prog.big_ops.push_back(b);
+    prog.steps.push_back(s);
+}
+
+// An LDS pass over tile T holding the ops `idx`.
+static void emit_lds_pass(int n, int K, uint64_t T, const std::vector<LOp>& lops, const std::vector<int>& idx,
+                          Program& prog) {
+    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
+    Step s;
+    s.k = K;
+    int pos[64];
+    for (int q = 0; q < 64; q++) pos[q] = -1;
+    int j = 0;
+    for (int q = 0; q < n; q++)
+        if (T >> q & 1) { s.tq[j] = (int8_t)q; pos[q] = j; j++; }
+    s.op_begin = (int)prog.dops.size();
+    for (int id : idx) emit_dop(lops[id], pos, prog);
+    s.op_count = (int)prog.dops.size() - s.op_begin;
+    prog.steps.push_back(s);
+}
+
+static void schedule_lds(int n, int tile, std::vector<LOp>& lops, Program& prog) {
+    const bool strict = prog.prune != 0;
     const int K = std::min(n, tile);
     const int c = std::min(4, K);
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
@@ -354,46 +413,198 @@ void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog)
     while (!rem.empty()) {
         const LOp& first = lops[rem[0]];
         if (first.big) {
-            Step s;
-            s.big = true;
-            s.big_op = (int)prog.big_ops.size();
-            BigOp b;
-            b.tg = first.tg;
-            b.mat = (int)prog.mats.size();
-            prog.mats.insert(prog.mats.end(), first.m.begin(), first.m.end());
-            prog.big_ops.push_back(b);
-            prog.steps.push_back(s);
+            emit_big(first, prog);
             rem.erase(rem.begin());
             continue;
         }
-        uint64_t T = forced, blocked = 0;
-        chosen.clear();
-        rest.clear();
-        size_t i = 0;
-        for (; i < rem.size(); i++) {
-            const LOp& o = lops[rem[i]];
-            if (o.big || (o.touch & blocked)) { blocked |= o.touch; rest.push_back(rem[i]); }
-            else {
-                uint64_t nt = T | o.need;
-                if (popc(nt) <= K) { T = nt; chosen.push_back(rem[i]); }
-                else { blocked |= o.touch; rest.push_back(rem[i]); }
+        uint64_t T = pick_pass(lops, rem, K, forced, all, [](const LOp& o) { return !o.big; }, chosen, rest,
+                               strict);
+        emit_lds_pass(n, K, T, lops, chosen, prog);
+        rem.swap(rest);
+    }
+}
+
+void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog) {
+    if (fuse) fuse_1q(n, lops);
+    schedule_lds(n, tile, lops, prog);
+}
+
+// --------------------------------------------------------------------------
+// Register-tile passes.
+// --------------------------------------------------------------------------
+static bool wave_ok(const LOp& o) {
+    if (o.big) return false;
+    if (o.cls == L_MAT && o.tg.size() > 2) return false;
+    if (o.cls == L_DIAG && o.tg.size() > 2) return false;
+    return true;
+}
+
+// where[] codes: -1 outside the tile, 0..5 lane bit, 8+s register slot
+static void wave_pass(int n, int K, int rlog, uint64_t T, const std::vector<LOp>& lops,
+                      const std::vector<int>& idx, Program& prog) {
+    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
+    Step s;
+    s.wave = true;
+    s.k = K;
+    s.rlog = rlog;
+    int where[64];
+    for (int q = 0; q < 64; q++) where[q] = -1;
+    int j = 0;
+    for (int q = 0; q < n; q++)
+        if (T >> q & 1) s.tq[j++] = (int8_t)q;
+    // targets that must sit in register slots, per op
+    std::vector<std::vector<int>> tgs(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        const LOp& o = lops[idx[i]];
+        if (o.cls != L_DIAG) tgs[i] = o.tg;
+    }
+    // initial placement: state bits 0..2 on lane bits 0..2; the earliest
+    // targets in the slots; the rest on lane bits 3..5
+    for (int b = 0; b < 3; b++) where[b] = b;
+    std::vector<int> order;
+    for (size_t i = 0; i < idx.size(); i++)
+        for (int q : tgs[i])
+            if (q > 2 && std::find(order.begin(), order.end(), q) == order.end()) order.push_back(q);
+    for (int q = 3; q < n; q++)
+        if ((T >> q & 1) && std::find(order.begin(), order.end(), q) == order.end()) order.push_back(q);
+    int slot_q[8], lane_q[6] = {0, 1, 2, -1, -1, -1};
+    for (int i = 0; i < rlog; i++) { slot_q[i] = order[i]; where[order[i]] = 8 + i; }
+    for (int i = 0; i < 3; i++) { lane_q[3 + i] = order[rlog + i]; where[order[rlog + i]] = 3 + i; }
+    for (int b = 0; b < 6; b++) s.lane_in[b] = (int8_t)lane_q[b];
+    for (int i = 0; i < rlog; i++) s.reg_in[i] = (int8_t)slot_q[i];
+
+    s.op_begin = (int)prog.wops.size();
+    auto next_use = [&](int q, size_t from) -> size_t {
+        for (size_t i = from; i < idx.size(); i++)
+            for (int t : tgs[i])
+                if (t == q) return i;
+        return (size_t)-1;
+    };
+    auto lswap = [&](int lanebit, int slot) {
+        WOp w;
+        std::memset(&w, 0, sizeof w);
+        w.kind = W_LSWAP;
+        w.a = (uint8_t)lanebit;
+        w.b = (uint8_t)slot;
+        prog.wops.push_back(w);
+        int ql = lane_q[lanebit], qs = slot_q[slot];
+        lane_q[lanebit] = qs;
+        slot_q[slot] = ql;
+        where[qs] = lanebit;
+        where[ql] = 8 + slot;
+    };
+    for (size_t i = 0; i < idx.size(); i++) {
+        const LOp& o = lops[idx[i]];
+        for (int q : tgs[i]) {
+            if (where[q] >= 8) continue;
+            // evict the slot whose qubit is needed furthest ahead
+            int best = -1;
+            size_t bestd = 0;
+            for (int sl = 0; sl < rlog; sl++) {
+                int qq = slot_q[sl];
+                if (std::find(tgs[i].begin(), tgs[i].end(), qq) != tgs[i].end()) continue;
+                size_t d = next_use(qq, i);
+                if (best < 0 || d > bestd) { best = sl; bestd = d; }
             }
-            if ((blocked & all) == all) { i++; break; }
+            lswap(where[q], best);
         }
-        for (; i < rem.size(); i++) rest.push_back(rem[i]);
-        // pad the tile with the lowest unused qubits (longer contiguous runs)
-        for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
-        Step s;
-        s.k = K;
-        int pos[64];
-        for (int q = 0; q < 64; q++) pos[q] = -1;
-        int j = 0;
-        for (int q = 0; q < n; q++)
-            if (T >> q & 1) { s.tq[j] = (int8_t)q; pos[q] = j; j++; }
-        s.op_begin = (int)prog.dops.size();
-        for (int idx : chosen) emit_dop(lops[idx], pos, prog);
-        s.op_count = (int)prog.dops.size() - s.op_begin;
-        prog.steps.push_back(s);
+        WOp w;
+        std::memset(&w, 0, sizeof w);
+        for (int c : o.ctl) {
+            int wc = where[c];
+            if (wc < 0) w.gctl |= 1ull << c;
+            else if (wc < 8) w.lctl |= 1u << wc;
+            else w.rctl |= 1u << (wc - 8);
+        }
+        w.mat = (int32_t)prog.mats.size();
+        switch (o.cls) {
+        case L_MAT:
+            if (o.tg.size() == 1) {
+                w.kind = W_M1;
+                w.a = (uint8_t)(where[o.tg[0]] - 8);
+                prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+            } else {
+                w.kind = W_M2;
+                int sa = where[o.tg[0]] - 8, sb = where[o.tg[1]] - 8;
+                if (sa > sb) {
+                    prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+                    w.a = (uint8_t)sa;
+                    w.b = (uint8_t)sb;
+                } else {
+                    static const int perm[4] = {0, 2, 1, 3};
+                    for (int r = 0; r < 4; r++)
+                        for (int c = 0; c < 4; c++) prog.mats.push_back(o.m[perm[r] * 4 + perm[c]]);
+                    w.a = (uint8_t)sb;
+                    w.b = (uint8_t)sa;
+                }
+            }
+            break;
+        case L_DIAG:
+            if (o.tg.empty()) {
+                w.kind = W_PHASE;
+            } else {
+                w.kind = W_DIAG;
+                w.k = (uint8_t)o.tg.size();
+                for (size_t t = 0; t < o.tg.size(); t++) {
+                    int q = o.tg[t], wq = where[q];
+                    w.loc[t] = (uint8_t)(wq < 0 ? 16 + q : wq);
+                }
+            }
+            prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+            break;
+        case L_PERMX:
+            w.kind = W_PERMX;
+            w.a = (uint8_t)(where[o.tg[0]] - 8);
+            break;
+        case L_SWAP: {
+            w.kind = W_SWAPR;
+            int sa = where[o.tg[0]] - 8, sb = where[o.tg[1]] - 8;
+            w.a = (uint8_t)std::max(sa, sb);
+            w.b = (uint8_t)std::min(sa, sb);
+            break;
+        }
+        }
+        prog.wops.push_back(w);
+    }
+    // bring state bits 0..2 back to lane bits 0..2 for a coalesced store
+    for (int p = 0; p < 3; p++)
+        if (where[p] >= 8) lswap(p, where[p] - 8);
+    for (int b = 0; b < 6; b++) s.lane_out[b] = (int8_t)lane_q[b];
+    for (int i = 0; i < rlog; i++) s.reg_out[i] = (int8_t)slot_q[i];
+    s.op_count = (int)prog.wops.size() - s.op_begin;
+    prog.steps.push_back(s);
+}
+
+void schedule_wave(int n, int rlog, bool fuse, std::vector<LOp>& lops, Program& prog) {
+    if (fuse) fuse_1q(n, lops);
+    const int K = WLANE_BITS + rlog;
+    if (n < K + 2) {
+        schedule_lds(n, prog.tile, lops, prog);
+        return;
+    }
+    prog.rlog = rlog;
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    const uint64_t forced = 7;
+    const int KL = std::min(n, prog.tile);
+    std::vector<int> rem(lops.size());
+    for (size_t i = 0; i < lops.size(); i++) rem[i] = (int)i;
+    std::vector<int> chosen, rest;
+    while (!rem.empty()) {
+        const LOp& first = lops[rem[0]];
+        if (first.big) {
+            emit_big(first, prog);
+            rem.erase(rem.begin());
+            continue;
+        }
+        if (!wave_ok(first)) {
+            // a dense 3- or 4-qubit matrix or a wide diagonal: one LDS pass of its own
+            uint64_t T = ((1ull << std::min(4, KL)) - 1) | first.need;
+            emit_lds_pass(n, KL, T, lops, {rem[0]}, prog);
+            rem.erase(rem.begin());
+            continue;
+        }
+        uint64_t T = pick_pass(lops, rem, K, forced, all, wave_ok, chosen, rest, prog.prune != 0);
+        wave_pass(n, K, rlog, T, lops, chosen, prog);
         rem.swap(rest);
     }
 }

@@ -20,9 +20,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False):
+def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", wave_regs=0):
     c = build(n, calls)
-    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse)
+    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse, kernel=kernel, wave_regs=wave_regs)
     if init:
         st.init_basis(init)
     st.apply(c.tape())
@@ -66,11 +66,31 @@ def test_random_circuits_every_op(n, prune):
 
 @pytest.mark.parametrize("n,tile", [(14, 8), (15, 10), (16, 12), (17, 11), (18, 13), (18, 12)])
 def test_multi_pass_tiles(n, tile):
-    """Many tiles and passes: global controls, diagonal gates on global qubits."""
+    """LDS-tile passes: global controls, diagonal gates on global qubits."""
     calls = random_calls(n, 120, seed=7 * n + tile, with_unitary=False)
     ref = dense_ref(n, calls, prune=False)
-    got = gpu_state(n, calls, prune=False, tile=tile)
+    got = gpu_state(n, calls, prune=False, tile=tile, kernel="lds")
     assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("n,regs", [(12, 4), (13, 5), (14, 4), (16, 5), (17, 4), (18, 5)])
+@pytest.mark.parametrize("prune", [True, False])
+def test_register_tile_passes(n, regs, prune):
+    """Register-tile passes: lane swaps on every lane bit, slot/lane/global controls."""
+    calls = random_calls(n, 160, seed=31 * n + regs, with_unitary=True)
+    ref = dense_ref(n, calls, prune=prune)
+    got = gpu_state(n, calls, prune=prune, kernel="wave", wave_regs=regs)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("regs", [4, 5])
+def test_register_tile_random_layers(regs):
+    n = 16
+    calls = workloads.random_layer_calls(n, 8, seed=17)
+    ref = dense_ref(n, calls, prune=False)
+    for fuse in (False, True):
+        got = gpu_state(n, calls, prune=False, kernel="wave", wave_regs=regs, fuse=fuse)
+        assert np.max(np.abs(got - ref)) < TOL
 
 
 def test_fused_one_qubit_runs_exact_mode():
