@@ -89,8 +89,11 @@ typedef struct ksv_program ksv_program;
 #define KSV_OPT_TIMING       2  /* 1: time every pass launch with HIP events on the state stream  */
 #define KSV_OPT_FUSE         3  /* 1: multiply runs of gates on the same qubits into one matrix
                                    (exact mode only; ket's per-gate pruning forbids it)            */
-#define KSV_OPT_PASS_KERNEL  4  /* 0 auto, 1 LDS-tile passes, 2 register-tile (wavefront) passes  */
-#define KSV_OPT_WAVE_REGS    5  /* log2 amplitudes per lane in register-tile passes (0 = default)  */
+#define KSV_OPT_PASS_KERNEL  4  /* 0 auto, 1 LDS-tile passes, 2 register-round passes            */
+#define KSV_OPT_ROUND_TILE   5  /* tile qubits of register-round passes: 0 = default (11 FP64,
+                                   12 FP32) or 11..13; 2^(tile-4) threads x 16 amplitudes     */
+#define KSV_OPT_PREFETCH     6  /* 1: register-round passes run persistent workgroups that load
+                                   their next tile while computing the current one            */
 
 /* Per-state launch statistics (KSV_OPT_TIMING). */
 typedef struct ksv_stats {
@@ -109,7 +112,8 @@ typedef struct ksv_program_summary {
     uint64_t passes;            /* HBM passes the schedule makes            */
     uint64_t device_ops;        /* ops after fusion                         */
     int32_t  tile_qubits;       /* qubits per tile of the main pass kind    */
-    int32_t  wave_passes;       /* how many passes are register-tile passes */
+    int32_t  reg_passes;        /* how many passes are register-round passes */
+    uint64_t exchanges;         /* LDS transposes over all register-round passes */
 } ksv_program_summary;
 
 /* ---- library ------------------------------------------------------------ */
@@ -137,10 +141,11 @@ int ksv_program_info(const ksv_program* p, ksv_program_summary* out);
 int ksv_program_destroy(ksv_program* p);
 /* Schedule a tape without a device (dry run): the passes and ops
  * ksv_program_create would produce for a state of n qubits with the given
- * pass kernel (KSV_OPT_PASS_KERNEL values), LDS tile width and register
- * count (0 = defaults), fusion and pruning.  Used by the autotuner. */
+ * pass kernel (KSV_OPT_PASS_KERNEL values), LDS tile width and
+ * register-round tile width (0 = defaults), fusion and pruning.  Used by the
+ * autotuner. */
 int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-             int kernel, int tile, int wave_regs, int fuse, int prune, ksv_program_summary* out);
+             int kernel, int tile, int rtile, int fuse, int prune, ksv_program_summary* out);
 
 /* ---- read-out ---------------------------------------------------------------- */
 /* Dense amplitudes [offset, offset+count) as interleaved doubles (statevector(),

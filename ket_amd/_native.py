@@ -16,7 +16,8 @@ KSV_F64, KSV_F32 = 0, 1
 (KSV_SINGLE, KSV_CNOT, KSV_SWAP, KSV_TWO, KSV_CONTROLLED, KSV_TOFFOLI, KSV_CSWAP, KSV_CSRSWAP,
  KSV_UNITARY, KSV_BARRIER) = range(10)
 KSV_MAX_OPERANDS = 12
-KSV_OPT_PRUNE, KSV_OPT_TILE_QUBITS, KSV_OPT_TIMING, KSV_OPT_FUSE, KSV_OPT_PASS_KERNEL, KSV_OPT_WAVE_REGS = range(6)
+(KSV_OPT_PRUNE, KSV_OPT_TILE_QUBITS, KSV_OPT_TIMING, KSV_OPT_FUSE, KSV_OPT_PASS_KERNEL, KSV_OPT_ROUND_TILE,
+ KSV_OPT_PREFETCH) = range(7)
 
 OP_DTYPE = np.dtype([("kind", "<i4"), ("nq", "<i4"), ("q", "<i4", (KSV_MAX_OPERANDS,)), ("mat", "<i8")])
 PARAM_DTYPE = np.dtype([("op", "<i4"), ("param", "<i4"), ("coeff", "<f8"), ("dmat", "<i8")])
@@ -35,7 +36,7 @@ class KsvStats(ctypes.Structure):
 
 class KsvProgramSummary(ctypes.Structure):
     _fields_ = [("ops", c_uint64), ("passes", c_uint64), ("device_ops", c_uint64),
-                ("tile_qubits", c_int32), ("wave_passes", c_int32)]
+                ("tile_qubits", c_int32), ("reg_passes", c_int32), ("exchanges", c_uint64)]
 
 
 # (name, restype, argtypes) of every entry point in include/ketsv.h
@@ -117,15 +118,15 @@ def device_count():
     return c.value
 
 
-KERNELS = {"auto": 0, "lds": 1, "wave": 2}
+KERNELS = {"auto": 0, "lds": 1, "reg": 2}
 
 
-def plan(n, tape, precision="f64", kernel="auto", tile=0, wave_regs=0, fuse=False, prune=True):
+def plan(n, tape, precision="f64", kernel="auto", tile=0, rtile=0, fuse=False, prune=True):
     """Dry-run schedule of a tape (no device needed)."""
     ops, mats = tape
     s = KsvProgramSummary()
     check(lib().ksv_plan(int(n), KSV_F64 if precision == "f64" else KSV_F32, ptr(ops), len(ops), ptr(mats),
-                         len(mats), KERNELS[kernel], int(tile), int(wave_regs), 1 if fuse else 0, 1 if prune else 0,
+                         len(mats), KERNELS[kernel], int(tile), int(rtile), 1 if fuse else 0, 1 if prune else 0,
                          ctypes.byref(s)))
     return {f: getattr(s, f) for f, _ in KsvProgramSummary._fields_}
 
@@ -141,7 +142,7 @@ class State:
     """Owning handle of one device statevector (2^n amplitudes in HBM)."""
 
     def __init__(self, n, precision="f64", prune=True, tile=0, fuse=False, timing=False, kernel="auto",
-                 wave_regs=0):
+                 rtile=0, prefetch=False):
         self.n = int(n)
         self.precision = precision
         prec = KSV_F64 if precision == "f64" else KSV_F32
@@ -157,8 +158,10 @@ class State:
             self.set_option(KSV_OPT_TIMING, 1)
         if kernel != "auto":
             self.set_option(KSV_OPT_PASS_KERNEL, KERNELS[kernel])
-        if wave_regs:
-            self.set_option(KSV_OPT_WAVE_REGS, wave_regs)
+        if rtile:
+            self.set_option(KSV_OPT_ROUND_TILE, rtile)
+        if prefetch:
+            self.set_option(KSV_OPT_PREFETCH, 1)
 
     @property
     def handle(self):

@@ -46,7 +46,8 @@ struct ksv_state {
     int timing = 0;
     int fuse = 0;
     int kernel = 0;                 // KSV_OPT_PASS_KERNEL
-    int wave_regs = 0;              // KSV_OPT_WAVE_REGS
+    int rtile = 0;                  // KSV_OPT_ROUND_TILE
+    int prefetch = 0;               // KSV_OPT_PREFETCH
     ksv_stats stats;
     std::vector<hipEvent_t> ev;
 };
@@ -138,11 +139,11 @@ int ksv_set_option(ksv_state* s, int option, double value) {
         s->kernel = k;
         return KSV_OK;
     }
-    case KSV_OPT_WAVE_REGS: {
+    case KSV_OPT_PREFETCH: s->prefetch = value != 0; return KSV_OK;
+    case KSV_OPT_ROUND_TILE: {
         int r = (int)value;
-        int lo = s->prec == KSV_F64 ? 4 : 5, hi = s->prec == KSV_F64 ? 5 : 6;
-        if (r != 0 && (r < lo || r > hi)) return fail(KSV_ERR_RANGE, "register log2 must be 0 or " + std::to_string(lo) + ".." + std::to_string(hi));
-        s->wave_regs = r;
+        if (r != 0 && (r < 11 || r > 13)) return fail(KSV_ERR_RANGE, "register-round tile qubits must be 0 or 11..13");
+        s->rtile = r;
         return KSV_OK;
     }
     default: return fail(KSV_ERR_TYPE, "unknown option " + std::to_string(option));
@@ -171,33 +172,24 @@ int ksv_copy_state(ksv_state* dst, const ksv_state* src) {
 // --------------------------------------------------------------------------
 // Lower and schedule a tape into `p` (host side only).
 static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                     int kernel, int tile, int wave_regs, bool fuse, int prune) {
-    if (nops && !ops) return fail(KSV_ERR_TYPE, "null op tape");
-    std::vector<LOp> lops;
+                     int kernel, int tile, int rtile, bool fuse, int prune) {
     std::string err;
-    int rc = lower_tape(n, ops, nops, mats, nmats, lops, err);
+    int rc = plan_program(p, n, prec, ops, nops, mats, nmats, kernel, tile, rtile, fuse, prune, err);
     if (rc) return fail(rc, err);
-    p.n = n;
-    p.precision = prec;
-    p.prune = prune;
-    p.tile = tile > 0 ? tile : (prec == KSV_F64 ? 12 : 13);
-    p.tape_ops = nops;
-    const int rlog = wave_regs > 0 ? wave_regs : (prec == KSV_F64 ? 5 : 6);
-    if (kernel == 1) schedule(n, p.tile, fuse && !prune, lops, p);
-    else schedule_wave(n, rlog, fuse && !prune, lops, p);
     return KSV_OK;
 }
 
 static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                          int prune, bool fuse, ksv_program** out) {
     std::unique_ptr<ksv_program> p(new ksv_program());
-    int rc = plan_into(*p, s->n, s->prec, ops, nops, mats, nmats, s->kernel, default_tile(s), s->wave_regs, fuse,
+    int rc = plan_into(*p, s->n, s->prec, ops, nops, mats, nmats, s->kernel, default_tile(s), s->rtile, fuse,
                        prune);
     if (rc) return rc;
     p->dev = s->dev;
-    if (!p->wops.empty()) {
-        HIPCHK(hipMalloc(&p->d_wops, sizeof(WOp) * p->wops.size()));
-        HIPCHK(hipMemcpy(p->d_wops, p->wops.data(), sizeof(WOp) * p->wops.size(), hipMemcpyHostToDevice));
+    p->prefetch = s->prefetch;
+    if (!p->rops.empty()) {
+        HIPCHK(hipMalloc(&p->d_rops, sizeof(ROp) * p->rops.size()));
+        HIPCHK(hipMemcpy(p->d_rops, p->rops.data(), sizeof(ROp) * p->rops.size(), hipMemcpyHostToDevice));
     }
     if (!p->dops.empty()) {
         HIPCHK(hipMalloc(&p->d_ops, sizeof(DOp) * p->dops.size()));
@@ -235,7 +227,7 @@ static int run_program_on(ksv_state* s, ksv_program* p, void* psi) {
     for (size_t i = 0; i < nst; i++) {
         const Step& st = p->steps[i];
         if (st.big) HIPCHK(launch_big(*p, st, psi, p->prune, s->st));
-        else if (st.wave) HIPCHK(launch_wave(*p, st, psi, s->st));
+        else if (st.reg) HIPCHK(launch_rpass(*p, st, psi, s->st));
         else HIPCHK(launch_pass(*p, st, psi, s->st));
         if (timing) HIPCHK(hipEventRecord(s->ev[i + 1], s->st));
     }
@@ -276,15 +268,14 @@ static int summarize(const Program& p, ksv_program_summary* out) {
     out->ops = p.tape_ops;
     out->passes = p.steps.size();
     out->device_ops = p.dops.size() + p.big_ops.size();
-    int waves = 0;
+    int regs = 0, exch = 0;
     for (const Step& st : p.steps)
-        if (st.wave) {
-            waves++;
-        }
-    for (const WOp& w : p.wops)
-        if (w.kind != W_LSWAP) out->device_ops++;
-    out->tile_qubits = waves ? WLANE_BITS + p.rlog : p.tile;
-    out->wave_passes = waves;
+        if (st.reg) { regs++; exch += st.exchanges; }
+    for (const ROp& r : p.rops)
+        if (r.kind != R_EXCH) out->device_ops++;
+    out->tile_qubits = regs ? p.rtile : p.tile;
+    out->reg_passes = regs;
+    out->exchanges = exch;
     return KSV_OK;
 }
 
@@ -296,20 +287,21 @@ int ksv_program_info(const ksv_program* p, ksv_program_summary* out) {
 int ksv_program_destroy(ksv_program* p) {
     if (!p) return fail(KSV_ERR_HANDLE, "null program");
     if (p->d_ops) hipFree(p->d_ops);
-    if (p->d_wops) hipFree(p->d_wops);
+    if (p->d_rops) hipFree(p->d_rops);
     if (p->d_mats) hipFree(p->d_mats);
     delete p;
     return KSV_OK;
 }
 
 int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel,
-             int tile, int wave_regs, int fuse, int prune, ksv_program_summary* out) {
+             int tile, int rtile, int fuse, int prune, ksv_program_summary* out) {
     if (!out) return fail(KSV_ERR_TYPE, "null output pointer");
     if (n < 1 || n > MAXN) return fail(KSV_ERR_RANGE, "qubit count out of range");
     if (precision != KSV_F64 && precision != KSV_F32) return fail(KSV_ERR_TYPE, "unknown precision");
     if (kernel < 0 || kernel > 2) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1 or 2");
     Program p;
-    int rc = plan_into(p, n, precision, ops, nops, mats, nmats, kernel, tile, wave_regs, fuse != 0, prune);
+    if (rtile != 0 && (rtile < 11 || rtile > 13)) return fail(KSV_ERR_RANGE, "register-round tile qubits must be 0 or 11..13");
+    int rc = plan_into(p, n, precision, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune);
     if (rc) return rc;
     return summarize(p, out);
 }

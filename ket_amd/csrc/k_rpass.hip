@@ -1,0 +1,476 @@
+// Register-round pass kernel.
+//
+// A workgroup of 2^TB threads owns a tile of 2^(TB+4) amplitudes (the
+// amplitudes whose non-tile bits equal the tile's base index).  Each thread
+// holds 16 of them in registers: in every round four tile qubits are the bits
+// of the register index j ("slots") and the other TB tile qubits are the bits
+// of the thread index.  One-qubit gates, permutations and diagonals on slots
+// are straight-line FP code on registers indexed by compile-time constants;
+// the op stream is dispatched by a small switch over (kind, slot, control
+// form).  Dense 2..4-qubit gates stage the thread's 16 amplitudes in LDS and
+// read their inputs back with run-time indices, so they need no register
+// temporaries.  An R_EXCH op writes the tile to LDS under the current
+// assignment and reads it back under the next one (XOR-swizzled so that both
+// sides are free of bank conflicts).
+//
+// Workgroups are persistent: each walks tiles blockIdx.x, +gridDim.x, ...
+// and issues the loads of its next tile before applying the ops of the
+// current one, so HBM traffic overlaps the arithmetic.  HBM sees one read
+// and one write of every amplitude per pass, coalesced: thread bits 0..2 are
+// physical qubits 0..2 at load and store, so each 8 lanes move a full
+// 128-byte run.
+//
+// Op records (128 bytes, matrix inline) are read through the constant
+// address space: uniform scalar loads into SGPRs.
+#include <algorithm>
+#include <cstring>
+
+#include "kcommon.h"
+
+namespace ksv {
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <typename T>
+struct CM {        // one complex matrix entry in scalar registers
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ CM<T> ldm(const KSV_CONST T* m, int i) {
+    CM<T> r;
+    r.x = m[2 * i];
+    r.y = m[2 * i + 1];
+    return r;
+}
+
+template <typename T, bool PR>
+__device__ __forceinline__ void prune2(T& r, T& i) {
+    if (PR) {
+        if (((double)r * r + (double)i * i) < 1e-14) { r = 0; i = 0; }
+    }
+}
+
+// (ore, oim) = a * x0 + b * x1, four fused multiply-adds per component pair
+template <typename T>
+__device__ __forceinline__ void mac2(T& ore, T& oim, CM<T> a, T x0r, T x0i, CM<T> b, T x1r, T x1i) {
+    ore = __builtin_fma(a.x, x0r, __builtin_fma(-a.y, x0i, __builtin_fma(b.x, x1r, -(b.y * x1i))));
+    oim = __builtin_fma(a.x, x0i, __builtin_fma(a.y, x0r, __builtin_fma(b.x, x1i, b.y * x1r)));
+}
+
+#define RREGS_ARGS T (&re)[RREGS], T (&im)[RREGS]
+
+// 2x2 on slot S.  SEL: a thread-bit control exists (lp: this thread is
+// selected).  RC: slot controls exist (rctl).
+template <typename T, int S, bool PR, bool SEL, bool RC>
+__device__ __forceinline__ void r_m1(RREGS_ARGS, const KSV_CONST T* m, uint32_t rctl, bool lp) {
+    const CM<T> m0 = ldm(m, 0), m1 = ldm(m, 1), m2 = ldm(m, 2), m3 = ldm(m, 3);
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        if ((j >> S) & 1) continue;
+        if (RC && ((uint32_t)j & rctl) != rctl) continue;
+        const int j1 = j | (1 << S);
+        const T x0r = re[j], x0i = im[j], x1r = re[j1], x1i = im[j1];
+        T y0r, y0i, y1r, y1i;
+        mac2(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
+        mac2(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
+        prune2<T, PR>(y0r, y0i);
+        prune2<T, PR>(y1r, y1i);
+        if (SEL) {
+            y0r = lp ? y0r : x0r; y0i = lp ? y0i : x0i;
+            y1r = lp ? y1r : x1r; y1i = lp ? y1i : x1i;
+        }
+        re[j] = y0r; im[j] = y0i;
+        re[j1] = y1r; im[j1] = y1i;
+    }
+}
+
+template <typename T, int S, bool SEL>
+__device__ __forceinline__ void r_permx(RREGS_ARGS, uint32_t rctl, bool lp) {
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        if ((j >> S) & 1) continue;
+        if (((uint32_t)j & rctl) != rctl) continue;
+        const int j1 = j | (1 << S);
+        const T ar = re[j], ai = im[j], br = re[j1], bi = im[j1];
+        re[j] = SEL ? (lp ? br : ar) : br;
+        im[j] = SEL ? (lp ? bi : ai) : bi;
+        re[j1] = SEL ? (lp ? ar : br) : ar;
+        im[j1] = SEL ? (lp ? ai : bi) : ai;
+    }
+}
+
+template <typename T, int A, int B, bool SEL>
+__device__ __forceinline__ void r_swap(RREGS_ARGS, uint32_t rctl, bool lp) {
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        if (!((j >> A) & 1) || ((j >> B) & 1)) continue;
+        if (((uint32_t)j & rctl) != rctl) continue;
+        const int j1 = j ^ (1 << A) ^ (1 << B);
+        const T ar = re[j], ai = im[j], br = re[j1], bi = im[j1];
+        re[j] = SEL ? (lp ? br : ar) : br;
+        im[j] = SEL ? (lp ? bi : ai) : bi;
+        re[j1] = SEL ? (lp ? ar : br) : ar;
+        im[j1] = SEL ? (lp ? ai : bi) : ai;
+    }
+}
+
+// Diagonal over k <= 6 qubits.  Row bit (k-1-i) of qubit i comes from a
+// thread bit, a slot or a physical bit outside the tile (loc[i]).  For
+// k <= 2 the entries are inline (scalar registers) and selected per
+// amplitude; wider diagonals are looked up per amplitude.
+template <typename T, bool PR, bool SEL>
+__device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, uint32_t rctl, const KSV_CONST T* pay,
+                                       const T* wide, uint32_t tid, uint64_t base, bool lp) {
+    uint32_t rl = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int i = 0; i < DIAG_MAX; i++) {
+        if ((uint32_t)i >= k) continue;
+        const uint32_t rb = 1u << (k - 1 - i);
+        const uint32_t L = (uint32_t)(locw >> (8 * i)) & 0xFF;
+        if (L < 16) {
+            if ((tid >> L) & 1) rl |= rb;
+        } else if (L < 32) {
+            const uint32_t s = L - 16;
+            if (s == 0) s0 |= rb;
+            else if (s == 1) s1 |= rb;
+            else if (s == 2) s2 |= rb;
+            else s3 |= rb;
+        } else if ((base >> (L - 32)) & 1) {
+            rl |= rb;
+        }
+    }
+    if (k <= 2) {
+        const CM<T> d0 = ldm(pay, 0), d1 = ldm(pay, 1), d2 = ldm(pay, 2), d3 = ldm(pay, 3);
+#pragma unroll
+        for (int j = 0; j < RREGS; j++) {
+            if (((uint32_t)j & rctl) != rctl) continue;
+            const uint32_t row = rl | ((j & 1) ? s0 : 0) | ((j & 2) ? s1 : 0) | ((j & 4) ? s2 : 0) | ((j & 8) ? s3 : 0);
+            const T dx = (row & 2) ? ((row & 1) ? d3.x : d2.x) : ((row & 1) ? d1.x : d0.x);
+            const T dy = (row & 2) ? ((row & 1) ? d3.y : d2.y) : ((row & 1) ? d1.y : d0.y);
+            T yr = __builtin_fma(dx, re[j], -(dy * im[j]));
+            T yi = __builtin_fma(dx, im[j], dy * re[j]);
+            prune2<T, PR>(yr, yi);
+            if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
+            re[j] = yr;
+            im[j] = yi;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < RREGS; j++) {
+            if (((uint32_t)j & rctl) != rctl) continue;
+            const uint32_t row = rl | ((j & 1) ? s0 : 0) | ((j & 2) ? s1 : 0) | ((j & 4) ? s2 : 0) | ((j & 8) ? s3 : 0);
+            const T dx = wide[2 * row], dy = wide[2 * row + 1];
+            T yr = __builtin_fma(dx, re[j], -(dy * im[j]));
+            T yi = __builtin_fma(dx, im[j], dy * re[j]);
+            prune2<T, PR>(yr, yi);
+            if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
+            re[j] = yr;
+            im[j] = yi;
+        }
+    }
+}
+
+// Dense 2^k x 2^k (k = 2..4) on slots ts[0] (MSB) .. ts[k-1].  The thread's
+// amplitudes go to its own LDS column; output j reads its 2^k inputs back.
+template <typename T, int TB, bool PR>
+__device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds, uint32_t k, uint64_t locw,
+                                        uint32_t rctl, const KSV_CONST T* m, uint32_t tid, bool sel, bool lp) {
+    using V = typename Vec2<T>::type;
+    constexpr int NT = 1 << TB;
+    const uint32_t dim = 1u << k;
+    uint32_t tmask = 0;
+    for (uint32_t i = 0; i < k; i++) tmask |= 1u << ((locw >> (8 * i)) & 0xFF);
+    __syncthreads();        // other threads may still read the last exchange
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        V v;
+        v.x = re[j];
+        v.y = im[j];
+        lds[j * NT + tid] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        if (((uint32_t)j & rctl) != rctl) continue;
+        uint32_t row = 0;
+        for (uint32_t i = 0; i < k; i++) row |= (((uint32_t)j >> ((locw >> (8 * i)) & 0xFF)) & 1) << (k - 1 - i);
+        const uint32_t jb = (uint32_t)j & ~tmask;
+        T ar = 0, ai = 0;
+        for (uint32_t c = 0; c < dim; c++) {
+            uint32_t jc = jb;
+            for (uint32_t i = 0; i < k; i++) jc |= ((c >> (k - 1 - i)) & 1) << ((locw >> (8 * i)) & 0xFF);
+            const V x = lds[jc * NT + tid];
+            const CM<T> e = ldm(m, (int)(row * dim + c));
+            ar = __builtin_fma(e.x, x.x, __builtin_fma(-e.y, x.y, ar));
+            ai = __builtin_fma(e.x, x.y, __builtin_fma(e.y, x.x, ai));
+        }
+        prune2<T, PR>(ar, ai);
+        if (!sel || lp) {
+            re[j] = ar;
+            im[j] = ai;
+        }
+    }
+}
+
+// LDS transpose from the current slot/thread assignment to the next one.
+template <typename T, int TB>
+__device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, const KSV_CONST uint16_t* x,
+                                       uint32_t tid) {
+    using V = typename Vec2<T>::type;
+    uint32_t wb = 0, rb = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        const uint32_t bit = (tid >> i) & 1;
+        wb ^= bit ? (uint32_t)x[i] : 0u;
+        rb ^= bit ? (uint32_t)x[16 + i] : 0u;
+    }
+    const uint32_t w0 = x[12], w1 = x[13], w2 = x[14], w3 = x[15];
+    const uint32_t r0 = x[28], r1 = x[29], r2 = x[30], r3 = x[31];
+    __syncthreads();        // the previous LDS use of every thread is finished
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        const uint32_t a = wb ^ ((j & 1) ? w0 : 0) ^ ((j & 2) ? w1 : 0) ^ ((j & 4) ? w2 : 0) ^ ((j & 8) ? w3 : 0);
+        V v;
+        v.x = re[j];
+        v.y = im[j];
+        lds[a] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        const uint32_t a = rb ^ ((j & 1) ? r0 : 0) ^ ((j & 2) ? r1 : 0) ^ ((j & 4) ? r2 : 0) ^ ((j & 8) ? r3 : 0);
+        const V v = lds[a];
+        re[j] = v.x;
+        im[j] = v.y;
+    }
+}
+
+#define RSLOT_CASES(SLOT, FN) \
+    switch (SLOT) {           \
+    case 0: FN(0); break;     \
+    case 1: FN(1); break;     \
+    case 2: FN(2); break;     \
+    default: FN(3); break;    \
+    }
+
+#define RPAIR_CASES(A, B, FN)          \
+    switch ((A) * 4 + (B)) {           \
+    case 4: FN(1, 0); break;           \
+    case 8: FN(2, 0); break;           \
+    case 9: FN(2, 1); break;           \
+    case 12: FN(3, 0); break;          \
+    case 13: FN(3, 1); break;          \
+    default: FN(3, 2); break;          \
+    }
+
+__device__ __forceinline__ uint64_t phys_offset(uint32_t bits, int nbits, uint64_t lo, uint64_t hi) {
+    uint64_t off = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++)
+        if (i < nbits) off |= (uint64_t)((bits >> i) & 1) << byte_at(i < 8 ? lo : hi, i & 7);
+    return off;
+}
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t tile, int K, uint64_t tq_lo, uint64_t tq_hi) {
+    uint64_t base = tile;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (i < K) base = ins0_64(base, byte_at(i < 8 ? tq_lo : tq_hi, i & 7));
+    return base;
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
+    uint64_t ro = 0;
+#pragma unroll
+    for (int s = 0; s < RSLOTS; s++)
+        if ((j >> s) & 1) ro |= 1ull << byte_at(rmap, s);
+    return ro;
+}
+
+// PF: persistent workgroups that prefetch their next tile into registers;
+// otherwise one tile per workgroup.
+template <typename T, int TB, bool PR, bool PF>
+__global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
+    using V = typename Vec2<T>::type;
+    constexpr int K = TB + RSLOTS;
+    __shared__ V lds[1 << K];
+    const uint32_t tid = threadIdx.x;
+    V* psi = reinterpret_cast<V*>(a.psi);
+    const T* wide_mats = reinterpret_cast<const T*>(a.mats);
+    const KSV_CONST T* cmats = to_const(reinterpret_cast<const T*>(a.mats));
+    const KSV_CONST uint64_t* ops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
+
+    uint64_t tile = blockIdx.x;
+    if (tile >= a.ntiles) return;
+    const uint64_t ld_off = phys_offset(tid, TB, a.ld_t_lo, a.ld_t_hi);
+    const uint64_t st_off = phys_offset(tid, TB, a.st_t_lo, a.st_t_hi);
+    uint64_t base = tile_base(tile, K, a.tq_lo, a.tq_hi);
+
+    T re[RREGS], im[RREGS];
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        const V t = psi[base | ld_off | slot_offset<T>(j, a.ld_r)];
+        re[j] = t.x;
+        im[j] = t.y;
+    }
+
+    for (;;) {
+        const uint64_t next = tile + gridDim.x;
+        const bool more = PF && next < a.ntiles;
+        uint64_t nbase = 0;
+        T nre[RREGS], nim[RREGS];
+        if (PF && more) {
+            nbase = tile_base(next, K, a.tq_lo, a.tq_hi);
+#pragma unroll
+            for (int j = 0; j < RREGS; j++) {
+                const V t = psi[nbase | ld_off | slot_offset<T>(j, a.ld_r)];
+                nre[j] = t.x;
+                nim[j] = t.y;
+            }
+        }
+
+        for (int o = 0; o < a.nops; o++) {
+            const KSV_CONST uint64_t* w = ops + 16 * o;
+            const uint64_t h0 = w[0], h1 = w[1], h2 = w[2], h3 = w[3];
+            const uint32_t kind = (uint32_t)h0 & 0xFF;
+            const uint32_t sa = (uint32_t)(h0 >> 8) & 0xFF, sb = (uint32_t)(h0 >> 16) & 0xFF;
+            const uint32_t kq = (uint32_t)(h0 >> 24) & 0xFF;
+            const uint64_t locw = (h0 >> 32) | ((h1 & 0xFFFF) << 32);
+            const uint32_t tctl = (uint32_t)(h1 >> 32);
+            const uint32_t rctl = (uint32_t)h2;
+            const int32_t mat = (int32_t)(h2 >> 32);
+            const uint64_t gctl = h3;
+            const KSV_CONST T* pay = reinterpret_cast<const KSV_CONST T*>(w + 4);
+            if (kind == R_EXCH) {
+                r_exch<T, TB>(re, im, lds, reinterpret_cast<const KSV_CONST uint16_t*>(w + 4), tid);
+                continue;
+            }
+            if (kind == R_DENSE) {
+                // every thread reaches the barrier inside, whatever gctl says
+                const bool on = (base & gctl) == gctl;
+                const bool lp = on && ((tid & tctl) == tctl);
+                r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
+                continue;
+            }
+            if ((base & gctl) != gctl) continue;
+            const bool lp = (tid & tctl) == tctl;
+            const bool sel = tctl != 0;
+            switch (kind) {
+            case R_M1:
+                if (rctl == 0) {
+#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
+                    RSLOT_CASES(sa, F)
+#undef F
+                } else {
+#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
+                    RSLOT_CASES(sa, F)
+#undef F
+                }
+                break;
+            case R_DIAG:
+                if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+                else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+                break;
+            case R_PERMX:
+#define F(S) (sel ? r_permx<T, S, true>(re, im, rctl, lp) : r_permx<T, S, false>(re, im, rctl, lp))
+                RSLOT_CASES(sa, F)
+#undef F
+                break;
+            case R_SWAP:
+#define F(A, B) (sel ? r_swap<T, A, B, true>(re, im, rctl, lp) : r_swap<T, A, B, false>(re, im, rctl, lp))
+                RPAIR_CASES(sa, sb, F)
+#undef F
+                break;
+            default:
+                break;
+            }
+        }
+
+#pragma unroll
+        for (int j = 0; j < RREGS; j++) {
+            V t;
+            t.x = re[j];
+            t.y = im[j];
+            psi[base | st_off | slot_offset<T>(j, a.st_r)] = t;
+        }
+        if (!PF || !more) break;
+        tile = next;
+        base = nbase;
+#pragma unroll
+        for (int j = 0; j < RREGS; j++) {
+            re[j] = nre[j];
+            im[j] = nim[j];
+        }
+    }
+}
+
+template <typename T, int TB, bool PR, bool PF>
+static void launch_one(const RPassArgs& a, hipStream_t st) {
+    uint64_t blocks = a.ntiles;
+    if (PF) {
+        static int resident = 0;      // workgroups resident on the device at once
+        if (!resident) {
+            int dev = 0, cus = 0, per = 0;
+            hipGetDevice(&dev);
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_rpass<T, TB, PR, PF>),
+                                                         1 << TB, 0);
+            resident = std::max(1, cus) * std::max(1, per);
+        }
+        blocks = std::min<uint64_t>(a.ntiles, (uint64_t)resident);
+    }
+    hipLaunchKernelGGL((k_rpass<T, TB, PR, PF>), dim3((unsigned)blocks), dim3(1 << TB), 0, st, a);
+}
+
+template <typename T, bool PR>
+static void launch_tb(const RPassArgs& a, int tb, bool pf, hipStream_t st) {
+    if (pf) {
+        if (tb == 7) launch_one<T, 7, PR, true>(a, st);
+        else if (tb == 8) launch_one<T, 8, PR, true>(a, st);
+        else launch_one<T, 9, PR, true>(a, st);
+    } else {
+        if (tb == 7) launch_one<T, 7, PR, false>(a, st);
+        else if (tb == 8) launch_one<T, 8, PR, false>(a, st);
+        else launch_one<T, 9, PR, false>(a, st);
+    }
+}
+
+hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st) {
+    RPassArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.psi = psi;
+    a.ops = p.d_rops + s.op_begin;
+    a.mats = p.d_mats;
+    a.nops = s.op_count;
+    a.k = s.k;
+    a.ntiles = 1ull << (p.n - s.k);
+    auto pack = [](const int8_t* b, int n) {
+        uint64_t w = 0;
+        for (int i = 0; i < n && i < 8; i++) w |= (uint64_t)(uint8_t)b[i] << (8 * i);
+        return w;
+    };
+    const int tb = s.k - RSLOTS;
+    if (tb < 7 || tb > 9) return hipErrorInvalidValue;
+    a.tq_lo = pack(s.tq, 8);
+    a.tq_hi = pack(s.tq + 8, 8);
+    a.ld_t_lo = pack(s.ld_t, std::min(tb, 8));
+    a.ld_t_hi = tb > 8 ? pack(s.ld_t + 8, tb - 8) : 0;
+    a.st_t_lo = pack(s.st_t, std::min(tb, 8));
+    a.st_t_hi = tb > 8 ? pack(s.st_t + 8, tb - 8) : 0;
+    a.ld_r = (uint32_t)pack(s.ld_r, RSLOTS);
+    a.st_r = (uint32_t)pack(s.st_r, RSLOTS);
+    const bool pf = p.prefetch != 0;
+    if (p.precision == KSV_F64) {
+        if (p.prune) launch_tb<double, true>(a, tb, pf, st);
+        else launch_tb<double, false>(a, tb, pf, st);
+    } else {
+        if (p.prune) launch_tb<float, true>(a, tb, pf, st);
+        else launch_tb<float, false>(a, tb, pf, st);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ksv

@@ -45,48 +45,68 @@ constexpr int DIAG_MAX = 6;        // widest in-pass diagonal
 constexpr int BIG_MAX = 12;        // widest standalone unitary
 
 // ---------------------------------------------------------------------------
-// Register-tile ("wave") pass: one wavefront owns 2^(6+r) amplitudes, lane l
-// register j holding the amplitude whose tile-local lane bits are l and
-// register-slot bits are j.  Ops act on register slots; a qubit on a lane
-// bit is brought into a slot by W_LSWAP (cross-lane exchange).
+// Register-round pass: a workgroup of 2^TB threads owns a tile of 2^(TB+4)
+// amplitudes; every thread holds 16 of them in registers.  In each round
+// four tile qubits are "slots" (bits of the register index j) and the other
+// TB tile qubits are thread bits.  Gates act on slots; an R_EXCH op
+// transposes the tile through LDS into the next round's assignment.
 // ---------------------------------------------------------------------------
-enum WKind : uint8_t {
-    W_M1 = 0,     // 2x2 on slot a
-    W_M2 = 1,     // 4x4 on slots a (MSB), b
-    W_PHASE = 2,  // multiply by mats[mat] where every control bit is 1
-    W_DIAG = 3,   // diagonal over k qubits at loc[]
-    W_PERMX = 4,  // X on slot a (controlled)
-    W_SWAPR = 5,  // exchange slots a, b (controlled)
-    W_LSWAP = 6,  // exchange lane bit a with slot b
+enum RKind : uint8_t {
+    R_M1 = 0,     // 2x2 on slot a; matrix inline
+    R_DENSE = 1,  // 2^k x 2^k (k = 2..4) on slots loc[0] (MSB) .. loc[k-1], through LDS; matrix at `mat`
+    R_DIAG = 2,   // diagonal over k <= 6 qubits at loc[] (k = 0: a phase under the controls);
+                  // k <= 2: entries inline, else at `mat`
+    R_PERMX = 3,  // X on slot a
+    R_SWAP = 4,   // exchange slots a, b; a > b
+    R_EXCH = 5,   // LDS transpose; its XRec inline
 };
 
-struct WOp {
+constexpr int RSLOTS = 4;          // qubits held in registers per round
+constexpr int RREGS = 1 << RSLOTS; // amplitudes per thread
+
+// LDS address (in amplitudes) of tile index x under an exchange's swizzle is
+// linear in x, so it is the XOR of one word per set bit: wt/rt per thread bit
+// of the writing/reading layout, wr/rr per slot.
+struct XRec {
+    uint16_t wt[12];
+    uint16_t wr[4];
+    uint16_t rt[12];
+    uint16_t rr[4];
+};
+static_assert(sizeof(XRec) == 64, "XRec layout");
+
+// 128-byte op record: a 32-byte header and an inline payload, so one
+// scalar fetch brings both the op and its (small) matrix.
+struct ROp {
     uint8_t kind;
     uint8_t a, b;
     uint8_t k;
-    uint8_t loc[6];    // W_DIAG: 0..5 lane bit, 8+s slot, 16+g global bit g; loc[0] = MSB of the row
+    uint8_t loc[6];    // R_DIAG, MSB first: 0..15 thread bit, 16+s slot s, 32+g physical bit g outside the tile;
+                       // R_DENSE: the target slots, MSB first
     uint16_t pad;
-    uint32_t lctl;     // lane-bit controls (mask of lane index bits)
-    uint32_t rctl;     // slot controls (mask of register index bits)
-    int32_t mat;       // complex offset into the program matrix array
-    uint64_t gctl;     // global controls
+    uint32_t tctl;     // thread-bit controls (mask over the thread index)
+    uint32_t rctl;     // slot controls (mask over the register index)
+    int32_t mat;       // complex offset into the program matrices (R_DENSE, R_DIAG with k > 2)
+    uint64_t gctl;     // controls on physical bits outside the tile
+    uint64_t pay[12];  // R_M1: 4 complex entries, R_DIAG k <= 2: 2^k entries, in the state's precision;
+                       // R_EXCH: XRec
 };
-static_assert(sizeof(WOp) == 32, "WOp layout");
+static_assert(sizeof(ROp) == 128, "ROp layout");
 
-constexpr int WLANE_BITS = 6;
-
-// Byte i of a packed word holds the state bit of lane bit / register slot /
-// tile bit i (arrays inside a by-value kernel argument would be copied to
-// scratch memory, packed words stay in scalar registers).
-struct WaveArgs {
+// Byte i of a packed word holds a physical state bit (arrays inside a
+// by-value kernel argument would be copied to scratch; packed words stay in
+// scalar registers).
+struct RPassArgs {
     void* psi;
-    const WOp* ops;
+    const ROp* ops;
     const void* mats;
     int nops;
-    int k;               // tile qubits = 6 + r
-    uint64_t lane_in, lane_out;   // state bit of each lane bit at load / store
-    uint64_t reg_in, reg_out;     // state bit of each register slot at load / store
-    uint64_t tq_lo, tq_hi;        // tile state bits ascending (to build the base)
+    int k;                        // tile qubits
+    uint64_t ntiles;              // 2^(n - k)
+    uint64_t tq_lo, tq_hi;        // physical bit of tile bit i, ascending
+    uint64_t ld_t_lo, ld_t_hi;    // physical bit of thread bit i at load
+    uint64_t st_t_lo, st_t_hi;    // ... at store
+    uint32_t ld_r, st_r;          // physical bit of slot s at load / store
 };
 
 struct PassArgs {
@@ -118,14 +138,14 @@ struct LOp {
 
 struct Step {
     bool big = false;        // standalone unitary step
-    bool wave = false;       // register-tile pass (WOps) instead of an LDS pass (DOps)
+    bool reg = false;        // register-round pass (ROps) instead of an LDS pass (DOps)
     int k = 0;               // tile qubits
     int8_t tq[16] = {0};
-    int op_begin = 0, op_count = 0;   // DOps (LDS pass) or WOps (wave pass) of this step
+    int op_begin = 0, op_count = 0;   // DOps (LDS pass) or ROps (register-round pass)
     int big_op = -1;         // index into Program::big_ops
-    int rlog = 0;            // wave pass: log2 registers per lane
-    int8_t lane_in[6] = {0}, lane_out[6] = {0};
-    int8_t reg_in[8] = {0}, reg_out[8] = {0};
+    int exchanges = 0;       // register-round pass: LDS transposes
+    int8_t ld_t[12] = {0}, st_t[12] = {0};   // physical bit of thread bit i at load / store
+    int8_t ld_r[4] = {0}, st_r[4] = {0};     // physical bit of slot s at load / store
 };
 
 struct BigOp {
@@ -141,10 +161,11 @@ struct Program {
     uint64_t tape_ops = 0;
     std::vector<Step> steps;
     std::vector<DOp> dops;
-    std::vector<WOp> wops;
+    std::vector<ROp> rops;
     std::vector<BigOp> big_ops;
-    int rlog = 0;            // > 0: wave passes with 2^rlog registers per lane
-    WOp* d_wops = nullptr;
+    int rtile = 0;           // > 0: tile qubits of the register-round passes
+    int prefetch = 0;        // register-round passes: persistent workgroups prefetching their next tile
+    ROp* d_rops = nullptr;
     std::vector<cd> mats;
     // device copies
     DOp* d_ops = nullptr;
@@ -156,12 +177,16 @@ struct Program {
 int lower_tape(int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                std::vector<LOp>& out, std::string& err);
 void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog);
-void schedule_wave(int n, int rlog, bool fuse, std::vector<LOp>& lops, Program& prog);
+void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Program& prog);
+// Lower and schedule a tape into `p` (host side only).  kernel: KSV_OPT_PASS_KERNEL
+// value; tile / rtile: LDS and register-round tile widths (0 = defaults).
+int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err);
 
 // kernels (k_pass.hip / k_reduce.hip)
 hipError_t launch_pass(const Program& p, const Step& s, void* psi, hipStream_t st);
 hipError_t launch_big(const Program& p, const Step& s, void* psi, int prune, hipStream_t st);
-hipError_t launch_wave(const Program& p, const Step& s, void* psi, hipStream_t st);
+hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st);
 
 struct Scratch {
     double* partial = nullptr;     // reduction partials

@@ -430,159 +430,253 @@ void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog)
 }
 
 // --------------------------------------------------------------------------
-// Register-tile passes.
+// Register-round passes.
+//
+// A pass over K = TB + 4 tile qubits is split into rounds.  In each round
+// four tile qubits are register slots and TB are thread bits; every op of
+// the round needs its non-diagonal targets in slots.  Rounds are chosen
+// greedily: scan the remaining ops in order, take each op whose slot needs
+// still fit in four slots and that no skipped op blocks.  Between rounds an
+// LDS transpose moves the tile into the next assignment.  Tile bits 0..2
+// (physical qubits 0..2) are thread bits 0..2 at load and store, so every 8
+// lanes move one 128-byte run of HBM.
 // --------------------------------------------------------------------------
-static bool wave_ok(const LOp& o) {
+static bool round_ok(const LOp& o) {
     if (o.big) return false;
-    if (o.cls == L_MAT && o.tg.size() > 2) return false;
-    if (o.cls == L_DIAG && o.tg.size() > 2) return false;
+    if (o.cls == L_MAT) return o.tg.size() <= 2 || (o.tg.size() <= (size_t)RSLOTS && o.ctl.empty());
     return true;
 }
 
-// where[] codes: -1 outside the tile, 0..5 lane bit, 8+s register slot
-static void wave_pass(int n, int K, int rlog, uint64_t T, const std::vector<LOp>& lops,
-                      const std::vector<int>& idx, Program& prog) {
-    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
-    Step s;
-    s.wave = true;
-    s.k = K;
-    s.rlog = rlog;
-    int where[64];
-    for (int q = 0; q < 64; q++) where[q] = -1;
-    int j = 0;
-    for (int q = 0; q < n; q++)
-        if (T >> q & 1) s.tq[j++] = (int8_t)q;
-    // targets that must sit in register slots, per op
-    std::vector<std::vector<int>> tgs(idx.size());
-    for (size_t i = 0; i < idx.size(); i++) {
-        const LOp& o = lops[idx[i]];
-        if (o.cls != L_DIAG) tgs[i] = o.tg;
-    }
-    // initial placement: state bits 0..2 on lane bits 0..2; the earliest
-    // targets in the slots; the rest on lane bits 3..5
-    for (int b = 0; b < 3; b++) where[b] = b;
-    std::vector<int> order;
-    for (size_t i = 0; i < idx.size(); i++)
-        for (int q : tgs[i])
-            if (q > 2 && std::find(order.begin(), order.end(), q) == order.end()) order.push_back(q);
-    for (int q = 3; q < n; q++)
-        if ((T >> q & 1) && std::find(order.begin(), order.end(), q) == order.end()) order.push_back(q);
-    int slot_q[8], lane_q[6] = {0, 1, 2, -1, -1, -1};
-    for (int i = 0; i < rlog; i++) { slot_q[i] = order[i]; where[order[i]] = 8 + i; }
-    for (int i = 0; i < 3; i++) { lane_q[3 + i] = order[rlog + i]; where[order[rlog + i]] = 3 + i; }
-    for (int b = 0; b < 6; b++) s.lane_in[b] = (int8_t)lane_q[b];
-    for (int i = 0; i < rlog; i++) s.reg_in[i] = (int8_t)slot_q[i];
+struct RLayout {
+    int slot[RSLOTS];   // tile bit of slot s
+    int thr[12];        // tile bit of thread bit i
+};
 
-    s.op_begin = (int)prog.wops.size();
-    auto next_use = [&](int q, size_t from) -> size_t {
-        for (size_t i = from; i < idx.size(); i++)
-            for (int t : tgs[i])
-                if (t == q) return i;
-        return (size_t)-1;
-    };
-    auto lswap = [&](int lanebit, int slot) {
-        WOp w;
-        std::memset(&w, 0, sizeof w);
-        w.kind = W_LSWAP;
-        w.a = (uint8_t)lanebit;
-        w.b = (uint8_t)slot;
-        prog.wops.push_back(w);
-        int ql = lane_q[lanebit], qs = slot_q[slot];
-        lane_q[lanebit] = qs;
-        slot_q[slot] = ql;
-        where[qs] = lanebit;
-        where[ql] = 8 + slot;
-    };
-    for (size_t i = 0; i < idx.size(); i++) {
-        const LOp& o = lops[idx[i]];
-        for (int q : tgs[i]) {
-            if (where[q] >= 8) continue;
-            // evict the slot whose qubit is needed furthest ahead
-            int best = -1;
-            size_t bestd = 0;
-            for (int sl = 0; sl < rlog; sl++) {
-                int qq = slot_q[sl];
-                if (std::find(tgs[i].begin(), tgs[i].end(), qq) != tgs[i].end()) continue;
-                size_t d = next_use(qq, i);
-                if (best < 0 || d > bestd) { best = sl; bestd = d; }
-            }
-            lswap(where[q], best);
-        }
-        WOp w;
-        std::memset(&w, 0, sizeof w);
-        for (int c : o.ctl) {
-            int wc = where[c];
-            if (wc < 0) w.gctl |= 1ull << c;
-            else if (wc < 8) w.lctl |= 1u << wc;
-            else w.rctl |= 1u << (wc - 8);
-        }
-        w.mat = (int32_t)prog.mats.size();
-        switch (o.cls) {
-        case L_MAT:
-            if (o.tg.size() == 1) {
-                w.kind = W_M1;
-                w.a = (uint8_t)(where[o.tg[0]] - 8);
-                prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
-            } else {
-                w.kind = W_M2;
-                int sa = where[o.tg[0]] - 8, sb = where[o.tg[1]] - 8;
-                if (sa > sb) {
-                    prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
-                    w.a = (uint8_t)sa;
-                    w.b = (uint8_t)sb;
-                } else {
-                    static const int perm[4] = {0, 2, 1, 3};
-                    for (int r = 0; r < 4; r++)
-                        for (int c = 0; c < 4; c++) prog.mats.push_back(o.m[perm[r] * 4 + perm[c]]);
-                    w.a = (uint8_t)sb;
-                    w.b = (uint8_t)sa;
-                }
-            }
-            break;
-        case L_DIAG:
-            if (o.tg.empty()) {
-                w.kind = W_PHASE;
-            } else {
-                w.kind = W_DIAG;
-                w.k = (uint8_t)o.tg.size();
-                for (size_t t = 0; t < o.tg.size(); t++) {
-                    int q = o.tg[t], wq = where[q];
-                    w.loc[t] = (uint8_t)(wq < 0 ? 16 + q : wq);
-                }
-            }
-            prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
-            break;
-        case L_PERMX:
-            w.kind = W_PERMX;
-            w.a = (uint8_t)(where[o.tg[0]] - 8);
-            break;
-        case L_SWAP: {
-            w.kind = W_SWAPR;
-            int sa = where[o.tg[0]] - 8, sb = where[o.tg[1]] - 8;
-            w.a = (uint8_t)std::max(sa, sb);
-            w.b = (uint8_t)std::min(sa, sb);
-            break;
-        }
-        }
-        prog.wops.push_back(w);
+static RLayout make_layout(int K, uint32_t S) {
+    RLayout L;
+    int ns = 0, nt = 0;
+    for (int p = 0; p < K; p++) {
+        if (S >> p & 1) L.slot[ns++] = p;
+        else L.thr[nt++] = p;
     }
-    // bring state bits 0..2 back to lane bits 0..2 for a coalesced store
-    for (int p = 0; p < 3; p++)
-        if (where[p] >= 8) lswap(p, where[p] - 8);
-    for (int b = 0; b < 6; b++) s.lane_out[b] = (int8_t)lane_q[b];
-    for (int i = 0; i < rlog; i++) s.reg_out[i] = (int8_t)slot_q[i];
-    s.op_count = (int)prog.wops.size() - s.op_begin;
-    prog.steps.push_back(s);
+    for (; nt < 12; nt++) L.thr[nt] = 0;
+    return L;
 }
 
-void schedule_wave(int n, int rlog, bool fuse, std::vector<LOp>& lops, Program& prog) {
+// Add the highest tile bits above 2 until S has RSLOTS members.
+static uint32_t fill_slots(int K, uint32_t S) {
+    for (int p = K - 1; p >= 3 && popc(S) < RSLOTS; p--) S |= 1u << p;
+    return S;
+}
+
+static int gf2_rank(std::vector<uint32_t> v) {
+    int r = 0;
+    for (int bit = 31; bit >= 0; bit--) {
+        int piv = -1;
+        for (size_t i = r; i < v.size(); i++)
+            if (v[i] >> bit & 1) { piv = (int)i; break; }
+        if (piv < 0) continue;
+        std::swap(v[r], v[piv]);
+        for (size_t i = 0; i < v.size(); i++)
+            if ((int)i != r && (v[i] >> bit & 1)) v[i] ^= v[r];
+        r++;
+    }
+    return r;
+}
+
+// LDS swizzle of one exchange: address(x) = x ^ XOR_{p >= 4, x_p = 1} v[p]
+// (v[p] < 16).  Chosen so that lanes 0..3 of both the writing and the reading
+// layout reach four distinct 16-byte columns of every 256-byte LDS row
+// (and lanes 0..2 distinct ones of every 128-byte half), i.e. conflict-free
+// 16-byte accesses on both sides.
+static void emit_exch(int K, int TB, const RLayout& W, const RLayout& R, Program& prog) {
+    uint32_t v[16] = {0};
+    auto addr = [&](int p) -> uint32_t { return (1u << p) ^ (p >= 4 ? v[p] : 0u); };
+    auto good = [&](const RLayout& L) {
+        std::vector<uint32_t> a3, a4;
+        for (int i = 0; i < 4 && i < TB; i++) {
+            uint32_t a = addr(L.thr[i]);
+            if (i < 3) a3.push_back(a & 7);
+            a4.push_back(a & 15);
+        }
+        return gf2_rank(a3) == (int)a3.size() && gf2_rank(a4) == (int)a4.size();
+    };
+    std::vector<int> free_p;
+    for (int i = 0; i < 4 && i < TB; i++) {
+        if (W.thr[i] >= 4 && std::find(free_p.begin(), free_p.end(), W.thr[i]) == free_p.end()) free_p.push_back(W.thr[i]);
+        if (R.thr[i] >= 4 && std::find(free_p.begin(), free_p.end(), R.thr[i]) == free_p.end()) free_p.push_back(R.thr[i]);
+    }
+    uint32_t rng = 0x9E3779B9u;
+    for (int attempt = 0; attempt < 4096 && !(good(W) && good(R)); attempt++) {
+        for (int p : free_p) {
+            rng ^= rng << 13;
+            rng ^= rng >> 17;
+            rng ^= rng << 5;
+            v[p] = rng & 15;
+        }
+    }
+    if (!(good(W) && good(R))) for (int p : free_p) v[p] = 0;   // unreachable in practice; only costs LDS cycles
+    XRec x;
+    std::memset(&x, 0, sizeof x);
+    for (int i = 0; i < TB; i++) { x.wt[i] = (uint16_t)addr(W.thr[i]); x.rt[i] = (uint16_t)addr(R.thr[i]); }
+    for (int s = 0; s < RSLOTS; s++) { x.wr[s] = (uint16_t)addr(W.slot[s]); x.rr[s] = (uint16_t)addr(R.slot[s]); }
+    ROp r;
+    std::memset(&r, 0, sizeof r);
+    r.kind = R_EXCH;
+    std::memcpy(r.pay, &x, sizeof x);
+    prog.rops.push_back(r);
+}
+
+// Up to four complex entries in the program's precision, inline in the record.
+static void put_inline(ROp& r, const cd* v, int count, int prec) {
+    if (prec == KSV_F64) {
+        double* d = reinterpret_cast<double*>(r.pay);
+        for (int i = 0; i < count; i++) { d[2 * i] = v[i].real(); d[2 * i + 1] = v[i].imag(); }
+    } else {
+        float* f = reinterpret_cast<float*>(r.pay);
+        for (int i = 0; i < count; i++) { f[2 * i] = (float)v[i].real(); f[2 * i + 1] = (float)v[i].imag(); }
+    }
+}
+
+// One op in the slot / thread-bit / global form of layout L.
+static void emit_rop(const LOp& o, const RLayout& L, const int* pos, Program& prog) {
+    auto slot_of = [&](int q) {
+        for (int s = 0; s < RSLOTS; s++)
+            if (L.slot[s] == pos[q]) return s;
+        return -1;
+    };
+    auto thr_of = [&](int q) {
+        for (int i = 0; i < 12; i++)
+            if (L.thr[i] == pos[q]) return i;
+        return -1;
+    };
+    ROp r;
+    std::memset(&r, 0, sizeof r);
+    for (int c : o.ctl) {
+        if (pos[c] < 0) r.gctl |= 1ull << c;
+        else if (slot_of(c) >= 0) r.rctl |= 1u << slot_of(c);
+        else r.tctl |= 1u << thr_of(c);
+    }
+    r.mat = (int32_t)prog.mats.size();
+    switch (o.cls) {
+    case L_MAT: {
+        const int k = (int)o.tg.size();
+        if (k == 1) {
+            r.kind = R_M1;
+            r.a = (uint8_t)slot_of(o.tg[0]);
+            put_inline(r, o.m.data(), 4, prog.precision);
+        } else {
+            // dense 2^k x 2^k (k = 2..4) on slots loc[0] (MSB) .. loc[k-1]
+            r.kind = R_DENSE;
+            r.k = (uint8_t)k;
+            for (int i = 0; i < k; i++) r.loc[i] = (uint8_t)slot_of(o.tg[i]);
+            prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+        }
+        break;
+    }
+    case L_DIAG:
+        r.kind = R_DIAG;
+        r.k = (uint8_t)o.tg.size();
+        for (size_t i = 0; i < o.tg.size(); i++) {
+            int q = o.tg[i];
+            if (pos[q] < 0) r.loc[i] = (uint8_t)(32 + q);
+            else if (slot_of(q) >= 0) r.loc[i] = (uint8_t)(16 + slot_of(q));
+            else r.loc[i] = (uint8_t)thr_of(q);
+        }
+        if (o.tg.size() <= 2) put_inline(r, o.m.data(), (int)o.m.size(), prog.precision);
+        else prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
+        break;
+    case L_PERMX:
+        r.kind = R_PERMX;
+        r.a = (uint8_t)slot_of(o.tg[0]);
+        break;
+    case L_SWAP: {
+        r.kind = R_SWAP;
+        int sa = slot_of(o.tg[0]), sb = slot_of(o.tg[1]);
+        r.a = (uint8_t)std::max(sa, sb);
+        r.b = (uint8_t)std::min(sa, sb);
+        break;
+    }
+    }
+    prog.rops.push_back(r);
+}
+
+static void round_pass(int n, int K, uint64_t T, const std::vector<LOp>& lops, const std::vector<int>& idx,
+                       bool strict, Program& prog) {
+    const int TB = K - RSLOTS;
+    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
+    Step st;
+    st.reg = true;
+    st.k = K;
+    int pos[64];
+    for (int q = 0; q < 64; q++) pos[q] = -1;
+    int j = 0;
+    for (int q = 0; q < n; q++)
+        if (T >> q & 1) { st.tq[j] = (int8_t)q; pos[q] = j; j++; }
+    const uint32_t low = 7;   // tile bits 0..2 = physical qubits 0..2
+
+    std::vector<uint32_t> sets;
+    std::vector<std::vector<int>> takes;
+    std::vector<int> rem = idx, rest, take;
+    bool first = true;
+    while (!rem.empty()) {
+        uint32_t S = 0;
+        uint64_t blocked = 0;
+        take.clear();
+        rest.clear();
+        for (size_t i = 0; i < rem.size(); i++) {
+            const LOp& o = lops[rem[i]];
+            bool ok = !(o.touch & blocked);
+            uint32_t need = 0;
+            if (ok && o.cls != L_DIAG)
+                for (int q : o.tg) need |= 1u << pos[q];
+            if (ok && first && (need & low)) ok = false;
+            if (ok && popc(S | need) > RSLOTS) ok = false;
+            if (ok) {
+                S |= need;
+                take.push_back(rem[i]);
+            } else {
+                blocked |= o.touch;
+                rest.push_back(rem[i]);
+                if (strict) {
+                    rest.insert(rest.end(), rem.begin() + i + 1, rem.end());
+                    break;
+                }
+            }
+        }
+        sets.push_back(fill_slots(K, S));
+        takes.push_back(take);
+        rem.swap(rest);
+        first = false;
+    }
+    if (sets.empty()) { sets.push_back(fill_slots(K, 0)); takes.emplace_back(); }
+    if (sets.back() & low) { sets.push_back(fill_slots(K, sets.back() & ~low)); takes.emplace_back(); }
+
+    const RLayout L0 = make_layout(K, sets.front()), LN = make_layout(K, sets.back());
+    for (int i = 0; i < TB; i++) { st.ld_t[i] = st.tq[L0.thr[i]]; st.st_t[i] = st.tq[LN.thr[i]]; }
+    for (int s = 0; s < RSLOTS; s++) { st.ld_r[s] = st.tq[L0.slot[s]]; st.st_r[s] = st.tq[LN.slot[s]]; }
+    st.op_begin = (int)prog.rops.size();
+    for (size_t r = 0; r < sets.size(); r++) {
+        const RLayout L = make_layout(K, sets[r]);
+        if (r > 0) {
+            emit_exch(K, TB, make_layout(K, sets[r - 1]), L, prog);
+            st.exchanges++;
+        }
+        for (int id : takes[r]) emit_rop(lops[id], L, pos, prog);
+    }
+    st.op_count = (int)prog.rops.size() - st.op_begin;
+    prog.steps.push_back(st);
+}
+
+void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Program& prog) {
     if (fuse) fuse_1q(n, lops);
-    const int K = WLANE_BITS + rlog;
-    if (n < K + 2) {
+    const int K = rtile;
+    if (n < K) {
         schedule_lds(n, prog.tile, lops, prog);
         return;
     }
-    prog.rlog = rlog;
+    prog.rtile = K;
+    const bool strict = prog.prune != 0;
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     const uint64_t forced = 7;
     const int KL = std::min(n, prog.tile);
@@ -596,17 +690,37 @@ void schedule_wave(int n, int rlog, bool fuse, std::vector<LOp>& lops, Program& 
             rem.erase(rem.begin());
             continue;
         }
-        if (!wave_ok(first)) {
-            // a dense 3- or 4-qubit matrix or a wide diagonal: one LDS pass of its own
+        if (!round_ok(first)) {
+            // a controlled dense 3- or 4-qubit matrix: one LDS pass of its own
             uint64_t T = ((1ull << std::min(4, KL)) - 1) | first.need;
             emit_lds_pass(n, KL, T, lops, {rem[0]}, prog);
             rem.erase(rem.begin());
             continue;
         }
-        uint64_t T = pick_pass(lops, rem, K, forced, all, wave_ok, chosen, rest, prog.prune != 0);
-        wave_pass(n, K, rlog, T, lops, chosen, prog);
+        uint64_t T = pick_pass(lops, rem, K, forced, all, round_ok, chosen, rest, strict);
+        round_pass(n, K, T, lops, chosen, strict, prog);
         rem.swap(rest);
     }
+}
+
+int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err) {
+    if (nops && !ops) {
+        err = "null op tape";
+        return KSV_ERR_TYPE;
+    }
+    std::vector<LOp> lops;
+    int rc = lower_tape(n, ops, nops, mats, nmats, lops, err);
+    if (rc) return rc;
+    p.n = n;
+    p.precision = prec;
+    p.prune = prune;
+    p.tile = tile > 0 ? tile : (prec == KSV_F64 ? 12 : 13);
+    p.tape_ops = nops;
+    const int rt = rtile > 0 ? rtile : (prec == KSV_F64 ? 11 : 12);
+    if (kernel == 1) schedule(n, p.tile, fuse && !prune, lops, p);
+    else schedule_rounds(n, rt, fuse && !prune, lops, p);
+    return KSV_OK;
 }
 
 }  // namespace ksv

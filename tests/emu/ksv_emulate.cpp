@@ -1,0 +1,372 @@
+// Host emulation of libketsv's scheduled programs (test infrastructure).
+//
+// Runs the exact program that ksv_program_create would upload (the same
+// plan_program() from ket_amd/csrc/schedule.cpp) on a host FP64 state,
+// executing every step the way its kernel does: register-round passes thread
+// by thread with 16 registers each and the LDS transposes through their
+// swizzled addresses, LDS-tile passes element by element, wide unitaries
+// group by group.  Every HBM index and every LDS address is checked: in
+// range, each tile loaded and stored exactly once, every LDS slot of an
+// exchange written exactly once.  The CPU tests compare the result with the
+// oracle, so scheduler and addressing mistakes show up without a GPU.
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ksv_internal.h"
+
+using namespace ksv;
+
+static std::string g_msg;
+
+static uint64_t ins0(uint64_t p, int q) { return ((p >> q) << (q + 1)) | (p & ((1ull << q) - 1)); }
+
+static bool prune_ok(cd& v, int prune) {
+    if (prune && (v.real() * v.real() + v.imag() * v.imag()) < 1e-14) v = 0;
+    return true;
+}
+
+#define CHECK(cond, msg)                     \
+    do {                                     \
+        if (!(cond)) {                       \
+            g_msg = std::string(msg);        \
+            return -1;                       \
+        }                                    \
+    } while (0)
+
+static int emu_reg(const Program& p, const Step& s, cd* psi, int n) {
+    const int K = s.k, TB = K - RSLOTS, NT = 1 << TB;
+    const uint64_t N = 1ull << n;
+    const ROp* ops = p.rops.data() + s.op_begin;
+    const int nops = s.op_count;
+    std::vector<cd> regs((size_t)NT * RREGS), lds((size_t)1 << K);
+    std::vector<int> mark((size_t)1 << K);
+    std::vector<uint64_t> ld_idx, st_idx;
+    uint64_t tmask = 0;
+    for (int i = 0; i < K; i++) tmask |= 1ull << s.tq[i];
+    for (uint64_t b = 0; b < (N >> K); b++) {
+        uint64_t base = b;
+        for (int i = 0; i < K; i++) base = ins0(base, s.tq[i]);
+        CHECK((base & tmask) == 0, "base overlaps the tile");
+        ld_idx.clear();
+        st_idx.clear();
+        for (int t = 0; t < NT; t++) {
+            uint64_t lo = base;
+            for (int i = 0; i < TB; i++) lo |= (uint64_t)((t >> i) & 1) << s.ld_t[i];
+            for (int j = 0; j < RREGS; j++) {
+                uint64_t idx = lo;
+                for (int q = 0; q < RSLOTS; q++)
+                    if ((j >> q) & 1) idx |= 1ull << s.ld_r[q];
+                CHECK(idx < N, "load index out of range");
+                CHECK((idx & ~tmask) == base, "load index leaves the tile");
+                ld_idx.push_back(idx);
+                regs[(size_t)t * RREGS + j] = psi[idx];
+            }
+        }
+        for (int o = 0; o < nops; o++) {
+            const ROp& op = ops[o];
+            if (op.kind == R_EXCH) {
+                XRec x;
+                std::memcpy(&x, op.pay, sizeof x);
+                std::fill(mark.begin(), mark.end(), 0);
+                for (int t = 0; t < NT; t++) {
+                    uint32_t wb = 0;
+                    for (int i = 0; i < TB; i++)
+                        if ((t >> i) & 1) wb ^= x.wt[i];
+                    for (int j = 0; j < RREGS; j++) {
+                        uint32_t a = wb;
+                        for (int q = 0; q < RSLOTS; q++)
+                            if ((j >> q) & 1) a ^= x.wr[q];
+                        CHECK(a < (1u << K), "LDS write address out of range");
+                        mark[a]++;
+                        lds[a] = regs[(size_t)t * RREGS + j];
+                    }
+                }
+                for (int a = 0; a < (1 << K); a++) CHECK(mark[a] == 1, "LDS slot not written exactly once");
+                for (int t = 0; t < NT; t++) {
+                    uint32_t rb = 0;
+                    for (int i = 0; i < TB; i++)
+                        if ((t >> i) & 1) rb ^= x.rt[i];
+                    for (int j = 0; j < RREGS; j++) {
+                        uint32_t a = rb;
+                        for (int q = 0; q < RSLOTS; q++)
+                            if ((j >> q) & 1) a ^= x.rr[q];
+                        CHECK(a < (1u << K), "LDS read address out of range");
+                        regs[(size_t)t * RREGS + j] = lds[a];
+                    }
+                }
+                continue;
+            }
+            if ((base & op.gctl) != op.gctl) continue;
+            CHECK(op.mat >= 0 && (size_t)op.mat <= p.mats.size(), "matrix offset out of range");
+            const cd* wide = p.mats.data() + op.mat;
+            cd inl[4];
+            {
+                const double* d = reinterpret_cast<const double*>(op.pay);
+                for (int i = 0; i < 4; i++) inl[i] = cd(d[2 * i], d[2 * i + 1]);
+            }
+            for (int t = 0; t < NT; t++) {
+                if ((t & op.tctl) != op.tctl) continue;
+                cd* r = regs.data() + (size_t)t * RREGS;
+                switch (op.kind) {
+                case R_M1: {
+                    CHECK(op.a < RSLOTS, "bad slot");
+                    const cd* m = inl;
+                    for (int j = 0; j < RREGS; j++) {
+                        if ((j >> op.a) & 1) continue;
+                        if (((uint32_t)j & op.rctl) != op.rctl) continue;
+                        const int j1 = j | (1 << op.a);
+                        cd x0 = r[j], x1 = r[j1];
+                        cd y0 = m[0] * x0 + m[1] * x1, y1 = m[2] * x0 + m[3] * x1;
+                        prune_ok(y0, p.prune);
+                        prune_ok(y1, p.prune);
+                        r[j] = y0;
+                        r[j1] = y1;
+                    }
+                    break;
+                }
+                case R_DENSE: {
+                    const int k = op.k, dim = 1 << k;
+                    CHECK(k >= 2 && k <= RSLOTS, "bad dense width");
+                    uint32_t tm = 0;
+                    for (int i = 0; i < k; i++) { CHECK(op.loc[i] < RSLOTS, "bad dense slot"); tm |= 1u << op.loc[i]; }
+                    CHECK(__builtin_popcount(tm) == k, "dense slots repeat");
+                    CHECK((size_t)op.mat + (size_t)dim * dim <= p.mats.size(), "dense matrix out of range");
+                    cd in[RREGS];
+                    for (int j = 0; j < RREGS; j++) in[j] = r[j];
+                    for (int j = 0; j < RREGS; j++) {
+                        if (((uint32_t)j & op.rctl) != op.rctl) continue;
+                        int row = 0;
+                        for (int i = 0; i < k; i++) row |= ((j >> op.loc[i]) & 1) << (k - 1 - i);
+                        cd acc = 0;
+                        for (int c = 0; c < dim; c++) {
+                            int jc = j & ~(int)tm;
+                            for (int i = 0; i < k; i++) jc |= ((c >> (k - 1 - i)) & 1) << op.loc[i];
+                            acc += wide[row * dim + c] * in[jc];
+                        }
+                        prune_ok(acc, p.prune);
+                        r[j] = acc;
+                    }
+                    break;
+                }
+                case R_DIAG: {
+                    const int k = op.k;
+                    for (int j = 0; j < RREGS; j++) {
+                        if (((uint32_t)j & op.rctl) != op.rctl) continue;
+                        uint32_t row = 0;
+                        for (int i = 0; i < k; i++) {
+                            const int L = op.loc[i];
+                            int bit;
+                            if (L < 16) { CHECK(L < TB, "bad thread bit"); bit = (t >> L) & 1; }
+                            else if (L < 32) { CHECK(L - 16 < RSLOTS, "bad slot"); bit = (j >> (L - 16)) & 1; }
+                            else { CHECK(L - 32 < n, "bad global bit"); bit = (int)((base >> (L - 32)) & 1); }
+                            row |= (uint32_t)bit << (k - 1 - i);
+                        }
+                        cd y = (k <= 2 ? inl[row] : wide[row]) * r[j];
+                        prune_ok(y, p.prune);
+                        r[j] = y;
+                    }
+                    break;
+                }
+                case R_PERMX: {
+                    CHECK(op.a < RSLOTS, "bad slot");
+                    for (int j = 0; j < RREGS; j++) {
+                        if ((j >> op.a) & 1) continue;
+                        if (((uint32_t)j & op.rctl) != op.rctl) continue;
+                        std::swap(r[j], r[j | (1 << op.a)]);
+                    }
+                    break;
+                }
+                case R_SWAP: {
+                    CHECK(op.a < RSLOTS && op.b < op.a, "bad slot pair");
+                    for (int j = 0; j < RREGS; j++) {
+                        if (!((j >> op.a) & 1) || ((j >> op.b) & 1)) continue;
+                        if (((uint32_t)j & op.rctl) != op.rctl) continue;
+                        std::swap(r[j], r[j ^ (1 << op.a) ^ (1 << op.b)]);
+                    }
+                    break;
+                }
+                default:
+                    CHECK(false, "unknown register-round op");
+                }
+            }
+        }
+        for (int t = 0; t < NT; t++) {
+            uint64_t lo = base;
+            for (int i = 0; i < TB; i++) lo |= (uint64_t)((t >> i) & 1) << s.st_t[i];
+            for (int j = 0; j < RREGS; j++) {
+                uint64_t idx = lo;
+                for (int q = 0; q < RSLOTS; q++)
+                    if ((j >> q) & 1) idx |= 1ull << s.st_r[q];
+                CHECK(idx < N, "store index out of range");
+                st_idx.push_back(idx);
+                psi[idx] = regs[(size_t)t * RREGS + j];
+            }
+        }
+        std::sort(ld_idx.begin(), ld_idx.end());
+        std::sort(st_idx.begin(), st_idx.end());
+        CHECK(std::adjacent_find(ld_idx.begin(), ld_idx.end()) == ld_idx.end(), "an amplitude is loaded twice");
+        CHECK(ld_idx == st_idx, "store set differs from load set");
+        CHECK(ld_idx.size() == ((size_t)1 << K), "tile size mismatch");
+    }
+    return 0;
+}
+
+static int emu_lds(const Program& p, const Step& s, cd* psi, int n) {
+    const int K = s.k;
+    const uint64_t N = 1ull << n;
+    const uint32_t tsize = 1u << K;
+    std::vector<cd> T(tsize);
+    for (uint64_t b = 0; b < (N >> K); b++) {
+        uint64_t base = b;
+        for (int i = 0; i < K; i++) base = ins0(base, s.tq[i]);
+        auto scatter = [&](uint32_t l) {
+            uint64_t off = 0;
+            for (int i = 0; i < K; i++) off |= (uint64_t)((l >> i) & 1) << s.tq[i];
+            return off;
+        };
+        for (uint32_t l = 0; l < tsize; l++) {
+            CHECK((base | scatter(l)) < N, "LDS pass index out of range");
+            T[l] = psi[base | scatter(l)];
+        }
+        for (int o = 0; o < s.op_count; o++) {
+            const DOp& op = p.dops[s.op_begin + o];
+            if ((base & op.gctl) != op.gctl) continue;
+            const cd* m = p.mats.data() + op.mat;
+            const uint32_t ne = tsize >> op.nins;
+            for (uint32_t e = 0; e < ne; e++) {
+                uint32_t l = e;
+                for (int i = 0; i < op.nins; i++) l = (uint32_t)ins0(l, op.ins[i]);
+                l |= op.lctl;
+                CHECK(l < tsize, "LDS element out of range");
+                switch (op.kind) {
+                case DK_M1: {
+                    uint32_t l1 = l | (1u << op.t[0]);
+                    cd x0 = T[l], x1 = T[l1];
+                    cd y0 = m[0] * x0 + m[1] * x1, y1 = m[2] * x0 + m[3] * x1;
+                    prune_ok(y0, p.prune);
+                    prune_ok(y1, p.prune);
+                    T[l] = y0;
+                    T[l1] = y1;
+                    break;
+                }
+                case DK_M2:
+                case DK_MK: {
+                    const int kk = op.k, dim = 1 << kk;
+                    std::vector<uint32_t> id(dim);
+                    std::vector<cd> x(dim);
+                    for (int r = 0; r < dim; r++) {
+                        uint32_t g = l;
+                        for (int bb = 0; bb < kk; bb++)
+                            if ((r >> (kk - 1 - bb)) & 1) g |= 1u << op.t[bb];
+                        id[r] = g;
+                        x[r] = T[g];
+                    }
+                    for (int r = 0; r < dim; r++) {
+                        cd acc = 0;
+                        for (int c = 0; c < dim; c++) acc += m[r * dim + c] * x[c];
+                        prune_ok(acc, p.prune);
+                        T[id[r]] = acc;
+                    }
+                    break;
+                }
+                case DK_DIAG: {
+                    uint32_t r = 0;
+                    for (int bb = 0; bb < op.k; bb++) {
+                        int bit;
+                        if (op.t[bb] >= 0) bit = (l >> op.t[bb]) & 1;
+                        else bit = (int)((base >> (-op.t[bb] - 1)) & 1);
+                        r |= (uint32_t)bit << (op.k - 1 - bb);
+                    }
+                    cd y = m[r] * T[l];
+                    prune_ok(y, p.prune);
+                    T[l] = y;
+                    break;
+                }
+                case DK_PERMX:
+                    std::swap(T[l], T[l | (1u << op.t[0])]);
+                    break;
+                case DK_SWAP:
+                    std::swap(T[l | (1u << op.t[0])], T[l | (1u << op.t[1])]);
+                    break;
+                }
+            }
+        }
+        for (uint32_t l = 0; l < tsize; l++) psi[base | scatter(l)] = T[l];
+    }
+    return 0;
+}
+
+static int emu_big(const Program& p, const Step& s, cd* psi, int n) {
+    const BigOp& b = p.big_ops[s.big_op];
+    const int k = (int)b.tg.size(), dim = 1 << k;
+    std::vector<int> srt = b.tg;
+    std::sort(srt.begin(), srt.end());
+    const cd* m = p.mats.data() + b.mat;
+    std::vector<cd> x(dim);
+    std::vector<uint64_t> gi(dim);
+    for (uint64_t g = 0; g < (1ull << (n - k)); g++) {
+        uint64_t base = g;
+        for (int q : srt) base = ins0(base, q);
+        for (int r = 0; r < dim; r++) {
+            uint64_t idx = base;
+            for (int bb = 0; bb < k; bb++)
+                if ((r >> (k - 1 - bb)) & 1) idx |= 1ull << b.tg[bb];
+            gi[r] = idx;
+            x[r] = psi[idx];
+        }
+        for (int r = 0; r < dim; r++) {
+            cd acc = 0;
+            for (int c = 0; c < dim; c++) acc += m[(size_t)r * dim + c] * x[c];
+            prune_ok(acc, p.prune);
+            psi[gi[r]] = acc;
+        }
+    }
+    return 0;
+}
+
+extern "C" {
+
+int emu_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        size_t m = std::min(len - 1, g_msg.size());
+        std::memcpy(buf, g_msg.data(), m);
+        buf[m] = 0;
+    }
+    return (int)g_msg.size();
+}
+
+// stats[0] passes, [1] register-round passes, [2] LDS transposes, [3] device ops
+int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel, int tile,
+            int rtile, int fuse, double* psi, int64_t* stats) {
+    Program p;
+    std::string err;
+    int rc = plan_program(p, n, KSV_F64, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, err);
+    if (rc) {
+        g_msg = err;
+        return rc;
+    }
+    cd* st = reinterpret_cast<cd*>(psi);
+    int64_t regs = 0, exch = 0;
+    for (const Step& s : p.steps) {
+        int r;
+        if (s.big) r = emu_big(p, s, st, n);
+        else if (s.reg) { r = emu_reg(p, s, st, n); regs++; exch += s.exchanges; }
+        else r = emu_lds(p, s, st, n);
+        if (r) return r;
+    }
+    if (stats) {
+        stats[0] = (int64_t)p.steps.size();
+        stats[1] = regs;
+        stats[2] = exch;
+        int64_t dev = (int64_t)(p.dops.size() + p.big_ops.size());
+        for (const ROp& r : p.rops)
+            if (r.kind != R_EXCH) dev++;
+        stats[3] = dev;
+    }
+    return 0;
+}
+
+}  // extern "C"

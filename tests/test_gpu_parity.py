@@ -20,9 +20,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", wave_regs=0):
+def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", rtile=0, prefetch=False):
     c = build(n, calls)
-    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse, kernel=kernel, wave_regs=wave_regs)
+    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, prefetch=prefetch)
     if init:
         st.init_basis(init)
     st.apply(c.tape())
@@ -73,23 +73,25 @@ def test_multi_pass_tiles(n, tile):
     assert np.max(np.abs(got - ref)) < TOL
 
 
-@pytest.mark.parametrize("n,regs", [(12, 4), (13, 5), (14, 4), (16, 5), (17, 4), (18, 5)])
+@pytest.mark.parametrize("n,rtile", [(11, 11), (12, 12), (13, 12), (13, 13), (14, 11), (16, 12), (17, 13), (18, 12)])
 @pytest.mark.parametrize("prune", [True, False])
-def test_register_tile_passes(n, regs, prune):
-    """Register-tile passes: lane swaps on every lane bit, slot/lane/global controls."""
-    calls = random_calls(n, 160, seed=31 * n + regs, with_unitary=True)
+@pytest.mark.parametrize("prefetch", [False, True])
+def test_register_round_passes(n, rtile, prune, prefetch):
+    """Register-round passes: LDS transposes, slot/thread/global controls, every op kind,
+    one tile per workgroup and persistent prefetching workgroups."""
+    calls = random_calls(n, 160, seed=31 * n + rtile, with_unitary=True)
     ref = dense_ref(n, calls, prune=prune)
-    got = gpu_state(n, calls, prune=prune, kernel="wave", wave_regs=regs)
+    got = gpu_state(n, calls, prune=prune, kernel="reg", rtile=rtile, prefetch=prefetch)
     assert np.max(np.abs(got - ref)) < TOL
 
 
-@pytest.mark.parametrize("regs", [4, 5])
-def test_register_tile_random_layers(regs):
+@pytest.mark.parametrize("rtile", [11, 12, 13])
+def test_register_round_random_layers(rtile):
     n = 16
     calls = workloads.random_layer_calls(n, 8, seed=17)
     ref = dense_ref(n, calls, prune=False)
     for fuse in (False, True):
-        got = gpu_state(n, calls, prune=False, kernel="wave", wave_regs=regs, fuse=fuse)
+        got = gpu_state(n, calls, prune=False, kernel="reg", rtile=rtile, fuse=fuse)
         assert np.max(np.abs(got - ref)) < TOL
 
 
