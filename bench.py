@@ -61,9 +61,10 @@ def main():
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-fuse", action="store_true")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "reg"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "reg", "mfma"])
+    ap.add_argument("--bq", type=int, default=0, help="fused-block width (4, 5)")
+    ap.add_argument("--btile", type=int, default=0, help="fused-block tile qubits (12, 13)")
     ap.add_argument("--rtile", type=int, default=0, help="register-round tile qubits (11..13)")
-    ap.add_argument("--prefetch", action="store_true", help="persistent prefetching workgroups")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -82,11 +83,12 @@ def main():
     ops, mats = circ.tape()
     gates = len(ops)
     st = State(n, "f64", prune=False, tile=args.tile, fuse=not args.no_fuse, kernel=args.kernel,
-               rtile=args.rtile, prefetch=args.prefetch)
+               rtile=args.rtile, bq=args.bq, btile=args.btile)
     prog = st.program((ops, mats))
     info = prog.info()
-    log(f"bench: {gates} gates -> {info['passes']} passes ({info['reg_passes']} register-round, "
-        f"{info['exchanges']} LDS transposes) of {info['tile_qubits']}-qubit tiles")
+    log(f"bench: {gates} gates -> {info['passes']} passes ({info['reg_passes']} register-round with "
+        f"{info['exchanges']} LDS transposes, {info['blk_passes']} fused-block with {info['blocks']} blocks) "
+        f"of {info['tile_qubits']}-qubit tiles")
     for w in range(args.warmup):
         st.init_basis(0)
         prog.run()
@@ -116,7 +118,8 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"k_pass_f64_{n}q"),
-                "kernel": "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass", "bytes_per_launch": per_launch,
+                "kernel": ("ksv::k_bpass" if info["blk_passes"] else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass"),
+                "bytes_per_launch": per_launch,
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps}
     line = {"metric": METRIC, "value": round(value, 2), "unit": "gate-updates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",

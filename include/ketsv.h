@@ -89,11 +89,14 @@ typedef struct ksv_program ksv_program;
 #define KSV_OPT_TIMING       2  /* 1: time every pass launch with HIP events on the state stream  */
 #define KSV_OPT_FUSE         3  /* 1: multiply runs of gates on the same qubits into one matrix
                                    (exact mode only; ket's per-gate pruning forbids it)            */
-#define KSV_OPT_PASS_KERNEL  4  /* 0 auto, 1 LDS-tile passes, 2 register-round passes            */
+#define KSV_OPT_PASS_KERNEL  4  /* 0 auto (3 in exact mode, 2 under ket pruning), 1 LDS-tile
+                                   passes, 2 register-round passes, 3 fused-block passes on
+                                   MFMA (exact mode; ket pruning falls back to 2)              */
 #define KSV_OPT_ROUND_TILE   5  /* tile qubits of register-round passes: 0 = default (11 FP64,
                                    12 FP32) or 11..13; 2^(tile-4) threads x 16 amplitudes     */
-#define KSV_OPT_PREFETCH     6  /* 1: register-round passes run persistent workgroups that load
-                                   their next tile while computing the current one            */
+#define KSV_OPT_BLOCK_QUBITS 6  /* fused-block width: 0 = default (4) or 4, 5                  */
+#define KSV_OPT_BLOCK_TILE   7  /* fused-block tile qubits: 0 = default (12 FP64, 13 FP32) or
+                                   12, 13 (5-qubit blocks need 13)                             */
 
 /* Per-state launch statistics (KSV_OPT_TIMING). */
 typedef struct ksv_stats {
@@ -114,6 +117,9 @@ typedef struct ksv_program_summary {
     int32_t  tile_qubits;       /* qubits per tile of the main pass kind    */
     int32_t  reg_passes;        /* how many passes are register-round passes */
     uint64_t exchanges;         /* LDS transposes over all register-round passes */
+    int32_t  blk_passes;        /* how many passes are fused-block passes    */
+    int32_t  pad;
+    uint64_t blocks;            /* fused dense blocks over all fused-block passes */
 } ksv_program_summary;
 
 /* ---- library ------------------------------------------------------------ */
@@ -146,6 +152,10 @@ int ksv_program_destroy(ksv_program* p);
  * autotuner. */
 int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
              int kernel, int tile, int rtile, int fuse, int prune, ksv_program_summary* out);
+/* ksv_plan with the fused-block tile width and block width (0 = defaults). */
+int ksv_plan_blocks(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                    int kernel, int tile, int rtile, int btile, int bq, int fuse, int prune,
+                    ksv_program_summary* out);
 
 /* ---- read-out ---------------------------------------------------------------- */
 /* Dense amplitudes [offset, offset+count) as interleaved doubles (statevector(),

@@ -17,7 +17,7 @@ KSV_F64, KSV_F32 = 0, 1
  KSV_UNITARY, KSV_BARRIER) = range(10)
 KSV_MAX_OPERANDS = 12
 (KSV_OPT_PRUNE, KSV_OPT_TILE_QUBITS, KSV_OPT_TIMING, KSV_OPT_FUSE, KSV_OPT_PASS_KERNEL, KSV_OPT_ROUND_TILE,
- KSV_OPT_PREFETCH) = range(7)
+ KSV_OPT_BLOCK_QUBITS, KSV_OPT_BLOCK_TILE) = range(8)
 
 OP_DTYPE = np.dtype([("kind", "<i4"), ("nq", "<i4"), ("q", "<i4", (KSV_MAX_OPERANDS,)), ("mat", "<i8")])
 PARAM_DTYPE = np.dtype([("op", "<i4"), ("param", "<i4"), ("coeff", "<f8"), ("dmat", "<i8")])
@@ -36,7 +36,8 @@ class KsvStats(ctypes.Structure):
 
 class KsvProgramSummary(ctypes.Structure):
     _fields_ = [("ops", c_uint64), ("passes", c_uint64), ("device_ops", c_uint64),
-                ("tile_qubits", c_int32), ("reg_passes", c_int32), ("exchanges", c_uint64)]
+                ("tile_qubits", c_int32), ("reg_passes", c_int32), ("exchanges", c_uint64),
+                ("blk_passes", c_int32), ("pad", c_int32), ("blocks", c_uint64)]
 
 
 # (name, restype, argtypes) of every entry point in include/ketsv.h
@@ -58,6 +59,8 @@ SIGNATURES = [
     ("ksv_program_destroy", c_int, [_P]),
     ("ksv_plan", c_int, [c_int, c_int, _P, c_size_t, _P, c_size_t, c_int, c_int, c_int, c_int, c_int,
                           POINTER(KsvProgramSummary)]),
+    ("ksv_plan_blocks", c_int, [c_int, c_int, _P, c_size_t, _P, c_size_t, c_int, c_int, c_int, c_int, c_int, c_int,
+                                 c_int, POINTER(KsvProgramSummary)]),
     ("ksv_get_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
     ("ksv_set_amplitudes", c_int, [_P, c_uint64, c_uint64, _P]),
     ("ksv_amplitude", c_int, [_P, c_uint64, _P]),
@@ -118,16 +121,16 @@ def device_count():
     return c.value
 
 
-KERNELS = {"auto": 0, "lds": 1, "reg": 2}
+KERNELS = {"auto": 0, "lds": 1, "reg": 2, "mfma": 3}
 
 
-def plan(n, tape, precision="f64", kernel="auto", tile=0, rtile=0, fuse=False, prune=True):
+def plan(n, tape, precision="f64", kernel="auto", tile=0, rtile=0, fuse=False, prune=True, btile=0, bq=0):
     """Dry-run schedule of a tape (no device needed)."""
     ops, mats = tape
     s = KsvProgramSummary()
-    check(lib().ksv_plan(int(n), KSV_F64 if precision == "f64" else KSV_F32, ptr(ops), len(ops), ptr(mats),
-                         len(mats), KERNELS[kernel], int(tile), int(rtile), 1 if fuse else 0, 1 if prune else 0,
-                         ctypes.byref(s)))
+    check(lib().ksv_plan_blocks(int(n), KSV_F64 if precision == "f64" else KSV_F32, ptr(ops), len(ops), ptr(mats),
+                                len(mats), KERNELS[kernel], int(tile), int(rtile), int(btile), int(bq),
+                                1 if fuse else 0, 1 if prune else 0, ctypes.byref(s)))
     return {f: getattr(s, f) for f, _ in KsvProgramSummary._fields_}
 
 
@@ -142,7 +145,7 @@ class State:
     """Owning handle of one device statevector (2^n amplitudes in HBM)."""
 
     def __init__(self, n, precision="f64", prune=True, tile=0, fuse=False, timing=False, kernel="auto",
-                 rtile=0, prefetch=False):
+                 rtile=0, bq=0, btile=0):
         self.n = int(n)
         self.precision = precision
         prec = KSV_F64 if precision == "f64" else KSV_F32
@@ -160,8 +163,10 @@ class State:
             self.set_option(KSV_OPT_PASS_KERNEL, KERNELS[kernel])
         if rtile:
             self.set_option(KSV_OPT_ROUND_TILE, rtile)
-        if prefetch:
-            self.set_option(KSV_OPT_PREFETCH, 1)
+        if bq:
+            self.set_option(KSV_OPT_BLOCK_QUBITS, bq)
+        if btile:
+            self.set_option(KSV_OPT_BLOCK_TILE, btile)
 
     @property
     def handle(self):

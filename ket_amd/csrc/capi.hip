@@ -47,7 +47,8 @@ struct ksv_state {
     int fuse = 0;
     int kernel = 0;                 // KSV_OPT_PASS_KERNEL
     int rtile = 0;                  // KSV_OPT_ROUND_TILE
-    int prefetch = 0;               // KSV_OPT_PREFETCH
+    int bq = 0;                     // KSV_OPT_BLOCK_QUBITS
+    int btile = 0;                  // KSV_OPT_BLOCK_TILE
     ksv_stats stats;
     std::vector<hipEvent_t> ev;
 };
@@ -135,11 +136,22 @@ int ksv_set_option(ksv_state* s, int option, double value) {
     case KSV_OPT_FUSE: s->fuse = value != 0; return KSV_OK;
     case KSV_OPT_PASS_KERNEL: {
         int k = (int)value;
-        if (k < 0 || k > 2) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1 or 2");
+        if (k < 0 || k > 3) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1, 2 or 3");
         s->kernel = k;
         return KSV_OK;
     }
-    case KSV_OPT_PREFETCH: s->prefetch = value != 0; return KSV_OK;
+    case KSV_OPT_BLOCK_QUBITS: {
+        int b = (int)value;
+        if (b != 0 && b != 4 && b != 5) return fail(KSV_ERR_RANGE, "fused-block width must be 0, 4 or 5");
+        s->bq = b;
+        return KSV_OK;
+    }
+    case KSV_OPT_BLOCK_TILE: {
+        int t = (int)value;
+        if (t != 0 && t != 12 && t != 13) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0, 12 or 13");
+        s->btile = t;
+        return KSV_OK;
+    }
     case KSV_OPT_ROUND_TILE: {
         int r = (int)value;
         if (r != 0 && (r < 11 || r > 13)) return fail(KSV_ERR_RANGE, "register-round tile qubits must be 0 or 11..13");
@@ -172,9 +184,11 @@ int ksv_copy_state(ksv_state* dst, const ksv_state* src) {
 // --------------------------------------------------------------------------
 // Lower and schedule a tape into `p` (host side only).
 static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                     int kernel, int tile, int rtile, bool fuse, int prune) {
+                     int kernel, int tile, int rtile, bool fuse, int prune, int btile = 0, int bq = 0) {
     std::string err;
-    int rc = plan_program(p, n, prec, ops, nops, mats, nmats, kernel, tile, rtile, fuse, prune, err);
+    int bqv = bq > 0 ? bq : 4, btv = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
+    if (kernel == 3 && bqv == 5 && btv != 13) return fail(KSV_ERR_RANGE, "5-qubit blocks need 13-qubit tiles");
+    int rc = plan_program(p, n, prec, ops, nops, mats, nmats, kernel, tile, rtile, fuse, prune, err, btile, bq);
     if (rc) return fail(rc, err);
     return KSV_OK;
 }
@@ -183,10 +197,9 @@ static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const dou
                          int prune, bool fuse, ksv_program** out) {
     std::unique_ptr<ksv_program> p(new ksv_program());
     int rc = plan_into(*p, s->n, s->prec, ops, nops, mats, nmats, s->kernel, default_tile(s), s->rtile, fuse,
-                       prune);
+                       prune, s->btile, s->bq);
     if (rc) return rc;
     p->dev = s->dev;
-    p->prefetch = s->prefetch;
     if (!p->rops.empty()) {
         HIPCHK(hipMalloc(&p->d_rops, sizeof(ROp) * p->rops.size()));
         HIPCHK(hipMemcpy(p->d_rops, p->rops.data(), sizeof(ROp) * p->rops.size(), hipMemcpyHostToDevice));
@@ -194,6 +207,31 @@ static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const dou
     if (!p->dops.empty()) {
         HIPCHK(hipMalloc(&p->d_ops, sizeof(DOp) * p->dops.size()));
         HIPCHK(hipMemcpy(p->d_ops, p->dops.data(), sizeof(DOp) * p->dops.size(), hipMemcpyHostToDevice));
+    }
+    if (!p->brecs.empty()) {
+        HIPCHK(hipMalloc(&p->d_brecs, sizeof(BRec) * p->brecs.size()));
+        HIPCHK(hipMemcpy(p->d_brecs, p->brecs.data(), sizeof(BRec) * p->brecs.size(), hipMemcpyHostToDevice));
+        size_t total = 0;
+        std::vector<int> bqs(p->brecs.size(), 4);
+        for (const Step& st : p->steps)
+            if (st.blk)
+                for (int i = 0; i < st.op_count; i++) bqs[st.op_begin + i] = st.bq;
+        for (size_t i = 0; i < p->brecs.size(); i++) {
+            const int R = 1 << bqs[i];
+            total = std::max(total, (size_t)p->brecs[i].frag + (size_t)(2 * R / 16) * (2 * R / 4) * 64);
+        }
+        std::vector<double> fr(total, 0.0);
+        for (size_t i = 0; i < p->brecs.size(); i++)
+            block_fragments(p->bmats.data() + p->bmat_off[i], bqs[i], fr.data() + p->brecs[i].frag);
+        if (s->prec == KSV_F64) {
+            HIPCHK(hipMalloc(&p->d_frags, 8 * total));
+            HIPCHK(hipMemcpy(p->d_frags, fr.data(), 8 * total, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> ff(total);
+            for (size_t i = 0; i < total; i++) ff[i] = (float)fr[i];
+            HIPCHK(hipMalloc(&p->d_frags, 4 * total));
+            HIPCHK(hipMemcpy(p->d_frags, ff.data(), 4 * total, hipMemcpyHostToDevice));
+        }
     }
     if (!p->mats.empty()) {
         if (s->prec == KSV_F64) {
@@ -228,6 +266,7 @@ static int run_program_on(ksv_state* s, ksv_program* p, void* psi) {
         const Step& st = p->steps[i];
         if (st.big) HIPCHK(launch_big(*p, st, psi, p->prune, s->st));
         else if (st.reg) HIPCHK(launch_rpass(*p, st, psi, s->st));
+        else if (st.blk) HIPCHK(launch_bpass(*p, st, psi, s->st));
         else HIPCHK(launch_pass(*p, st, psi, s->st));
         if (timing) HIPCHK(hipEventRecord(s->ev[i + 1], s->st));
     }
@@ -268,12 +307,19 @@ static int summarize(const Program& p, ksv_program_summary* out) {
     out->ops = p.tape_ops;
     out->passes = p.steps.size();
     out->device_ops = p.dops.size() + p.big_ops.size();
-    int regs = 0, exch = 0;
-    for (const Step& st : p.steps)
+    int regs = 0, exch = 0, blks = 0;
+    for (const Step& st : p.steps) {
         if (st.reg) { regs++; exch += st.exchanges; }
+        if (st.blk) blks++;
+    }
+    out->blk_passes = blks;
+    out->blocks = p.brecs.size();
+    out->device_ops += p.brecs.size();
     for (const ROp& r : p.rops)
         if (r.kind != R_EXCH) out->device_ops++;
-    out->tile_qubits = regs ? p.rtile : p.tile;
+    out->tile_qubits = blks ? p.steps[0].k : (regs ? p.rtile : p.tile);
+    for (const Step& st : p.steps)
+        if (st.blk) { out->tile_qubits = st.k; break; }
     out->reg_passes = regs;
     out->exchanges = exch;
     return KSV_OK;
@@ -288,6 +334,8 @@ int ksv_program_destroy(ksv_program* p) {
     if (!p) return fail(KSV_ERR_HANDLE, "null program");
     if (p->d_ops) hipFree(p->d_ops);
     if (p->d_rops) hipFree(p->d_rops);
+    if (p->d_brecs) hipFree(p->d_brecs);
+    if (p->d_frags) hipFree(p->d_frags);
     if (p->d_mats) hipFree(p->d_mats);
     delete p;
     return KSV_OK;
@@ -295,13 +343,20 @@ int ksv_program_destroy(ksv_program* p) {
 
 int ksv_plan(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel,
              int tile, int rtile, int fuse, int prune, ksv_program_summary* out) {
+    return ksv_plan_blocks(n, precision, ops, nops, mats, nmats, kernel, tile, rtile, 0, 0, fuse, prune, out);
+}
+
+int ksv_plan_blocks(int n, int precision, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                    int kernel, int tile, int rtile, int btile, int bq, int fuse, int prune, ksv_program_summary* out) {
     if (!out) return fail(KSV_ERR_TYPE, "null output pointer");
     if (n < 1 || n > MAXN) return fail(KSV_ERR_RANGE, "qubit count out of range");
     if (precision != KSV_F64 && precision != KSV_F32) return fail(KSV_ERR_TYPE, "unknown precision");
-    if (kernel < 0 || kernel > 2) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1 or 2");
+    if (kernel < 0 || kernel > 3) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1, 2 or 3");
+    if (bq != 0 && bq != 4 && bq != 5) return fail(KSV_ERR_RANGE, "fused-block width must be 0, 4 or 5");
+    if (btile != 0 && btile != 12 && btile != 13) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0, 12 or 13");
     Program p;
     if (rtile != 0 && (rtile < 11 || rtile > 13)) return fail(KSV_ERR_RANGE, "register-round tile qubits must be 0 or 11..13");
-    int rc = plan_into(p, n, precision, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune);
+    int rc = plan_into(p, n, precision, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, btile, bq);
     if (rc) return rc;
     return summarize(p, out);
 }

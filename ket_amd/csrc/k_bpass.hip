@@ -1,0 +1,214 @@
+// Fused-block pass kernel: dense 2^BQ x 2^BQ blocks applied on MFMA.
+//
+// A workgroup of 256 threads stages one K-qubit tile (the amplitudes whose
+// non-tile bits equal its base index) in LDS.  For every block of the pass
+// the tile is a complex matrix X with 2^BQ rows (the block's local index,
+// bit i = block qubit i) and 2^(K-BQ) columns (the other tile bits), and the
+// block computes Y = U X as one real GEMM on the real embedding
+//
+//     [Yr]   [Ur  -Ui] [Xr]
+//     [Yi] = [Ui   Ur] [Xi]
+//
+// with v_mfma_{f64,f32}_16x16x4: each wave takes 16-column slices, reads its
+// B fragments (lane l: column l%16, rows 4t + l/16) from LDS, accumulates
+// 2^(BQ+1)/16 output tiles over 2^(BQ+1)/4 k-steps, and writes the result
+// back in place (lane l, element v: row 4v + l/16 for f64, 4(l/16) + v
+// for f32).  A fragments come pre-arranged
+// per lane from the host.  Columns differ between waves, so blocks need a
+// barrier only between them.  The LDS address of tile index x is x with its
+// low four bits XORed by a per-pass word of its higher bits, chosen so that
+// each 16-lane group hits 16 distinct 16-byte bank groups.  HBM sees one
+// coalesced read and one coalesced write of the tile per pass.
+#include <cstring>
+
+#include "kcommon.h"
+
+namespace ksv {
+
+template <typename T> struct BVec2;
+template <> struct BVec2<double> { using type = double2; };
+template <> struct BVec2<float> { using type = float2; };
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// Output layout of 16x16x4 (measured on gfx950): D[i][j] sits in lane
+// j + 16 * (i % 4), element i / 4 for f64, and in lane j + 16 * (i / 4),
+// element i % 4 for f32.  A[i][k] is lane i + 16k, B[k][j] lane j + 16k.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    static constexpr bool lane_is_row_low = true;
+    using acc = d4_t;
+    static __device__ __forceinline__ acc step(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    static constexpr bool lane_is_row_low = false;
+    using acc = f4_t;
+    static __device__ __forceinline__ acc step(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ uint32_t swz_part(uint32_t x, uint64_t w, int lo, int hi) {
+    // XOR of the swizzle words of the set bits p in [lo, hi) of x (p >= 4)
+    uint32_t g = 0;
+#pragma unroll
+    for (int p = 4; p < 16; p++)
+        if (p >= lo && p < hi && ((x >> p) & 1)) g ^= (uint32_t)((w >> (4 * (p - 4))) & 15);
+    return g;
+}
+
+template <typename T, int BQ, int K>
+__global__ __launch_bounds__(256) void k_bpass(BPassArgs a) {
+    using V = typename BVec2<T>::type;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc;
+    constexpr int NT = 256;
+    constexpr int R = 1 << BQ;          // complex rows
+    constexpr int MB = 2 * R / 16;      // output 16-row tiles (real embedding)
+    constexpr int KS = 2 * R / 4;       // k-steps
+    constexpr int NC = 1 << (K - BQ);   // columns
+    constexpr int NBW = NC / 64;        // 16-column slices per wave
+    constexpr int PER = (1 << K) / NT;  // amplitudes per thread at load / store
+    static_assert(NBW >= 1, "tile too small for the block width");
+    __shared__ V lds[1 << K];
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    V* psi = reinterpret_cast<V*>(a.psi);
+    const T* frags = reinterpret_cast<const T*>(a.frags);
+    const KSV_CONST uint64_t* recs = to_const(reinterpret_cast<const uint64_t*>(a.recs));
+
+    uint64_t base = blockIdx.x;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (i < K) base = ins0_64(base, byte_at(i < 8 ? a.tq_lo : a.tq_hi, i & 7));
+
+    // tile index x = tid + i * NT: thread part and uniform part of the
+    // physical offset and of the swizzled LDS address
+    uint64_t g_t = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(a.tq_lo, b);
+    const uint32_t l_t = tid ^ swz_part(tid, a.swz, 4, 8);
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t xh = (uint32_t)i * NT;
+        uint64_t g_h = 0;
+#pragma unroll
+        for (int b = 8; b < K; b++)
+            if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
+        lds[l_t ^ xh ^ swz_part(xh, a.swz, 8, K)] = psi[base | g_t | g_h];
+    }
+    __syncthreads();
+
+    const uint32_t rl = lane >> 4, cl = lane & 15;
+    for (int blk = 0; blk < a.nblk; blk++) {
+        const KSV_CONST uint64_t* w = recs + 8 * blk;
+        uint16_t ar[8], ac[16];
+        {
+            const uint64_t r0 = w[0], r1 = w[1], r2 = w[2], r3 = w[3], r4 = w[4], r5 = w[5];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                ar[i] = (uint16_t)(r0 >> (16 * i));
+                ar[4 + i] = (uint16_t)(r1 >> (16 * i));
+                ac[i] = (uint16_t)(r2 >> (16 * i));
+                ac[4 + i] = (uint16_t)(r3 >> (16 * i));
+                ac[8 + i] = (uint16_t)(r4 >> (16 * i));
+                ac[12 + i] = (uint16_t)(r5 >> (16 * i));
+            }
+        }
+        const int64_t foff = (int64_t)w[6];
+        T af[MB * KS];
+#pragma unroll
+        for (int f = 0; f < MB * KS; f++) af[f] = frags[foff + f * 64 + lane];
+
+        // read rows r = 4t + rl: bits 0,1 from rl, the rest from t
+        const uint32_t rd_l = ((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u);
+        // write rows r = 16 h + (4 v + rl) (f64) or 16 h + (4 rl + v) (f32)
+        constexpr bool LOW = M::lane_is_row_low;
+        const uint32_t wr_l = LOW ? (((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u))
+                                  : (((rl & 1) ? ar[2] : 0u) ^ ((rl & 2) ? ar[3] : 0u));
+        const uint32_t col_l = ((cl & 1) ? ac[0] : 0u) ^ ((cl & 2) ? ac[1] : 0u) ^ ((cl & 4) ? ac[2] : 0u) ^
+                               ((cl & 8) ? ac[3] : 0u);
+#pragma unroll
+        for (int m = 0; m < NBW; m++) {
+            const uint32_t nb = wave + 4 * m;
+            uint32_t col = col_l;
+#pragma unroll
+            for (int b = 0; b < K - BQ - 4; b++)
+                if ((nb >> b) & 1) col ^= ac[4 + b];
+            V x[R / 4];
+#pragma unroll
+            for (int t = 0; t < R / 4; t++) {
+                uint32_t ra = rd_l;
+#pragma unroll
+                for (int b = 0; b < BQ - 2; b++)
+                    if ((t >> b) & 1) ra ^= ar[2 + b];
+                x[t] = lds[ra ^ col];
+            }
+            acc_t d[MB];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) d[mb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < KS; kk++) {
+                const T bv = kk < R / 4 ? x[kk].x : x[kk - R / 4].y;
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) d[mb] = M::step(af[mb * KS + kk], bv, d[mb]);
+            }
+#pragma unroll
+            for (int h = 0; h < R / 16; h++) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    uint32_t wa = wr_l ^ (LOW ? (((v & 1) ? ar[2] : 0u) ^ ((v & 2) ? ar[3] : 0u))
+                                              : (((v & 1) ? ar[0] : 0u) ^ ((v & 2) ? ar[1] : 0u)));
+                    if (h) wa ^= ar[4];
+                    V y;
+                    y.x = d[h][v];
+                    y.y = d[h + R / 16][v];
+                    lds[wa ^ col] = y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t xh = (uint32_t)i * NT;
+        uint64_t g_h = 0;
+#pragma unroll
+        for (int b = 8; b < K; b++)
+            if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
+        psi[base | g_t | g_h] = lds[l_t ^ xh ^ swz_part(xh, a.swz, 8, K)];
+    }
+}
+
+template <typename T>
+static hipError_t launch_bpass_t(const Program& p, const Step& s, void* psi, hipStream_t st) {
+    BPassArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.psi = psi;
+    a.recs = p.d_brecs + s.op_begin;
+    a.frags = p.d_frags;
+    a.nblk = s.op_count;
+    a.k = s.k;
+    for (int i = 0; i < 8; i++) {
+        a.tq_lo |= (uint64_t)(uint8_t)s.tq[i] << (8 * i);
+        a.tq_hi |= (uint64_t)(uint8_t)s.tq[8 + i] << (8 * i);
+    }
+    a.swz = s.swz;
+    const dim3 grid((unsigned)(1ull << (p.n - s.k))), block(256);
+    if (s.bq == 4 && s.k == 12) hipLaunchKernelGGL((k_bpass<T, 4, 12>), grid, block, 0, st, a);
+    else if (s.bq == 4 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 4, 13>), grid, block, 0, st, a);
+    else if (s.bq == 5 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 5, 13>), grid, block, 0, st, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_bpass(const Program& p, const Step& s, void* psi, hipStream_t st) {
+    if (p.precision == KSV_F64) return launch_bpass_t<double>(p, s, psi, st);
+    return launch_bpass_t<float>(p, s, psi, st);
+}
+
+}  // namespace ksv

@@ -13,12 +13,13 @@
 // assignment and reads it back under the next one (XOR-swizzled so that both
 // sides are free of bank conflicts).
 //
-// Workgroups are persistent: each walks tiles blockIdx.x, +gridDim.x, ...
-// and issues the loads of its next tile before applying the ops of the
-// current one, so HBM traffic overlaps the arithmetic.  HBM sees one read
-// and one write of every amplitude per pass, coalesced: thread bits 0..2 are
-// physical qubits 0..2 at load and store, so each 8 lanes move a full
-// 128-byte run.
+// One tile per workgroup; the workgroups resident on a CU overlap each
+// other's HBM traffic and arithmetic.  (Persistent workgroups that prefetch
+// their next tile into registers measured slower: the second register set
+// pushes the kernel to 256 VGPRs and the compiler waits on the prefetch
+// before the first gate.)  HBM sees one read and one write of every
+// amplitude per pass, coalesced: thread bits 0..2 are physical qubits 0..2
+// at load and store, so each 8 lanes move a full 128-byte run.
 //
 // Op records (128 bytes, matrix inline) are read through the constant
 // address space: uniform scalar loads into SGPRs.
@@ -290,9 +291,7 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
     return ro;
 }
 
-// PF: persistent workgroups that prefetch their next tile into registers;
-// otherwise one tile per workgroup.
-template <typename T, int TB, bool PR, bool PF>
+template <typename T, int TB, bool PR>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     using V = typename Vec2<T>::type;
     constexpr int K = TB + RSLOTS;
@@ -317,21 +316,7 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
         im[j] = t.y;
     }
 
-    for (;;) {
-        const uint64_t next = tile + gridDim.x;
-        const bool more = PF && next < a.ntiles;
-        uint64_t nbase = 0;
-        T nre[RREGS], nim[RREGS];
-        if (PF && more) {
-            nbase = tile_base(next, K, a.tq_lo, a.tq_hi);
-#pragma unroll
-            for (int j = 0; j < RREGS; j++) {
-                const V t = psi[nbase | ld_off | slot_offset<T>(j, a.ld_r)];
-                nre[j] = t.x;
-                nim[j] = t.y;
-            }
-        }
-
+    {
         for (int o = 0; o < a.nops; o++) {
             const KSV_CONST uint64_t* w = ops + 16 * o;
             const uint64_t h0 = w[0], h1 = w[1], h2 = w[2], h3 = w[3];
@@ -396,46 +381,15 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
             t.y = im[j];
             psi[base | st_off | slot_offset<T>(j, a.st_r)] = t;
         }
-        if (!PF || !more) break;
-        tile = next;
-        base = nbase;
-#pragma unroll
-        for (int j = 0; j < RREGS; j++) {
-            re[j] = nre[j];
-            im[j] = nim[j];
-        }
     }
-}
-
-template <typename T, int TB, bool PR, bool PF>
-static void launch_one(const RPassArgs& a, hipStream_t st) {
-    uint64_t blocks = a.ntiles;
-    if (PF) {
-        static int resident = 0;      // workgroups resident on the device at once
-        if (!resident) {
-            int dev = 0, cus = 0, per = 0;
-            hipGetDevice(&dev);
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_rpass<T, TB, PR, PF>),
-                                                         1 << TB, 0);
-            resident = std::max(1, cus) * std::max(1, per);
-        }
-        blocks = std::min<uint64_t>(a.ntiles, (uint64_t)resident);
-    }
-    hipLaunchKernelGGL((k_rpass<T, TB, PR, PF>), dim3((unsigned)blocks), dim3(1 << TB), 0, st, a);
 }
 
 template <typename T, bool PR>
-static void launch_tb(const RPassArgs& a, int tb, bool pf, hipStream_t st) {
-    if (pf) {
-        if (tb == 7) launch_one<T, 7, PR, true>(a, st);
-        else if (tb == 8) launch_one<T, 8, PR, true>(a, st);
-        else launch_one<T, 9, PR, true>(a, st);
-    } else {
-        if (tb == 7) launch_one<T, 7, PR, false>(a, st);
-        else if (tb == 8) launch_one<T, 8, PR, false>(a, st);
-        else launch_one<T, 9, PR, false>(a, st);
-    }
+static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
+    const dim3 grid((unsigned)a.ntiles);
+    if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR>), grid, dim3(1 << 7), 0, st, a);
+    else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR>), grid, dim3(1 << 8), 0, st, a);
+    else hipLaunchKernelGGL((k_rpass<T, 9, PR>), grid, dim3(1 << 9), 0, st, a);
 }
 
 hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st) {
@@ -462,13 +416,12 @@ hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t 
     a.st_t_hi = tb > 8 ? pack(s.st_t + 8, tb - 8) : 0;
     a.ld_r = (uint32_t)pack(s.ld_r, RSLOTS);
     a.st_r = (uint32_t)pack(s.st_r, RSLOTS);
-    const bool pf = p.prefetch != 0;
     if (p.precision == KSV_F64) {
-        if (p.prune) launch_tb<double, true>(a, tb, pf, st);
-        else launch_tb<double, false>(a, tb, pf, st);
+        if (p.prune) launch_tb<double, true>(a, tb, st);
+        else launch_tb<double, false>(a, tb, st);
     } else {
-        if (p.prune) launch_tb<float, true>(a, tb, pf, st);
-        else launch_tb<float, false>(a, tb, pf, st);
+        if (p.prune) launch_tb<float, true>(a, tb, st);
+        else launch_tb<float, false>(a, tb, st);
     }
     return hipGetLastError();
 }

@@ -93,6 +93,31 @@ struct ROp {
 };
 static_assert(sizeof(ROp) == 128, "ROp layout");
 
+// ---------------------------------------------------------------------------
+// Fused-block pass: the circuit is fused on the host into dense blocks of
+// BQ (4 or 5) qubits; a pass stages a K-qubit tile in LDS and applies its
+// blocks as real GEMMs on MFMA:  [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi],
+// rows = the block's local index (bit i = qubit q[i]), columns = the other
+// K - BQ tile bits.
+// ---------------------------------------------------------------------------
+struct BRec {
+    uint16_t ar[8];    // LDS address (swizzled) contribution of row bit i, i < BQ
+    uint16_t ac[16];   // ... of column bit i, i < K - BQ
+    int64_t frag;      // offset (elements) of the block's A fragments: [mb][kk][lane]
+    int64_t pad;
+};
+static_assert(sizeof(BRec) == 64, "BRec layout");
+
+struct BPassArgs {
+    void* psi;
+    const BRec* recs;
+    const void* frags;
+    int nblk;
+    int k;
+    uint64_t tq_lo, tq_hi;      // physical bit of tile bit i, ascending
+    uint64_t swz;               // 4-bit XOR word of tile bit p (p >= 4) at bits 4(p-4)
+};
+
 // Byte i of a packed word holds a physical state bit (arrays inside a
 // by-value kernel argument would be copied to scratch; packed words stay in
 // scalar registers).
@@ -139,6 +164,9 @@ struct LOp {
 struct Step {
     bool big = false;        // standalone unitary step
     bool reg = false;        // register-round pass (ROps) instead of an LDS pass (DOps)
+    bool blk = false;        // fused-block MFMA pass (BRecs)
+    int bq = 0;              // fused-block pass: block width
+    uint64_t swz = 0;        // fused-block pass: LDS swizzle word
     int k = 0;               // tile qubits
     int8_t tq[16] = {0};
     int op_begin = 0, op_count = 0;   // DOps (LDS pass) or ROps (register-round pass)
@@ -164,8 +192,12 @@ struct Program {
     std::vector<ROp> rops;
     std::vector<BigOp> big_ops;
     int rtile = 0;           // > 0: tile qubits of the register-round passes
-    int prefetch = 0;        // register-round passes: persistent workgroups prefetching their next tile
     ROp* d_rops = nullptr;
+    std::vector<BRec> brecs;         // fused-block passes
+    std::vector<cd> bmats;           // their dense block matrices (2^bq x 2^bq each, row-major)
+    std::vector<int64_t> bmat_off;   // per BRec: offset of its matrix in bmats
+    BRec* d_brecs = nullptr;
+    void* d_frags = nullptr;
     std::vector<cd> mats;
     // device copies
     DOp* d_ops = nullptr;
@@ -178,15 +210,22 @@ int lower_tape(int n, const ksv_op* ops, size_t nops, const double* mats, size_t
                std::vector<LOp>& out, std::string& err);
 void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog);
 void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Program& prog);
+void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& prog);
+// A fragments of one block for MFMA 16x16x4: [mb][kk][lane] real values of
+// the real embedding of the 2^bq x 2^bq matrix u (row-major).
+void block_fragments(const cd* u, int bq, double* out);
 // Lower and schedule a tape into `p` (host side only).  kernel: KSV_OPT_PASS_KERNEL
-// value; tile / rtile: LDS and register-round tile widths (0 = defaults).
+// value; tile / rtile / btile: LDS, register-round and fused-block tile widths;
+// bq: fused-block width (0 = defaults).
 int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err);
+                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err, int btile = 0,
+                 int bq = 0);
 
 // kernels (k_pass.hip / k_reduce.hip)
 hipError_t launch_pass(const Program& p, const Step& s, void* psi, hipStream_t st);
 hipError_t launch_big(const Program& p, const Step& s, void* psi, int prune, hipStream_t st);
 hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st);
+hipError_t launch_bpass(const Program& p, const Step& s, void* psi, hipStream_t st);
 
 struct Scratch {
     double* partial = nullptr;     // reduction partials

@@ -703,8 +703,295 @@ void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Progra
     }
 }
 
+// --------------------------------------------------------------------------
+// Fused-block passes (exact mode).
+//
+// Fusion: every op joins the best existing block it may join, else opens a
+// new one.  With b the latest block touching any qubit of the op, the op
+// may join any block c >= b whose qubit set stays within bq qubits: blocks
+// after b touch none of its qubits, so moving the op to block c keeps every
+// dependence.  Blocks are then packed into passes over K-qubit tiles.
+// --------------------------------------------------------------------------
+struct FBlock {
+    std::vector<int> q;       // row bit i = qubit q[i]
+    std::vector<cd> u;        // 2^|q| x 2^|q|, row-major
+    int lop = -1;             // >= 0: a standalone op too wide for a block
+    uint64_t touch = 0;
+};
+
+// Apply a logical op to a small dense vector over qubits Q (bit i = Q[i]).
+static void apply_small(const LOp& o, const std::vector<int>& Q, cd* v) {
+    auto bit = [&](int q) {
+        for (size_t i = 0; i < Q.size(); i++)
+            if (Q[i] == q) return (int)i;
+        return -1;
+    };
+    const uint32_t dim = 1u << Q.size();
+    uint32_t cm = 0;
+    for (int c : o.ctl) cm |= 1u << bit(c);
+    switch (o.cls) {
+    case L_MAT: {
+        const int k = (int)o.tg.size(), kd = 1 << k;
+        uint32_t tm = 0;
+        int tb[16];
+        for (int i = 0; i < k; i++) { tb[i] = bit(o.tg[i]); tm |= 1u << tb[i]; }
+        std::vector<cd> x(kd), y(kd);
+        std::vector<uint32_t> id(kd);
+        for (uint32_t b0 = 0; b0 < dim; b0++) {
+            if ((b0 & tm) || (b0 & cm) != cm) continue;
+            for (int r = 0; r < kd; r++) {
+                uint32_t g = b0;
+                for (int i = 0; i < k; i++)
+                    if ((r >> (k - 1 - i)) & 1) g |= 1u << tb[i];
+                id[r] = g;
+                x[r] = v[g];
+            }
+            for (int r = 0; r < kd; r++) {
+                cd acc = 0;
+                for (int c = 0; c < kd; c++) acc += o.m[(size_t)r * kd + c] * x[c];
+                y[r] = acc;
+            }
+            for (int r = 0; r < kd; r++) v[id[r]] = y[r];
+        }
+        break;
+    }
+    case L_DIAG: {
+        const int k = (int)o.tg.size();
+        for (uint32_t i = 0; i < dim; i++) {
+            if ((i & cm) != cm) continue;
+            uint32_t row = 0;
+            for (int t = 0; t < k; t++) row |= ((i >> bit(o.tg[t])) & 1) << (k - 1 - t);
+            v[i] *= o.m[row];
+        }
+        break;
+    }
+    case L_PERMX: {
+        const int t = bit(o.tg[0]);
+        for (uint32_t i = 0; i < dim; i++)
+            if (!((i >> t) & 1) && (i & cm) == cm) std::swap(v[i], v[i | (1u << t)]);
+        break;
+    }
+    case L_SWAP: {
+        const int a = bit(o.tg[0]), b = bit(o.tg[1]);
+        for (uint32_t i = 0; i < dim; i++)
+            if (((i >> a) & 1) && !((i >> b) & 1) && (i & cm) == cm) std::swap(v[i], v[i ^ (1u << a) ^ (1u << b)]);
+        break;
+    }
+    }
+}
+
+// Grow block f to also cover qubits `add` (appended as higher row bits).
+static void grow_block(FBlock& f, const std::vector<int>& add) {
+    if (add.empty()) return;
+    const uint32_t d0 = 1u << f.q.size();
+    for (int q : add) f.q.push_back(q);
+    const uint32_t d1 = 1u << f.q.size();
+    std::vector<cd> u((size_t)d1 * d1, cd(0, 0));
+    for (uint32_t r = 0; r < d1; r++)
+        for (uint32_t c = 0; c < d1; c++)
+            if ((r / d0) == (c / d0)) u[(size_t)r * d1 + c] = f.u[(size_t)(r % d0) * d0 + (c % d0)];
+    f.u.swap(u);
+    for (int q : add) f.touch |= 1ull << q;
+}
+
+static void absorb(FBlock& f, const LOp& o) {
+    const uint32_t d = 1u << f.q.size();
+    std::vector<cd> col(d);
+    for (uint32_t c = 0; c < d; c++) {
+        for (uint32_t r = 0; r < d; r++) col[r] = f.u[(size_t)r * d + c];
+        apply_small(o, f.q, col.data());
+        for (uint32_t r = 0; r < d; r++) f.u[(size_t)r * d + c] = col[r];
+    }
+}
+
+static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector<FBlock>& blocks) {
+    std::vector<int> last(n, -1);
+    const int window = 96;       // candidate blocks searched per op
+    for (size_t i = 0; i < lops.size(); i++) {
+        const LOp& o = lops[i];
+        std::vector<int> S;
+        for (int q = 0; q < n; q++)
+            if (o.touch >> q & 1) S.push_back(q);
+        int b = -1;
+        for (int q : S) b = std::max(b, last[q]);
+        if (o.big || (int)S.size() > bq) {
+            FBlock f;
+            f.lop = (int)i;
+            f.q = S;
+            f.touch = o.touch;
+            blocks.push_back(f);
+            for (int q : S) last[q] = (int)blocks.size() - 1;
+            continue;
+        }
+        int best = -1, bestov = -1;
+        const int from = std::max(std::max(b, 0), (int)blocks.size() - window);
+        for (int c = from; c < (int)blocks.size(); c++) {
+            const FBlock& f = blocks[c];
+            if (f.lop >= 0) continue;
+            const int uni = popc(f.touch | o.touch);
+            if (uni > bq) continue;
+            const int ov = popc(f.touch & o.touch);
+            if (ov >= bestov) { best = c; bestov = ov; }
+        }
+        if (best < 0) {
+            FBlock f;
+            f.u.assign(1, cd(1, 0));
+            blocks.push_back(f);
+            best = (int)blocks.size() - 1;
+        }
+        FBlock& f = blocks[best];
+        std::vector<int> add;
+        for (int q : S)
+            if (!(f.touch >> q & 1)) add.push_back(q);
+        grow_block(f, add);
+        absorb(f, o);
+        for (int q : S) last[q] = best;
+    }
+}
+
+void block_fragments(const cd* u, int bq, double* out) {
+    const int R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
+    auto ue = [&](int i, int k) -> double {     // real embedding [[Ur, -Ui], [Ui, Ur]]
+        const int r = i % R, c = k % R;
+        const cd v = u[(size_t)r * R + c];
+        if (i < R) return k < R ? v.real() : -v.imag();
+        return k < R ? v.imag() : v.real();
+    };
+    for (int mb = 0; mb < MB; mb++)
+        for (int kk = 0; kk < KS; kk++)
+            for (int l = 0; l < 64; l++) out[((size_t)mb * KS + kk) * 64 + l] = ue(16 * mb + l % 16, 4 * kk + l / 16);
+}
+
+// Pass swizzle: tile bit p >= 4 XORs v[p] into the low four address bits.
+// Chosen so that, for every block of the pass, the lowest four column bits
+// land in four independent low-bit directions (conflict-free 16-lane LDS
+// accesses).
+static uint64_t choose_swizzle(int K, const std::vector<std::vector<int>>& colbits) {
+    uint32_t v[16] = {0};
+    auto good = [&]() {
+        for (const auto& cb : colbits) {
+            std::vector<uint32_t> a;
+            for (size_t i = 0; i < cb.size() && i < 4; i++) {
+                const int p = cb[i];
+                a.push_back(p < 4 ? (1u << p) : v[p]);
+            }
+            if (gf2_rank(a) != (int)a.size()) return false;
+        }
+        return true;
+    };
+    std::vector<int> free_p;
+    for (const auto& cb : colbits)
+        for (size_t i = 0; i < cb.size() && i < 4; i++)
+            if (cb[i] >= 4 && std::find(free_p.begin(), free_p.end(), cb[i]) == free_p.end()) free_p.push_back(cb[i]);
+    uint32_t rng = 0x2545F491u;
+    bool ok = good();
+    for (int attempt = 0; attempt < 4096 && !ok; attempt++) {
+        for (int p : free_p) {
+            rng ^= rng << 13;
+            rng ^= rng >> 17;
+            rng ^= rng << 5;
+            v[p] = rng & 15;
+        }
+        ok = good();
+    }
+    if (!ok)
+        for (int p : free_p) v[p] = (uint32_t)(1u << (p & 3));   // fold by nibble: conflicts, never wrong
+    uint64_t w = 0;
+    for (int p = 4; p < K && p < 20; p++) w |= (uint64_t)(v[p] & 15) << (4 * (p - 4));
+    return w;
+}
+
+static uint32_t swz_addr(uint64_t w, uint32_t x) {
+    uint32_t g = 0;
+    for (int p = 4; p < 20; p++)
+        if ((x >> p) & 1) g ^= (uint32_t)((w >> (4 * (p - 4))) & 15);
+    return x ^ g;
+}
+
+static void block_pass(int n, int K, int bq, uint64_t T, std::vector<FBlock>& blocks, const std::vector<int>& idx,
+                       Program& prog) {
+    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
+    Step st;
+    st.blk = true;
+    st.k = K;
+    st.bq = bq;
+    int pos[64];
+    for (int q = 0; q < 64; q++) pos[q] = -1;
+    int j = 0;
+    for (int q = 0; q < n; q++)
+        if (T >> q & 1) { st.tq[j] = (int8_t)q; pos[q] = j; j++; }
+    // pad every block to bq qubits with tile qubits it does not use
+    std::vector<std::vector<int>> rowpos(idx.size()), colpos(idx.size());
+    for (size_t b = 0; b < idx.size(); b++) {
+        FBlock& f = blocks[idx[b]];
+        std::vector<int> add;
+        for (int p = 0; p < K && (int)(f.q.size() + add.size()) < bq; p++) {
+            const int q = st.tq[p];
+            if (!(f.touch >> q & 1)) add.push_back(q);
+        }
+        grow_block(f, add);
+        for (int q : f.q) rowpos[b].push_back(pos[q]);
+        for (int p = 0; p < K; p++)
+            if (std::find(rowpos[b].begin(), rowpos[b].end(), p) == rowpos[b].end()) colpos[b].push_back(p);
+    }
+    st.swz = choose_swizzle(K, colpos);
+    st.op_begin = (int)prog.brecs.size();
+    const int64_t nfrag = (int64_t)(2 * (1 << bq) / 16) * (2 * (1 << bq) / 4) * 64;
+    for (size_t b = 0; b < idx.size(); b++) {
+        const FBlock& f = blocks[idx[b]];
+        BRec r;
+        std::memset(&r, 0, sizeof r);
+        for (int i = 0; i < bq; i++) r.ar[i] = (uint16_t)swz_addr(st.swz, 1u << rowpos[b][i]);
+        for (int i = 0; i < K - bq; i++) r.ac[i] = (uint16_t)swz_addr(st.swz, 1u << colpos[b][i]);
+        r.frag = (int64_t)prog.brecs.size() * nfrag;
+        prog.bmat_off.push_back((int64_t)prog.bmats.size());
+        prog.bmats.insert(prog.bmats.end(), f.u.begin(), f.u.end());
+        prog.brecs.push_back(r);
+    }
+    st.op_count = (int)prog.brecs.size() - st.op_begin;
+    prog.steps.push_back(st);
+}
+
+void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& prog) {
+    const int K = btile;
+    if (n < K || prog.prune) {
+        schedule_rounds(n, prog.precision == KSV_F64 ? 11 : 12, false, lops, prog);
+        return;
+    }
+    fuse_1q(n, lops);
+    std::vector<FBlock> blocks;
+    fuse_blocks(n, bq, lops, blocks);
+    // pack blocks into passes (a standalone op is a step of its own)
+    std::vector<LOp> proxies(blocks.size());
+    for (size_t i = 0; i < blocks.size(); i++) {
+        proxies[i].touch = blocks[i].touch;
+        proxies[i].need = blocks[i].touch;
+        proxies[i].big = blocks[i].lop >= 0;
+    }
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    const int KL = std::min(n, prog.tile);
+    std::vector<int> rem(blocks.size()), chosen, rest;
+    for (size_t i = 0; i < blocks.size(); i++) rem[i] = (int)i;
+    while (!rem.empty()) {
+        const FBlock& first = blocks[rem[0]];
+        if (first.lop >= 0) {
+            const LOp& o = lops[first.lop];
+            if (o.big) emit_big(o, prog);
+            else {
+                uint64_t T = ((1ull << std::min(4, KL)) - 1) | o.need;
+                emit_lds_pass(n, KL, T, lops, {first.lop}, prog);
+            }
+            rem.erase(rem.begin());
+            continue;
+        }
+        uint64_t T = pick_pass(proxies, rem, K, 7, all, [](const LOp& o) { return !o.big; }, chosen, rest, false);
+        block_pass(n, K, bq, T, blocks, chosen, prog);
+        rem.swap(rest);
+    }
+}
+
 int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err) {
+                 int kernel, int tile, int rtile, bool fuse, int prune, std::string& err, int btile, int bq) {
     if (nops && !ops) {
         err = "null op tape";
         return KSV_ERR_TYPE;
@@ -718,7 +1005,11 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
     p.tile = tile > 0 ? tile : (prec == KSV_F64 ? 12 : 13);
     p.tape_ops = nops;
     const int rt = rtile > 0 ? rtile : (prec == KSV_F64 ? 11 : 12);
+    // auto: fused MFMA blocks in exact mode, ordered register rounds under ket's pruning
+    const int bt = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
+    if (kernel == 0) kernel = prune ? 2 : 3;
     if (kernel == 1) schedule(n, p.tile, fuse && !prune, lops, p);
+    else if (kernel == 3) schedule_blocks(n, bt, bq > 0 ? bq : 4, lops, p);
     else schedule_rounds(n, rt, fuse && !prune, lops, p);
     return KSV_OK;
 }

@@ -299,6 +299,123 @@ static int emu_lds(const Program& p, const Step& s, cd* psi, int n) {
     return 0;
 }
 
+static uint32_t emu_swz(uint64_t w, uint32_t x) {
+    uint32_t g = 0;
+    for (int p = 4; p < 20; p++)
+        if ((x >> p) & 1) g ^= (uint32_t)((w >> (4 * (p - 4))) & 15);
+    return x ^ g;
+}
+
+// Fused-block pass, lane by lane as k_bpass runs it; MFMA 16x16x4 f64
+// operand layout (measured on gfx950): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4.
+static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
+    const int K = s.k, bq = s.bq, R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
+    const int NC = 1 << (K - bq), NBW = NC / 64, NT = 256, PER = (1 << K) / NT;
+    const uint64_t N = 1ull << n;
+    CHECK(NBW >= 1, "tile too small for the block width");
+    std::vector<cd> lds((size_t)1 << K);
+    std::vector<int> rd((size_t)1 << K), wr((size_t)1 << K);
+    std::vector<std::vector<double>> fr(s.op_count);
+    const size_t nf = (size_t)MB * KS * 64;
+    for (int i = 0; i < s.op_count; i++) {
+        fr[i].resize(nf);
+        block_fragments(p.bmats.data() + p.bmat_off[s.op_begin + i], bq, fr[i].data());
+    }
+    for (uint64_t b = 0; b < (N >> K); b++) {
+        uint64_t base = b;
+        for (int i = 0; i < K; i++) base = ins0(base, s.tq[i]);
+        std::fill(wr.begin(), wr.end(), 0);
+        for (int t = 0; t < NT; t++)
+            for (int i = 0; i < PER; i++) {
+                const uint32_t x = (uint32_t)t + (uint32_t)i * NT;
+                uint64_t g = base;
+                for (int q = 0; q < K; q++)
+                    if ((x >> q) & 1) g |= 1ull << s.tq[q];
+                const uint32_t la = emu_swz(s.swz, x);
+                CHECK(g < N && la < (1u << K), "block pass load out of range");
+                wr[la]++;
+                lds[la] = psi[g];
+            }
+        for (uint32_t a = 0; a < (1u << K); a++) CHECK(wr[a] == 1, "block pass LDS slot not loaded exactly once");
+        for (int bi = 0; bi < s.op_count; bi++) {
+            const BRec& r = p.brecs[s.op_begin + bi];
+            const double* af = fr[bi].data();
+            std::fill(rd.begin(), rd.end(), 0);
+            std::fill(wr.begin(), wr.end(), 0);
+            std::vector<cd> out(lds);
+            for (int wv = 0; wv < 4; wv++)
+                for (int m = 0; m < NBW; m++) {
+                    const uint32_t nb = (uint32_t)wv + 4u * (uint32_t)m;
+                    // per-lane gathered B values and addresses
+                    std::vector<std::vector<cd>> x(64, std::vector<cd>(R / 4));
+                    std::vector<uint32_t> colv(64);
+                    for (int l = 0; l < 64; l++) {
+                        const uint32_t rl = l >> 4, cl = l & 15;
+                        uint32_t col = 0;
+                        for (int bb = 0; bb < 4; bb++)
+                            if ((cl >> bb) & 1) col ^= r.ac[bb];
+                        for (int bb = 0; bb < K - bq - 4; bb++)
+                            if ((nb >> bb) & 1) col ^= r.ac[4 + bb];
+                        colv[l] = col;
+                        for (int t = 0; t < R / 4; t++) {
+                            const uint32_t row = 4u * (uint32_t)t + rl;
+                            uint32_t ra = 0;
+                            for (int bb = 0; bb < bq; bb++)
+                                if ((row >> bb) & 1) ra ^= r.ar[bb];
+                            const uint32_t a = ra ^ col;
+                            CHECK(a < (1u << K), "block read address out of range");
+                            rd[a]++;
+                            x[l][t] = lds[a];
+                        }
+                    }
+                    // MFMA accumulate
+                    std::vector<std::vector<double>> D(MB, std::vector<double>(256, 0.0));   // D[mb][i*16+j]
+                    for (int kk = 0; kk < KS; kk++)
+                        for (int mb = 0; mb < MB; mb++)
+                            for (int i = 0; i < 16; i++)
+                                for (int j = 0; j < 16; j++) {
+                                    double acc = 0;
+                                    for (int k = 0; k < 4; k++) {
+                                        const double av = af[((size_t)mb * KS + kk) * 64 + (i + 16 * k)];
+                                        const int lb = j + 16 * k;
+                                        const double bv = kk < R / 4 ? x[lb][kk].real() : x[lb][kk - R / 4].imag();
+                                        acc += av * bv;
+                                    }
+                                    D[mb][i * 16 + j] += acc;
+                                }
+                    for (int l = 0; l < 64; l++) {
+                        const uint32_t rl = l >> 4;
+                        for (int h = 0; h < R / 16; h++)
+                            for (int v = 0; v < 4; v++) {
+                                const uint32_t row = 16u * (uint32_t)h + 4u * (uint32_t)v + rl;
+                                uint32_t wa = 0;
+                                for (int bb = 0; bb < bq; bb++)
+                                    if ((row >> bb) & 1) wa ^= r.ar[bb];
+                                const uint32_t a = wa ^ colv[l];
+                                CHECK(a < (1u << K), "block write address out of range");
+                                wr[a]++;
+                                const int i = 4 * v + (int)rl, j = l & 15;
+                                out[a] = cd(D[h][i * 16 + j], D[h + R / 16][i * 16 + j]);
+                            }
+                    }
+                }
+            for (uint32_t a = 0; a < (1u << K); a++)
+                CHECK(rd[a] == 1 && wr[a] == 1, "block LDS slot not read and written exactly once");
+            lds.swap(out);
+        }
+        for (int t = 0; t < NT; t++)
+            for (int i = 0; i < PER; i++) {
+                const uint32_t x = (uint32_t)t + (uint32_t)i * NT;
+                uint64_t g = base;
+                for (int q = 0; q < K; q++)
+                    if ((x >> q) & 1) g |= 1ull << s.tq[q];
+                psi[g] = lds[emu_swz(s.swz, x)];
+            }
+    }
+    return 0;
+}
+
 static int emu_big(const Program& p, const Step& s, cd* psi, int n) {
     const BigOp& b = p.big_ops[s.big_op];
     const int k = (int)b.tg.size(), dim = 1 << k;
@@ -338,22 +455,25 @@ int emu_last_error(char* buf, size_t len) {
     return (int)g_msg.size();
 }
 
-// stats[0] passes, [1] register-round passes, [2] LDS transposes, [3] device ops
+// stats[0] passes, [1] register-round passes, [2] LDS transposes, [3] device ops, [4] fused-block passes,
+// [5] fused blocks
 int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel, int tile,
-            int rtile, int fuse, double* psi, int64_t* stats) {
+            int rtile, int fuse, double* psi, int64_t* stats, int btile, int bq) {
     Program p;
     std::string err;
-    int rc = plan_program(p, n, KSV_F64, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, err);
+    int rc = plan_program(p, n, KSV_F64, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, err, btile,
+                          bq);
     if (rc) {
         g_msg = err;
         return rc;
     }
     cd* st = reinterpret_cast<cd*>(psi);
-    int64_t regs = 0, exch = 0;
+    int64_t regs = 0, exch = 0, blks = 0;
     for (const Step& s : p.steps) {
         int r;
         if (s.big) r = emu_big(p, s, st, n);
         else if (s.reg) { r = emu_reg(p, s, st, n); regs++; exch += s.exchanges; }
+        else if (s.blk) { r = emu_blk(p, s, st, n); blks++; }
         else r = emu_lds(p, s, st, n);
         if (r) return r;
     }
@@ -365,6 +485,8 @@ int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats
         for (const ROp& r : p.rops)
             if (r.kind != R_EXCH) dev++;
         stats[3] = dev;
+        stats[4] = blks;
+        stats[5] = (int64_t)p.brecs.size();
     }
     return 0;
 }

@@ -20,9 +20,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", rtile=0, prefetch=False):
+def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", rtile=0, bq=0, btile=0,
+              precision="f64"):
     c = build(n, calls)
-    st = State(n, "f64", prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, prefetch=prefetch)
+    st = State(n, precision, prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, bq=bq, btile=btile)
     if init:
         st.init_basis(init)
     st.apply(c.tape())
@@ -75,14 +76,35 @@ def test_multi_pass_tiles(n, tile):
 
 @pytest.mark.parametrize("n,rtile", [(11, 11), (12, 12), (13, 12), (13, 13), (14, 11), (16, 12), (17, 13), (18, 12)])
 @pytest.mark.parametrize("prune", [True, False])
-@pytest.mark.parametrize("prefetch", [False, True])
-def test_register_round_passes(n, rtile, prune, prefetch):
-    """Register-round passes: LDS transposes, slot/thread/global controls, every op kind,
-    one tile per workgroup and persistent prefetching workgroups."""
+def test_register_round_passes(n, rtile, prune):
+    """Register-round passes: LDS transposes, slot/thread/global controls, every op kind."""
     calls = random_calls(n, 160, seed=31 * n + rtile, with_unitary=True)
     ref = dense_ref(n, calls, prune=prune)
-    got = gpu_state(n, calls, prune=prune, kernel="reg", rtile=rtile, prefetch=prefetch)
+    got = gpu_state(n, calls, prune=prune, kernel="reg", rtile=rtile)
     assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("n,btile,bq", [(12, 12, 4), (14, 12, 4), (15, 13, 4), (15, 13, 5), (17, 12, 4)])
+def test_fused_block_passes(n, btile, bq):
+    """Dense fused blocks on MFMA (FP64): every op kind fused, wide unitaries as their own steps."""
+    calls = random_calls(n, 140, seed=7 * n + bq, with_unitary=True)
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+F32_TOL = 1e-6      # complex64 state, FP32 arithmetic: max |amp error| against the FP64 oracle
+
+
+@pytest.mark.parametrize("bq,btile", [(4, 13), (5, 13)])
+def test_fused_block_passes_f32(bq, btile):
+    n = 16
+    calls = workloads.random_layer_calls(n, 10, seed=5)
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq, precision="f32")
+    err = np.max(np.abs(got - ref))
+    print(f"f32 fused-block bq={bq}: max |d amp| = {err:.3e}")
+    assert err < F32_TOL
 
 
 @pytest.mark.parametrize("rtile", [11, 12, 13])

@@ -29,24 +29,24 @@ def emu():
         L.emu_run.restype = ctypes.c_int
         L.emu_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                               ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_void_p, ctypes.c_void_p]
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.emu_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
     return _lib
 
 
-KERNEL = {"auto": 0, "lds": 1, "reg": 2}
+KERNEL = {"auto": 0, "lds": 1, "reg": 2, "mfma": 3}
 
 
-def run_emu(n, calls, prune=True, kernel="auto", tile=0, rtile=0, fuse=False, init=0):
+def run_emu(n, calls, prune=True, kernel="auto", tile=0, rtile=0, fuse=False, init=0, btile=0, bq=0):
     ops, mats = build(n, calls).tape()
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     mats = np.ascontiguousarray(mats, dtype=np.float64)
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[init] = 1
-    stats = np.zeros(4, dtype=np.int64)
+    stats = np.zeros(6, dtype=np.int64)
     rc = emu().emu_run(n, 1 if prune else 0, ops.ctypes.data, len(ops), mats.ctypes.data, len(mats), KERNEL[kernel],
-                       tile, rtile, 1 if fuse else 0, psi.ctypes.data, stats.ctypes.data)
+                       tile, rtile, 1 if fuse else 0, psi.ctypes.data, stats.ctypes.data, btile, bq)
     if rc:
         buf = ctypes.create_string_buffer(512)
         emu().emu_last_error(buf, 512)
@@ -86,10 +86,10 @@ def test_register_round_qft_with_initial_state():
     O.apply_tape(n, ref, ops, mats, prune=False)
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[x] = 1
-    stats = np.zeros(4, dtype=np.int64)
+    stats = np.zeros(6, dtype=np.int64)
     ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
     rc = emu().emu_run(n, 0, ops.ctypes.data, len(ops), mats.ctypes.data, len(mats), 2, 0, 12, 0, psi.ctypes.data,
-                       stats.ctypes.data)
+                       stats.ctypes.data, 0, 0)
     assert rc == 0
     assert np.max(np.abs(psi - ref)) < TOL
     k = np.arange(1 << n)
@@ -136,3 +136,31 @@ def test_thirty_qubit_schedule_shape():
     info = plan(30, c.tape(), kernel="reg", fuse=True, prune=False)
     assert info["reg_passes"] == info["passes"]
     assert info["passes"] <= 70
+
+
+@pytest.mark.parametrize("n,btile,bq", [(12, 12, 4), (13, 12, 4), (14, 13, 4), (14, 13, 5), (15, 12, 4)])
+def test_fused_block_passes_every_op(n, btile, bq):
+    calls = random_calls(n, 140, seed=7 * n + bq, with_unitary=True)
+    ref = dense_ref(n, calls, prune=False)
+    got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    assert stats[4] > 0 and stats[5] > 0
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("bq,btile", [(4, 12), (5, 13)])
+def test_fused_block_random_layers(bq, btile):
+    n = 15
+    calls = workloads.random_layer_calls(n, 6, seed=23)
+    ref = dense_ref(n, calls, prune=False)
+    got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    assert stats[1] == 0 and stats[4] > 0
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+def test_fused_blocks_fall_back_under_pruning():
+    n = 13
+    calls = random_calls(n, 80, seed=99)
+    ref = dense_ref(n, calls, prune=True)
+    got, stats = run_emu(n, calls, prune=True, kernel="mfma")
+    assert stats[4] == 0
+    assert np.max(np.abs(got - ref)) < TOL
