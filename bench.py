@@ -52,6 +52,172 @@ def traffic_from_profiles(kernel_tag):
         return None
 
 
+def run_program_steps(st, prog, steps, warmup):
+    """Reset to |0...0> and run the program `warmup` + `steps` times; time the last `steps`."""
+    for _ in range(warmup):
+        st.init_basis(0)
+        prog.run()
+    st.set_option(2, 1)     # KSV_OPT_TIMING: HIP events around every pass
+    st.reset_stats()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        st.init_basis(0)
+        prog.run()
+        log(f"bench: step {s + 1}/{steps} done")
+    return time.perf_counter() - t0, st.stats()
+
+
+def compare_states(a, b, n, chunk_log=26):
+    """max |a - b| over amplitudes and total-variation distance of |a|^2, |b|^2 (chunked read-back)."""
+    import numpy as np
+    N, step = 1 << n, 1 << min(chunk_log, n)
+    maxd, tvd = 0.0, 0.0
+    for off in range(0, N, step):
+        x = a.amplitudes(off, step)
+        y = b.amplitudes(off, step)
+        maxd = max(maxd, float(np.max(np.abs(x - y))))
+        tvd += float(np.sum(np.abs(np.abs(x) ** 2 - np.abs(y) ** 2)))
+    return maxd, 0.5 * tvd
+
+
+def workload_f32(args):
+    """Complex64 state at 32 qubits (32 GiB), the 40-layer random circuit fused
+    into 5-qubit blocks on FP32 MFMA; accuracy against FP64 at 30 qubits."""
+    from ket_amd import workloads
+    from ket_amd._native import State
+    n = args.qubits if args.qubits != 30 else 32
+    circ = workloads.random_layers(n, args.layers, seed=42)
+    ops, mats = circ.tape()
+    st = State(n, "f32", prune=False, kernel="mfma", bq=args.bq or 5, btile=args.btile or 13)
+    prog = st.program((ops, mats))
+    info = prog.info()
+    log(f"bench f32: {n} qubits, {len(ops)} gates -> {info['blocks']} blocks in {info['passes']} passes")
+    dt, stats = run_program_steps(st, prog, args.steps, args.warmup)
+    norm = st.norm2()
+    prog.free()
+    st.free()
+    launches = stats["pass_launches"]
+    avg_ms = stats["pass_ms"] / launches
+    achieved = stats["pass_bytes"] / launches / (avg_ms * 1e-3) / 1e9
+    # accuracy at 30 qubits: the same seeded circuit in FP64 and in FP32
+    na = 30
+    ca = workloads.random_layers(na, args.layers, seed=42).tape()
+    s64 = State(na, "f64", prune=False)
+    s64.apply(ca)
+    s32 = State(na, "f32", prune=False, kernel="mfma", bq=args.bq or 5, btile=args.btile or 13)
+    s32.apply(ca)
+    maxd, tvd = compare_states(s64, s32, na)
+    s64.free()
+    s32.free()
+    gates = len(ops)
+    return {"metric": "gate-updates/s & % HBM peak, complex64 statevector with FP32 MFMA blocks",
+            "value": round(gates * args.steps / dt, 2), "unit": "gate-updates/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"random Rx/Ry/Rz + CNOT-brick circuit, {n} qubits, {args.layers} layers, "
+                                   f"complex64, fused {args.bq or 5}-qubit blocks",
+                       "qubits": n, "gates": gates, "blocks": info["blocks"], "passes": info["passes"]},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "ksv::k_bpass",
+                         "avg_launch_ms": round(avg_ms, 4)},
+            "norm2_fp64_accumulated": norm,
+            "accuracy_vs_fp64_30q": {"max_abs_amp_err": maxd, "tvd": tvd}}
+
+
+def workload_qft(args):
+    """qft(n) on the seeded basis state x = floor(2^n r): closed form
+    amplitude_k = exp(2 pi i x k / 2^n) / sqrt(2^n), every p = 2^-n."""
+    import math
+    import numpy as np
+    from ket_amd import workloads
+    from ket_amd._native import State
+    n = args.qubits
+    c, bs, x = workloads.qft_input(n, seed=7)
+    ops, mats = c.tape()
+    st = State(n, "f64", prune=False)
+    prog = st.program((ops, mats))
+    info = prog.info()
+    for _ in range(args.warmup):
+        st.init_basis(x)
+        prog.run()
+    st.set_option(2, 1)
+    st.reset_stats()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        st.init_basis(x)
+        prog.run()
+    dt = time.perf_counter() - t0
+    stats = st.stats()
+    rng = np.random.RandomState(1)
+    ks = rng.randint(0, 1 << n, size=256, dtype=np.int64)
+    err = 0.0
+    for k in ks.tolist():
+        want = complex(math.cos(2 * math.pi * ((x * k) % (1 << n)) / (1 << n)),
+                       math.sin(2 * math.pi * ((x * k) % (1 << n)) / (1 << n))) / math.sqrt(1 << n)
+        err = max(err, abs(st.amplitude(k) - want))
+    idx, p = st.topk(1024)
+    prel = float(np.max(np.abs(p * (1 << n) - 1)))
+    norm = st.norm2()
+    prog.free()
+    st.free()
+    launches = stats["pass_launches"] + stats["other_launches"]
+    ms = (stats["pass_ms"] + stats["other_ms"]) / max(1, launches)
+    achieved = 2 * 16 * 2 ** n / (ms * 1e-3) / 1e9
+    return {"metric": "gate-updates/s, qft on a seeded basis state", "value": round(len(ops) * args.steps / dt, 2),
+            "unit": "gate-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"qft({n}) on |x>, x = {x}", "qubits": n, "gates": len(ops),
+                       "passes": info["passes"], "blocks": info["blocks"]},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(ms, 4)},
+            "checks": {"max_abs_amp_err_256_samples": err, "top1024_max_rel_p_err": prel, "norm2": norm}}
+
+
+def adjoint_section(n=24, reps=24, psr_sample=24, fd_sample=8):
+    """efficientSU2(n, reps) (2 n (reps + 1) parameters, seeded angles) with the
+    transverse-field Ising H = -sum Z_i Z_i+1 - 0.7 sum X_i (open chain):
+    adjoint gradient against the device parameter-shift rule (a sample of
+    parameters, per-parameter cost extrapolated to all) and central finite
+    differences (h = 1e-6)."""
+    import numpy as np
+    from ket_amd import workloads
+    from ket_amd.algorithms import efficientSU2, energy_and_gradient, parameter_shift, realAmplitudes
+    ans = efficientSU2(n, reps)
+    npar = 2 * n * (reps + 1)
+    params = workloads.ansatz_params(npar, seed=1234)
+    H = workloads.tfim_hamiltonian(n)
+    energy_and_gradient(efficientSU2(n, 1), H, params[:4 * n])       # warm-up
+    t0 = time.perf_counter()
+    e, g = energy_and_gradient(ans, H, params)
+    t_adj = time.perf_counter() - t0
+    log(f"bench: adjoint {n} qubits, {npar} params: {t_adj:.3f} s, E = {e:.12f}")
+    idx = [int(i) for i in np.linspace(0, npar - 1, psr_sample)]
+    tm = {}
+    _, gp = parameter_shift(ans, H, params, idx, timings=tm)
+    per_param = tm["loop_s"] / len(idx)
+    err_psr = max(abs(g[i] - v) for i, v in zip(idx, gp))
+    log(f"bench: parameter shift {len(idx)} params: {tm['loop_s']:.3f} s; max |adj - psr| = {err_psr:.2e}")
+    fidx = idx[:: max(1, len(idx) // fd_sample)][:fd_sample]
+    _, gf = parameter_shift(ans, H, params, fidx, shift=1e-6, central_difference=True)
+    err_fd = max(abs(g[i] - v) for i, v in zip(fidx, gf))
+    ra = realAmplitudes(n, reps)
+    pr = workloads.ansatz_params(n * (reps + 1), seed=1234)
+    t0 = time.perf_counter()
+    energy_and_gradient(ra, H, pr)
+    t_ra = time.perf_counter() - t0
+    return {"workload": f"efficientSU2({n},{reps}), {npar} parameters, transverse-field Ising H with "
+                        f"{len(H)} Pauli terms, FP64",
+            "adjoint_s": round(t_adj, 4), "adjoint_grads_per_s": round(1 / t_adj, 3),
+            "param_shift_s_per_param": round(per_param, 5),
+            "param_shift_s_full_extrapolated": round(per_param * npar, 3),
+            "param_shift_grads_per_s": round(1 / (per_param * npar), 5),
+            "speedup_vs_param_shift": round(per_param * npar / t_adj, 1),
+            "param_shift_sample": len(idx), "max_abs_err_vs_param_shift": err_psr,
+            "fd_sample": len(fidx), "max_abs_err_vs_central_fd_h1e-6": err_fd,
+            "realAmplitudes_adjoint_s": round(t_ra, 4), "realAmplitudes_params": n * (reps + 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,11 +233,20 @@ def main():
     ap.add_argument("--rtile", type=int, default=0, help="register-round tile qubits (11..13)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 24-qubit adjoint vs parameter-shift section")
+    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft"],
+                    help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
+                         "with accuracy against FP64; qft: qft(n) on a seeded basis state")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design, not a run)", flush=True)
         sys.exit(2)
+
+    if args.workload != "random":
+        line = workload_f32(args) if args.workload == "f32" else workload_qft(args)
+        print(json.dumps(line), flush=True)
+        return
 
     import numpy as np
     from ket_amd import workloads
@@ -130,6 +305,9 @@ def main():
                        "unfused_bound_gates_per_s": round(HBM_PEAK_GBS * 1e9 / (2 * 16 * 2 ** n), 1),
                        "parallelism": "single GPU"},
             "roofline": roofline}
+    if not args.no_adjoint:
+        log("bench: adjoint section")
+        line["adjoint"] = adjoint_section()
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(n, ops, mats, args.cpu_seconds)

@@ -144,6 +144,14 @@ int ksv_program_create(ksv_state* s, const ksv_op* ops, size_t nops, const doubl
                        size_t nmats, ksv_program** out);
 int ksv_program_run(ksv_state* s, ksv_program* p);
 int ksv_program_info(const ksv_program* p, ksv_program_summary* out);
+/* Give tape op `op` a new matrix (same layout as in the tape) without
+ * rescheduling: the op's records (or the fused block holding it) are
+ * recomputed and uploaded.  The matrix must keep the op's structure
+ * (dense / diagonal / permutation); a program built with one-qubit fusion
+ * (KSV_OPT_FUSE) cannot be updated.  Used for re-runs with new angles
+ * (parameter shifts, variational loops); no counterpart in ket, which
+ * rebuilds the circuit (src/circuit.ts:1108-1111). */
+int ksv_program_set_matrix(ksv_program* p, size_t op, const double* mat, size_t nmat);
 int ksv_program_destroy(ksv_program* p);
 /* Schedule a tape without a device (dry run): the passes and ops
  * ksv_program_create would produce for a state of n qubits with the given

@@ -56,6 +56,7 @@ SIGNATURES = [
     ("ksv_program_create", c_int, [_P, _P, c_size_t, _P, c_size_t, POINTER(c_void_p)]),
     ("ksv_program_run", c_int, [_P, _P]),
     ("ksv_program_info", c_int, [_P, POINTER(KsvProgramSummary)]),
+    ("ksv_program_set_matrix", c_int, [_P, c_size_t, _P, c_size_t]),
     ("ksv_program_destroy", c_int, [_P]),
     ("ksv_plan", c_int, [c_int, c_int, _P, c_size_t, _P, c_size_t, c_int, c_int, c_int, c_int, c_int,
                           POINTER(KsvProgramSummary)]),
@@ -305,6 +306,11 @@ class Program:
     def run(self, state=None):
         st = state or self.state
         check(lib().ksv_program_run(st.handle, self._h))
+
+    def set_matrix(self, op, matrix):
+        """Give tape op `op` a new matrix (complex, row-major) without rescheduling."""
+        m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128).reshape(-1)).view(np.float64)
+        check(lib().ksv_program_set_matrix(self._h, int(op), ptr(m), len(m)))
 
     def info(self):
         s = KsvProgramSummary()

@@ -325,6 +325,45 @@ static int summarize(const Program& p, ksv_program_summary* out) {
     return KSV_OK;
 }
 
+int ksv_program_set_matrix(ksv_program* p, size_t op, const double* mat, size_t nmat) {
+    if (!p) return fail(KSV_ERR_HANDLE, "null program");
+    if (!mat) return fail(KSV_ERR_TYPE, "null matrix");
+    std::vector<int> rops, blocks;
+    std::vector<std::pair<size_t, size_t>> ranges;
+    std::string err;
+    int rc = update_program(*p, op, mat, nmat, rops, blocks, ranges, err);
+    if (rc) return fail(rc, err);
+    for (int r : rops) HIPCHK(hipMemcpy(p->d_rops + r, &p->rops[r], sizeof(ROp), hipMemcpyHostToDevice));
+    const bool f64 = p->precision == KSV_F64;
+    for (const auto& rg : ranges) {
+        if (f64) {
+            HIPCHK(hipMemcpy((char*)p->d_mats + 16 * rg.first, p->mats.data() + rg.first, 16 * rg.second,
+                             hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(2 * rg.second);
+            for (size_t i = 0; i < rg.second; i++) {
+                f[2 * i] = (float)p->mats[rg.first + i].real();
+                f[2 * i + 1] = (float)p->mats[rg.first + i].imag();
+            }
+            HIPCHK(hipMemcpy((char*)p->d_mats + 8 * rg.first, f.data(), 8 * rg.second, hipMemcpyHostToDevice));
+        }
+    }
+    for (int b : blocks) {
+        const int R = 1 << p->blk_bq[b];
+        const size_t nf = (size_t)(2 * R / 16) * (2 * R / 4) * 64;
+        std::vector<double> fr(nf);
+        block_fragments(p->bmats.data() + p->bmat_off[b], p->blk_bq[b], fr.data());
+        if (f64) {
+            HIPCHK(hipMemcpy((double*)p->d_frags + p->brecs[b].frag, fr.data(), 8 * nf, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> ff(nf);
+            for (size_t i = 0; i < nf; i++) ff[i] = (float)fr[i];
+            HIPCHK(hipMemcpy((float*)p->d_frags + p->brecs[b].frag, ff.data(), 4 * nf, hipMemcpyHostToDevice));
+        }
+    }
+    return KSV_OK;
+}
+
 int ksv_program_info(const ksv_program* p, ksv_program_summary* out) {
     if (!p || !out) return fail(KSV_ERR_HANDLE, "null program");
     return summarize(*p, out);

@@ -342,11 +342,17 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         hipFree(d_cr);
         hipFree(d_ci);
     }
-    // backward sweep
+    // backward sweep: every parametrised op writes its per-block partial
+    // sums into its own slice of `slots`; one read-back at the end
     std::vector<ksv_op> inv;
     std::vector<double> im;
     size_t end = nops;
-    std::vector<double> hp;
+    const int GMAX = 1024;
+    double* slots = nullptr;
+    const size_t nslots = std::max<size_t>(recs.size(), 1);
+    AHIP(hipMalloc(&slots, sizeof(double) * 3 * GMAX * nslots));
+    struct Pending { int op; int g; size_t slot; };
+    std::vector<Pending> pend;
     while (end > 0) {
         // maximal run of parameter-free ops ending at `end`
         size_t b = end;
@@ -354,9 +360,11 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         if (b < end) {
             invert_tape(ops, b, end, mats, inv, im);
             rc = adj_run_tape(s, inv.data(), inv.size(), im.data(), im.size(), 0, psi);
-            if (rc) return rc;
-            rc = adj_run_tape(s, inv.data(), inv.size(), im.data(), im.size(), 0, psi2);
-            if (rc) return rc;
+            if (!rc) rc = adj_run_tape(s, inv.data(), inv.size(), im.data(), im.size(), 0, psi2);
+            if (rc) {
+                hipFree(slots);
+                return rc;
+            }
             end = b;
             if (end == 0) break;
         }
@@ -376,7 +384,10 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
             a.k = 1; a.t[0] = (int8_t)o.q[1]; dim = 2; insb = {o.q[0], o.q[1]};
             a.ctl = 1ull << o.q[0];
         } else {
-            if (o.q[0] == o.q[1]) return adj_fail(KSV_ERR_TYPE, "two-qubit gate needs distinct qubits");
+            if (o.q[0] == o.q[1]) {
+                hipFree(slots);
+                return adj_fail(KSV_ERR_TYPE, "two-qubit gate needs distinct qubits");
+            }
             a.k = 2; a.t[0] = (int8_t)o.q[0]; a.t[1] = (int8_t)o.q[1]; dim = 4; insb = {o.q[0], o.q[1]};
         }
         std::sort(insb.begin(), insb.end());
@@ -389,20 +400,30 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         for (size_t r = 0; r < rv.size(); r++)
             for (int j = 0; j < 2 * dim * dim; j++) a.du[r][j] = mats[rv[r]->dmat + j];
         const uint64_t groups = N >> a.nins;
-        int g = (int)std::min<uint64_t>((groups + 255) / 256, 1024);
+        int g = (int)std::min<uint64_t>((groups + 255) / 256, GMAX);
         if (g < 1) g = 1;
-        if (prec == KSV_F64) hipLaunchKernelGGL((k_adj<double2>), dim3(g), dim3(256), 0, st, a, groups, part);
-        else hipLaunchKernelGGL((k_adj<float2>), dim3(g), dim3(256), 0, st, a, groups, part);
-        AHIP(hipGetLastError());
-        hp.resize((size_t)g * 3);
-        AHIP(hipMemcpyAsync(hp.data(), part, 8 * hp.size(), hipMemcpyDeviceToHost, st));
-        AHIP(hipStreamSynchronize(st));
+        double* out = slots + pend.size() * 3 * GMAX;
+        if (prec == KSV_F64) hipLaunchKernelGGL((k_adj<double2>), dim3(g), dim3(256), 0, st, a, groups, out);
+        else hipLaunchKernelGGL((k_adj<float2>), dim3(g), dim3(256), 0, st, a, groups, out);
+        if (hipGetLastError() != hipSuccess) {
+            hipFree(slots);
+            return adj_fail(KSV_ERR_DEVICE, "adjoint kernel launch failed");
+        }
+        pend.push_back({oi, g, pend.size()});
+        end = end - 1;
+    }
+    std::vector<double> hp(3 * GMAX * nslots);
+    AHIP(hipMemcpyAsync(hp.data(), slots, 8 * hp.size(), hipMemcpyDeviceToHost, st));
+    AHIP(hipStreamSynchronize(st));
+    hipFree(slots);
+    for (const Pending& pd : pend) {
+        const auto& rv = recs[pd.op];
+        const double* h = hp.data() + pd.slot * 3 * GMAX;
         for (size_t r = 0; r < rv.size(); r++) {
             double sum = 0;
-            for (int b2 = 0; b2 < g; b2++) sum += hp[(size_t)b2 * 3 + r];
+            for (int b2 = 0; b2 < pd.g; b2++) sum += h[(size_t)b2 * 3 + r];
             grad[rv[r]->param] += 2.0 * rv[r]->coeff * sum;
         }
-        end = end - 1;
     }
     return KSV_OK;
 }

@@ -199,11 +199,28 @@ struct Program {
     BRec* d_brecs = nullptr;
     void* d_frags = nullptr;
     std::vector<cd> mats;
+    // re-parameterisation (ksv_program_set_matrix): the tape, its lowered
+    // ops and where each lowered op landed
+    std::vector<ksv_op> tape;
+    std::vector<LOp> lops;
+    std::vector<std::vector<int>> lop_of_src;   // tape op -> lowered ops
+    std::vector<int> rop_of_lop;                // register-round record of a lowered op, or -1
+    std::vector<int> blk_of_lop;                // fused block (BRec index) of a lowered op, or -1
+    std::vector<int64_t> mat_of_lop;            // LDS-pass / wide-unitary steps: offset of the op's matrix in mats
+    std::vector<std::vector<int>> blk_lops;     // per BRec: its lowered ops in application order
+    std::vector<std::vector<int>> blk_q;        // per BRec: its qubits, row bit i = qubit q[i]
+    std::vector<int> blk_bq;                    // per BRec: block width
     // device copies
     DOp* d_ops = nullptr;
     void* d_mats = nullptr;
     int dev = 0;
 };
+
+// Host side of ksv_program_set_matrix: re-lower tape op `op` with its new
+// matrix and recompute what depends on it.  Fills the ROps, BRecs and
+// complex ranges of `mats` that must be uploaded again.
+int update_program(Program& p, size_t op, const double* mat, size_t nmat, std::vector<int>& rops,
+                   std::vector<int>& blocks, std::vector<std::pair<size_t, size_t>>& mat_ranges, std::string& err);
 
 // schedule.cpp
 int lower_tape(int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,

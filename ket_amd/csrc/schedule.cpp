@@ -372,7 +372,8 @@ static uint64_t pick_pass(const std::vector<LOp>& lops, std::vector<int>& rem, i
     return T;
 }
 
-static void emit_big(const LOp& o, Program& prog) {
+static void emit_big(const LOp& o, int id, Program& prog) {
+    if (id >= 0 && (size_t)id < prog.mat_of_lop.size()) prog.mat_of_lop[id] = (int64_t)prog.mats.size();
     Step s;
     s.big = true;
     s.big_op = (int)prog.big_ops.size();
@@ -396,7 +397,11 @@ static void emit_lds_pass(int n, int K, uint64_t T, const std::vector<LOp>& lops
     for (int q = 0; q < n; q++)
         if (T >> q & 1) { s.tq[j] = (int8_t)q; pos[q] = j; j++; }
     s.op_begin = (int)prog.dops.size();
-    for (int id : idx) emit_dop(lops[id], pos, prog);
+    for (int id : idx) {
+        if ((size_t)id < prog.mat_of_lop.size() && lops[id].cls != L_PERMX && lops[id].cls != L_SWAP)
+            prog.mat_of_lop[id] = (int64_t)prog.mats.size();
+        emit_dop(lops[id], pos, prog);
+    }
     s.op_count = (int)prog.dops.size() - s.op_begin;
     prog.steps.push_back(s);
 }
@@ -413,7 +418,7 @@ static void schedule_lds(int n, int tile, std::vector<LOp>& lops, Program& prog)
     while (!rem.empty()) {
         const LOp& first = lops[rem[0]];
         if (first.big) {
-            emit_big(first, prog);
+            emit_big(first, rem[0], prog);
             rem.erase(rem.begin());
             continue;
         }
@@ -662,7 +667,10 @@ static void round_pass(int n, int K, uint64_t T, const std::vector<LOp>& lops, c
             emit_exch(K, TB, make_layout(K, sets[r - 1]), L, prog);
             st.exchanges++;
         }
-        for (int id : takes[r]) emit_rop(lops[id], L, pos, prog);
+        for (int id : takes[r]) {
+            if ((size_t)id < prog.rop_of_lop.size()) prog.rop_of_lop[id] = (int)prog.rops.size();
+            emit_rop(lops[id], L, pos, prog);
+        }
     }
     st.op_count = (int)prog.rops.size() - st.op_begin;
     prog.steps.push_back(st);
@@ -686,7 +694,7 @@ void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Progra
     while (!rem.empty()) {
         const LOp& first = lops[rem[0]];
         if (first.big) {
-            emit_big(first, prog);
+            emit_big(first, rem[0], prog);
             rem.erase(rem.begin());
             continue;
         }
@@ -717,6 +725,7 @@ struct FBlock {
     std::vector<cd> u;        // 2^|q| x 2^|q|, row-major
     int lop = -1;             // >= 0: a standalone op too wide for a block
     uint64_t touch = 0;
+    std::vector<int> ops;     // lowered ops absorbed, in application order
 };
 
 // Apply a logical op to a small dense vector over qubits Q (bit i = Q[i]).
@@ -845,6 +854,7 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
             if (!(f.touch >> q & 1)) add.push_back(q);
         grow_block(f, add);
         absorb(f, o);
+        f.ops.push_back((int)i);
         for (int q : S) last[q] = best;
     }
 }
@@ -946,6 +956,11 @@ static void block_pass(int n, int K, int bq, uint64_t T, std::vector<FBlock>& bl
         r.frag = (int64_t)prog.brecs.size() * nfrag;
         prog.bmat_off.push_back((int64_t)prog.bmats.size());
         prog.bmats.insert(prog.bmats.end(), f.u.begin(), f.u.end());
+        for (int id : f.ops)
+            if ((size_t)id < prog.blk_of_lop.size()) prog.blk_of_lop[id] = (int)prog.brecs.size();
+        prog.blk_lops.push_back(f.ops);
+        prog.blk_q.push_back(f.q);
+        prog.blk_bq.push_back(bq);
         prog.brecs.push_back(r);
     }
     st.op_count = (int)prog.brecs.size() - st.op_begin;
@@ -958,7 +973,6 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
         schedule_rounds(n, prog.precision == KSV_F64 ? 11 : 12, false, lops, prog);
         return;
     }
-    fuse_1q(n, lops);
     std::vector<FBlock> blocks;
     fuse_blocks(n, bq, lops, blocks);
     // pack blocks into passes (a standalone op is a step of its own)
@@ -976,7 +990,7 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
         const FBlock& first = blocks[rem[0]];
         if (first.lop >= 0) {
             const LOp& o = lops[first.lop];
-            if (o.big) emit_big(o, prog);
+            if (o.big) emit_big(o, first.lop, prog);
             else {
                 uint64_t T = ((1ull << std::min(4, KL)) - 1) | o.need;
                 emit_lds_pass(n, KL, T, lops, {first.lop}, prog);
@@ -1004,6 +1018,10 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
     p.prune = prune;
     p.tile = tile > 0 ? tile : (prec == KSV_F64 ? 12 : 13);
     p.tape_ops = nops;
+    p.tape.assign(ops, ops + nops);
+    p.rop_of_lop.assign(lops.size(), -1);
+    p.blk_of_lop.assign(lops.size(), -1);
+    p.mat_of_lop.assign(lops.size(), -1);
     const int rt = rtile > 0 ? rtile : (prec == KSV_F64 ? 11 : 12);
     // auto: fused MFMA blocks in exact mode, ordered register rounds under ket's pruning
     const int bt = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
@@ -1011,6 +1029,96 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
     if (kernel == 1) schedule(n, p.tile, fuse && !prune, lops, p);
     else if (kernel == 3) schedule_blocks(n, bt, bq > 0 ? bq : 4, lops, p);
     else schedule_rounds(n, rt, fuse && !prune, lops, p);
+    // one-qubit fusion (register rounds with `fuse`) merges lowered ops: such
+    // programs keep no op map and cannot be re-parameterised
+    if (p.rop_of_lop.size() == lops.size()) {
+        p.lops = lops;
+        p.lop_of_src.assign(nops, {});
+        for (size_t i = 0; i < lops.size(); i++)
+            if (lops[i].src >= 0 && (size_t)lops[i].src < nops) p.lop_of_src[lops[i].src].push_back((int)i);
+    }
+    return KSV_OK;
+}
+
+// Payload of an already emitted register-round record after its op's matrix changed.
+static void refresh_rop(const LOp& o, ROp& r, Program& p, std::vector<std::pair<size_t, size_t>>& ranges) {
+    switch (r.kind) {
+    case R_M1: put_inline(r, o.m.data(), 4, p.precision); break;
+    case R_DIAG:
+        if (o.tg.size() <= 2) put_inline(r, o.m.data(), (int)o.m.size(), p.precision);
+        else {
+            std::copy(o.m.begin(), o.m.end(), p.mats.begin() + r.mat);
+            ranges.push_back({(size_t)r.mat, o.m.size()});
+        }
+        break;
+    case R_DENSE:
+        std::copy(o.m.begin(), o.m.end(), p.mats.begin() + r.mat);
+        ranges.push_back({(size_t)r.mat, o.m.size()});
+        break;
+    default: break;       // permutations carry no matrix
+    }
+}
+
+static bool same_shape(const LOp& a, const LOp& b) {
+    return a.cls == b.cls && a.tg == b.tg && a.ctl == b.ctl && a.m.size() == b.m.size() && a.big == b.big;
+}
+
+int update_program(Program& p, size_t op, const double* mat, size_t nmat, std::vector<int>& rops,
+                   std::vector<int>& blocks, std::vector<std::pair<size_t, size_t>>& ranges, std::string& err) {
+    rops.clear();
+    blocks.clear();
+    ranges.clear();
+    if (p.lops.empty() && !p.tape.empty()) {
+        err = "this program fused one-qubit gates and cannot be re-parameterised";
+        return KSV_ERR_TYPE;
+    }
+    if (op >= p.tape.size()) {
+        err = "op index out of range";
+        return KSV_ERR_RANGE;
+    }
+    ksv_op o = p.tape[op];
+    o.mat = 0;
+    std::vector<LOp> nl;
+    int rc = lower_tape(p.n, &o, 1, mat, nmat, nl, err);
+    if (rc) return rc;
+    const std::vector<int>& old = p.lop_of_src[op];
+    if (nl.size() != old.size()) {
+        err = "the new matrix changes the op's structure (diagonal / permutation / dense)";
+        return KSV_ERR_TYPE;
+    }
+    for (size_t i = 0; i < old.size(); i++)
+        if (!same_shape(nl[i], p.lops[old[i]])) {
+            err = "the new matrix changes the op's structure (diagonal / permutation / dense)";
+            return KSV_ERR_TYPE;
+        }
+    for (size_t i = 0; i < old.size(); i++) {
+        const int id = old[i];
+        p.lops[id].m = nl[i].m;
+        if (p.rop_of_lop[id] >= 0) {
+            refresh_rop(p.lops[id], p.rops[p.rop_of_lop[id]], p, ranges);
+            rops.push_back(p.rop_of_lop[id]);
+        } else if (p.blk_of_lop[id] >= 0) {
+            if (std::find(blocks.begin(), blocks.end(), p.blk_of_lop[id]) == blocks.end())
+                blocks.push_back(p.blk_of_lop[id]);
+        } else if (p.mat_of_lop[id] >= 0) {
+            std::copy(p.lops[id].m.begin(), p.lops[id].m.end(), p.mats.begin() + p.mat_of_lop[id]);
+            ranges.push_back({(size_t)p.mat_of_lop[id], p.lops[id].m.size()});
+        } else if (p.lops[id].cls == L_PERMX || p.lops[id].cls == L_SWAP) {
+            // permutations carry no matrix
+        } else {
+            err = "the op was scheduled into an LDS or wide-unitary step, which cannot be re-parameterised";
+            return KSV_ERR_TYPE;
+        }
+    }
+    for (int b : blocks) {
+        FBlock f;
+        f.q = p.blk_q[b];
+        const size_t d = (size_t)1 << f.q.size();
+        f.u.assign(d * d, cd(0, 0));
+        for (size_t i = 0; i < d; i++) f.u[i * d + i] = 1;
+        for (int id : p.blk_lops[b]) absorb(f, p.lops[id]);
+        std::copy(f.u.begin(), f.u.end(), p.bmats.begin() + p.bmat_off[b]);
+    }
     return KSV_OK;
 }
 

@@ -457,8 +457,11 @@ int emu_last_error(char* buf, size_t len) {
 
 // stats[0] passes, [1] register-round passes, [2] LDS transposes, [3] device ops, [4] fused-block passes,
 // [5] fused blocks
-int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel, int tile,
-            int rtile, int fuse, double* psi, int64_t* stats, int btile, int bq) {
+// emu_run_updated: after scheduling, tape op upd_ops[u] gets the matrix
+// upd_mats[u * stride ...] through update_program (ksv_program_set_matrix's host side).
+int emu_run_updated(int n, int prune, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel,
+                    int tile, int rtile, int fuse, double* psi, int64_t* stats, int btile, int bq,
+                    const int64_t* upd_ops, const double* upd_mats, size_t nupd, size_t stride, size_t upd_nmat) {
     Program p;
     std::string err;
     int rc = plan_program(p, n, KSV_F64, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, err, btile,
@@ -466,6 +469,15 @@ int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats
     if (rc) {
         g_msg = err;
         return rc;
+    }
+    for (size_t u = 0; u < nupd; u++) {
+        std::vector<int> r, b;
+        std::vector<std::pair<size_t, size_t>> rg;
+        rc = update_program(p, (size_t)upd_ops[u], upd_mats + u * stride, upd_nmat, r, b, rg, err);
+        if (rc) {
+            g_msg = err;
+            return rc;
+        }
     }
     cd* st = reinterpret_cast<cd*>(psi);
     int64_t regs = 0, exch = 0, blks = 0;
@@ -489,6 +501,12 @@ int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats
         stats[5] = (int64_t)p.brecs.size();
     }
     return 0;
+}
+
+int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int kernel, int tile,
+            int rtile, int fuse, double* psi, int64_t* stats, int btile, int bq) {
+    return emu_run_updated(n, prune, ops, nops, mats, nmats, kernel, tile, rtile, fuse, psi, stats, btile, bq, nullptr,
+                           nullptr, 0, 0, 0);
 }
 
 }  // extern "C"

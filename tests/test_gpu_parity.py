@@ -347,3 +347,57 @@ def test_program_reuse_and_stats():
     assert s["pass_launches"] == 2 * info["passes"] and s["pass_ms"] > 0
     prog.free()
     st.free()
+
+
+@pytest.mark.parametrize("kernel,kw", [("mfma", {"btile": 12, "bq": 4}), ("reg", {"rtile": 11}), ("lds", {"tile": 9})])
+def test_program_set_matrix(kernel, kw):
+    """A scheduled program re-run with new angles equals the rebuilt circuit."""
+    import random
+    n = 13
+    rnd = random.Random(3)
+    angles_a = [rnd.uniform(0.2, 3.0) for _ in range(3 * n)]
+    angles_b = [rnd.uniform(0.2, 3.0) for _ in range(3 * n)]
+
+    def calls(an):
+        out = []
+        for q in range(n):
+            out.append(("ry", (an[q], q)))
+            out.append(("rz", (an[n + q], q)))
+        for q in range(n - 1):
+            out.append(("cnot", (q, q + 1)))
+        for q in range(n - 1):
+            out.append(("xx", (an[2 * n + q], q, q + 1)))
+        return out
+
+    ta, tb = build(n, calls(angles_a)).tape(), build(n, calls(angles_b)).tape()
+    st = State(n, "f64", prune=False, kernel=kernel, **kw)
+    prog = st.program(ta)
+    for i in range(len(ta[0])):
+        o = int(ta[0][i]["mat"])
+        if o < 0:
+            continue
+        dim = {0: 2, 3: 4, 4: 2}[int(ta[0][i]["kind"])]
+        m = tb[1][o:o + 2 * dim * dim].view(np.complex128)
+        prog.set_matrix(i, m)
+    st.init_basis(0)
+    prog.run()
+    got = st.amplitudes()
+    prog.free()
+    st.free()
+    assert np.max(np.abs(got - dense_ref(n, calls(angles_b), prune=False))) < TOL
+
+
+def test_adjoint_vs_parameter_shift_vs_finite_differences():
+    """efficientSU2(8, 2), transverse-field Ising H: adjoint = device shift rule to 1e-10,
+    = central finite differences (h = 1e-6) to 1e-6."""
+    from ket_amd.algorithms import efficientSU2, energy_and_gradient, parameter_shift
+    n = 8
+    ans = efficientSU2(n, 2)
+    params = workloads.ansatz_params(2 * n * 3, seed=1234)
+    H = workloads.tfim_hamiltonian(n)
+    e, g = energy_and_gradient(ans, H, params)
+    _, gp = parameter_shift(ans, H, params)
+    assert max(abs(a - b) for a, b in zip(g, gp)) < 1e-10
+    idx = list(range(0, len(params), 5))
+    _, gf = parameter_shift(ans, H, params, idx, shift=1e-6, central_difference=True)
+    assert max(abs(g[i] - v) for i, v in zip(idx, gf)) < 1e-6

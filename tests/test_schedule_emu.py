@@ -164,3 +164,79 @@ def test_fused_blocks_fall_back_under_pruning():
     got, stats = run_emu(n, calls, prune=True, kernel="mfma")
     assert stats[4] == 0
     assert np.max(np.abs(got - ref)) < TOL
+
+
+def _param_pair(n, seed, layers=5):
+    """Two circuits of one structure with different angles (every gate dense or diagonal as before)."""
+    import math
+    import random
+    rnd = random.Random(seed)
+
+    def build_c(angles):
+        calls, k = [], 0
+        for _ in range(layers):
+            for q in range(n):
+                calls.append(("rx", (angles[k], q)))
+                calls.append(("rz", (angles[k + 1], q)))
+                k += 2
+            for q in range(0, n - 1, 2):
+                calls.append(("cnot", (q, q + 1)))
+            calls.append(("crz", (angles[k], 0, n - 1)))
+            calls.append(("xx", (angles[k + 1], 1, n - 2)))
+            k += 2
+        return calls
+
+    cnt = layers * (2 * n + 2)
+    a = [rnd.uniform(0.2, 3.0) for _ in range(cnt)]
+    b = [rnd.uniform(0.2, 3.0) for _ in range(cnt)]
+    return build_c(a), build_c(b)
+
+
+@pytest.mark.parametrize("kernel,n,kw", [("reg", 13, {"rtile": 11}), ("mfma", 13, {"btile": 12, "bq": 4}),
+                                          ("mfma", 14, {"btile": 13, "bq": 5}), ("lds", 10, {"tile": 8}),
+                                          ("auto", 6, {})])
+def test_program_update_matches_rebuilt_circuit(kernel, n, kw):
+    """ksv_program_set_matrix's host side: a scheduled program given new angles
+    equals the oracle on the circuit built with those angles."""
+    ca, cb = _param_pair(n, seed=n)
+    ta, tb = build(n, ca).tape(), build(n, cb).tape()
+    ops = np.ascontiguousarray(ta[0], dtype=OP_DTYPE)
+    upd, mats = [], []
+    width = 32
+    for i in range(len(ops)):
+        if ops[i]["mat"] < 0:
+            continue
+        dim = {0: 2, 3: 4, 4: 2}.get(int(ops[i]["kind"]))
+        if dim is None:
+            continue
+        o = int(tb[0][i]["mat"])
+        m = tb[1][o:o + 2 * dim * dim]
+        row = np.zeros(width)
+        row[:len(m)] = m
+        upd.append(i)
+        mats.append(row)
+    upd = np.ascontiguousarray(upd, dtype=np.int64)
+    mats = np.ascontiguousarray(np.asarray(mats), dtype=np.float64)
+    tm = np.ascontiguousarray(ta[1], dtype=np.float64)
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1
+    stats = np.zeros(6, dtype=np.int64)
+    L = emu()
+    L.emu_run_updated.restype = ctypes.c_int
+    # every row holds one op's matrix; width bounds the matrix each update may read
+    rc = L.emu_run_updated(ctypes.c_int(n), ctypes.c_int(0), ctypes.c_void_p(ops.ctypes.data), ctypes.c_size_t(len(ops)),
+                           ctypes.c_void_p(tm.ctypes.data), ctypes.c_size_t(len(tm)), ctypes.c_int(KERNEL[kernel]),
+                           ctypes.c_int(kw.get("tile", 0)), ctypes.c_int(kw.get("rtile", 0)), ctypes.c_int(0),
+                           ctypes.c_void_p(psi.ctypes.data), ctypes.c_void_p(stats.ctypes.data),
+                           ctypes.c_int(kw.get("btile", 0)), ctypes.c_int(kw.get("bq", 0)),
+                           ctypes.c_void_p(upd.ctypes.data), ctypes.c_void_p(mats.ctypes.data),
+                           ctypes.c_size_t(len(upd)), ctypes.c_size_t(width), ctypes.c_size_t(width))
+    if rc:
+        buf = ctypes.create_string_buffer(512)
+        L.emu_last_error(buf, 512)
+        raise AssertionError(buf.value.decode())
+    ref = dense_ref(n, cb, prune=False)
+    assert np.max(np.abs(psi - ref)) < TOL
+    # the un-updated program still reproduces circuit A
+    got_a, _ = run_emu(n, ca, prune=False, kernel=kernel, **kw)
+    assert np.max(np.abs(got_a - dense_ref(n, ca, prune=False))) < TOL
