@@ -218,6 +218,66 @@ def adjoint_section(n=24, reps=24, psr_sample=24, fd_sample=8):
             "realAmplitudes_adjoint_s": round(t_ra, 4), "realAmplitudes_params": n * (reps + 1)}
 
 
+def workload_qaoa(args):
+    """512 seeded Erdos-Renyi Max-Cut instances (n uniform in 14..22, edge
+    probability 0.3, p = 4, gamma and beta uniform in [0, pi)) in one batch:
+    <H_C> and adjoint gradients for every instance per evaluation (HIP graph
+    replay); complex64 variant against FP64; a parity sample against the
+    general engine."""
+    import numpy as np
+    from ket_amd import workloads
+    from ket_amd._native import QaoaBatch
+    from ket_amd.algorithms import AnsatzFn, energy_and_gradient, maxCutHamiltonian, qaoa
+    import ket_amd
+    ket_amd.configure(prune=False)
+    P = 4
+    inst = workloads.qaoa_instances(count=512, seed=2026, nmin=14, nmax=22, p=P)
+    G = np.array([g for _, _, g, _ in inst])
+    B = np.array([b for _, _, _, b in inst])
+    out = {}
+    for prec in ("f64", "f32"):
+        batch = QaoaBatch([(n, e) for n, e, _, _ in inst], p=P, precision=prec)
+        info = batch.info()
+        batch.run(G, B, grad=True)              # capture + warm-up
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e, dg, db = batch.run(G, B, grad=True)
+        dt = (time.perf_counter() - t0) / args.steps
+        batch.run(G, B, grad=False)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ef = batch.run(G, B, grad=False)
+        dtf = (time.perf_counter() - t0) / args.steps
+        out[prec] = (e, dg, db, dt, dtf, info)
+        batch.free()
+        log(f"bench qaoa {prec}: {dt * 1e3:.2f} ms per energy+gradient batch, {dtf * 1e3:.2f} ms energies only")
+    e, dg, db, dt, dtf, info = out["f64"]
+    e32, dg32, db32, dt32, dtf32, _ = out["f32"]
+    amp_bytes = 16 * info["amplitudes"]
+    # passes: forward 2P + init (write) + energy (read psi, write lambda); backward 2P passes of psi and lambda
+    fwd_bytes = amp_bytes * (1 + 2 * 2 * P + 2)
+    bwd_bytes = amp_bytes * (2 * P * 4)
+    errs = []
+    for i in (0, 1, 2):
+        n, edges, g, b = inst[i]
+        ans = AnsatzFn(2 * P, lambda ps, n=n, edges=edges: qaoa(n, edges, list(ps[:P]), list(ps[P:])))
+        re_, rg = energy_and_gradient(ans, maxCutHamiltonian(n, edges), list(g) + list(b))
+        errs.append(max(abs(e[i] - re_), float(np.max(np.abs(np.concatenate([dg[i], db[i]]) - np.array(rg))))))
+    return {"metric": "QAOA instances/s (energy + adjoint gradient), batched",
+            "value": round(len(inst) / dt, 2), "unit": "instances/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{len(inst)} Max-Cut QAOA instances, n uniform 14..22, ER p=0.3, depth {P}",
+                       "amplitudes_total": info["amplitudes"], "work_items_per_launch": info["work_items"],
+                       "state_bytes": amp_bytes},
+            "energies_only_ms": round(dtf * 1e3, 3),
+            "hbm_gbs_energy_and_gradient": round((fwd_bytes + bwd_bytes) / dt / 1e9, 1),
+            "hbm_gbs_energies_only": round(fwd_bytes / dtf / 1e9, 1),
+            "parity_vs_general_engine_3_instances": max(errs),
+            "f32": {"ms_per_step": round(dt32 * 1e3, 3), "max_abs_energy_err_vs_f64": float(np.max(np.abs(e32 - e))),
+                    "max_abs_grad_err_vs_f64": float(max(np.max(np.abs(dg32 - dg)), np.max(np.abs(db32 - db))))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,9 +294,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 24-qubit adjoint vs parameter-shift section")
-    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft"],
+    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
-                         "with accuracy against FP64; qft: qft(n) on a seeded basis state")
+                         "with accuracy against FP64; qft: qft(n) on a seeded basis state; qaoa: 512 batched "
+                         "Max-Cut QAOA instances with adjoint gradients")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -244,7 +305,7 @@ def main():
         sys.exit(2)
 
     if args.workload != "random":
-        line = workload_f32(args) if args.workload == "f32" else workload_qft(args)
+        line = {"f32": workload_f32, "qft": workload_qft, "qaoa": workload_qaoa}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
 

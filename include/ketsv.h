@@ -218,6 +218,31 @@ int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats
  * by 1/sqrt(p_outcome) in FP64. */
 int ksv_collapse(ksv_state* s, int q, double r, int* outcome);
 
+/* ---- batched QAOA (Max-Cut) ----------------------------------------------------------- *
+ * Many instances of ket's qaoa(n, edges, gamma, beta) with
+ * maxCutHamiltonian(n, edges) (src/algorithms.ts:231-276) run together:
+ * every launch covers all instances (uneven widths, 12..22 qubits), the
+ * cost layer (all edges' cnot . rz(-gamma) . cnot) is one diagonal phase and
+ * the mixer (rx(2 beta) on every qubit) runs in registers.  The launch
+ * sequence is captured once in a HIP graph and replayed with new angles.
+ * Instance i has nq[i] qubits and the edges edges[2e], edges[2e+1] for
+ * e in [edge_off[i], edge_off[i+1]); gamma and beta are count x p, row-major.
+ * ksv_qaoa_batch_run writes E_i = <H_C> (expected cut) and, with want_grad,
+ * dE_i/dgamma_il and dE_i/dbeta_il by the adjoint method (no counterpart in
+ * ket, which runs instances one by one and differentiates by parameter
+ * shifts, src/algorithms.ts:536-547). */
+typedef struct ksv_batch ksv_batch;
+int ksv_qaoa_batch_create(int count, const int32_t* nq, const int64_t* edge_off, const int32_t* edges, int p,
+                          int precision, ksv_batch** out);
+int ksv_qaoa_batch_run(ksv_batch* b, const double* gamma, const double* beta, int want_grad, double* energy,
+                       double* dgamma, double* dbeta);
+/* The final state of one instance (after the last ksv_qaoa_batch_run's forward sweep;
+ * with want_grad the backward sweep returns it to H^n|0>). */
+int ksv_qaoa_batch_amplitudes(ksv_batch* b, int inst, uint64_t offset, uint64_t count, double* out);
+int ksv_qaoa_batch_info(const ksv_batch* b, uint64_t* total_amplitudes, uint64_t* work_items);
+int ksv_qaoa_batch_destroy(ksv_batch* b);
+int ksv_batch_last_error(char* buf, size_t len);
+
 /* ---- instrumentation -------------------------------------------------------------- */
 int ksv_get_stats(ksv_state* s, ksv_stats* out);
 int ksv_reset_stats(ksv_state* s);

@@ -74,6 +74,12 @@ SIGNATURES = [
     ("ksv_expect_pauli", c_int, [_P, _P, _P, _P, c_size_t, c_int, _P, _DP]),
     ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
     ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
+    ("ksv_qaoa_batch_create", c_int, [c_int, _P, _P, _P, c_int, c_int, POINTER(c_void_p)]),
+    ("ksv_qaoa_batch_run", c_int, [_P, _P, _P, c_int, _P, _P, _P]),
+    ("ksv_qaoa_batch_amplitudes", c_int, [_P, c_int, c_uint64, c_uint64, _P]),
+    ("ksv_qaoa_batch_info", c_int, [_P, _U64P, _U64P]),
+    ("ksv_qaoa_batch_destroy", c_int, [_P]),
+    ("ksv_batch_last_error", c_int, [ctypes.c_char_p, c_size_t]),
     ("ksv_get_stats", c_int, [_P, POINTER(KsvStats)]),
     ("ksv_reset_stats", c_int, [_P]),
 ]
@@ -320,6 +326,74 @@ class Program:
     def free(self):
         if getattr(self, "_h", None) is not None:
             lib().ksv_program_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _check_batch(rc):
+    if rc == KSV_OK:
+        return
+    buf = ctypes.create_string_buffer(1024)
+    lib().ksv_batch_last_error(buf, len(buf))
+    msg = buf.value.decode(errors="replace")
+    if rc == KSV_ERR_RANGE:
+        raise RangeError(msg)
+    if rc == KSV_ERR_TYPE:
+        raise TypeError(msg)
+    raise RuntimeError(f"libketsv: {msg}")
+
+
+class QaoaBatch:
+    """Many Max-Cut QAOA instances (ket's qaoa() + maxCutHamiltonian()) resident
+    on the device together; `run` returns <H_C> per instance and, optionally,
+    adjoint gradients with respect to every gamma and beta."""
+
+    def __init__(self, instances, p, precision="f64"):
+        """instances: list of (n, edges); p: layers."""
+        self.count = len(instances)
+        self.p = int(p)
+        nq = np.array([n for n, _ in instances], dtype=np.int32)
+        off = np.zeros(self.count + 1, dtype=np.int64)
+        flat = []
+        for i, (_, edges) in enumerate(instances):
+            for u, v in edges:
+                flat.extend((u, v))
+            off[i + 1] = off[i] + len(edges)
+        ed = np.array(flat if flat else [0, 0], dtype=np.int32)
+        h = c_void_p()
+        _check_batch(lib().ksv_qaoa_batch_create(self.count, ptr(nq), ptr(off), ptr(ed), self.p,
+                                                 KSV_F64 if precision == "f64" else KSV_F32, ctypes.byref(h)))
+        self._h = h
+        self.nq = nq
+
+    def run(self, gamma, beta, grad=False):
+        g = np.ascontiguousarray(gamma, dtype=np.float64).reshape(self.count, self.p)
+        b = np.ascontiguousarray(beta, dtype=np.float64).reshape(self.count, self.p)
+        e = np.empty(self.count)
+        dg = np.empty((self.count, self.p))
+        db = np.empty((self.count, self.p))
+        _check_batch(lib().ksv_qaoa_batch_run(self._h, ptr(g), ptr(b), 1 if grad else 0, ptr(e), ptr(dg), ptr(db)))
+        return (e, dg, db) if grad else e
+
+    def amplitudes(self, inst):
+        n = int(self.nq[inst])
+        out = np.empty(2 << n, dtype=np.float64)
+        _check_batch(lib().ksv_qaoa_batch_amplitudes(self._h, int(inst), 0, 1 << n, ptr(out)))
+        return out.view(np.complex128)
+
+    def info(self):
+        a, w = c_uint64(), c_uint64()
+        _check_batch(lib().ksv_qaoa_batch_info(self._h, ctypes.byref(a), ctypes.byref(w)))
+        return {"amplitudes": a.value, "work_items": w.value}
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ksv_qaoa_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
