@@ -43,13 +43,18 @@ def cpu_baseline(n, ops, mats, budget_s):
 
 
 def traffic_from_profiles(kernel_tag):
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    """Measured HBM bytes per launch of `kernel_tag` from the committed counter summary, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_k_bpass_random30.md")
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get(kernel_tag, {}).get("bytes_per_launch")
-    except (OSError, ValueError):
-        return None
+            for line in f:
+                m = re.match(r"^\| `([^`]+)` \| (\d+) \| (\d+) \|", line)
+                if m and m.group(1) == kernel_tag:
+                    return int(m.group(3))
+    except OSError:
+        pass
+    return None
 
 
 def run_program_steps(st, prog, steps, warmup):
@@ -289,11 +294,15 @@ def main():
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--kernel", default="auto", choices=["auto", "lds", "reg", "mfma"])
     ap.add_argument("--bq", type=int, default=0, help="fused-block width (4, 5)")
-    ap.add_argument("--btile", type=int, default=0, help="fused-block tile qubits (12, 13)")
+    ap.add_argument("--btile", type=int, default=0, help="fused-block tile qubits (11..13)")
+    ap.add_argument("--bthreads", type=int, default=0, help="fused-block workgroup size (256, 512)")
     ap.add_argument("--rtile", type=int, default=0, help="register-round tile qubits (11..13)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 24-qubit adjoint vs parameter-shift section")
+    ap.add_argument("--no-autotune", action="store_true", help="use the library's default schedule")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with one 16 GiB device-to-device state copy (HBM counter calibration)")
     ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
                          "with accuracy against FP64; qft: qft(n) on a seeded basis state; qaoa: 512 batched "
@@ -318,8 +327,17 @@ def main():
     circ = workloads.random_layers(n, layers, seed=42)
     ops, mats = circ.tape()
     gates = len(ops)
+    tuned = None
+    if not args.no_autotune and args.kernel == "auto" and not (args.rtile or args.bq or args.btile or args.bthreads):
+        from ket_amd.autotune import autotune
+        log("bench: autotuning the schedule")
+        best, cands = autotune(n, (ops, mats), precision="f64", prune=False, reps=2, verbose=log)
+        tuned = {"chosen": best, "candidates_ms": [[c, round(ms, 2)] for c, ms in cands]}
+        args.kernel, args.rtile = best["kernel"], best.get("rtile", 0)
+        args.bq, args.btile = best.get("bq", 0), best.get("btile", 0)
+        args.bthreads = best.get("bthreads", 0)
     st = State(n, "f64", prune=False, tile=args.tile, fuse=not args.no_fuse, kernel=args.kernel,
-               rtile=args.rtile, bq=args.bq, btile=args.btile)
+               rtile=args.rtile, bq=args.bq, btile=args.btile, bthreads=args.bthreads)
     prog = st.program((ops, mats))
     info = prog.info()
     log(f"bench: {gates} gates -> {info['passes']} passes ({info['reg_passes']} register-round with "
@@ -344,6 +362,11 @@ def main():
     if abs(norm - 1) > 1e-10:
         raise SystemExit(f"bench: norm check failed ({norm})")
     prog.free()
+    if args.copy_probe:
+        cp = State(n, "f64", prune=False)
+        cp.copy_from(st)
+        cp.free()
+        log("bench: copy probe done")
     st.free()
 
     ms_per_step = dt / args.steps * 1e3
@@ -353,7 +376,10 @@ def main():
     per_launch = stats["pass_bytes"] / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"k_pass_f64_{n}q"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(
+                    f"k_bpass<double,{args.bq or 4},{args.btile or 12},{args.bthreads or 256}>@{n}q")
+                if info["blk_passes"] == info["passes"] else None,
                 "kernel": ("ksv::k_bpass" if info["blk_passes"] else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass"),
                 "bytes_per_launch": per_launch,
                 "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps}
@@ -364,7 +390,9 @@ def main():
                        "qubits": n, "layers": layers, "gates": gates, "seed": 42, "passes": info["passes"],
                        "tile_qubits": info["tile_qubits"], "fused_1q": not args.no_fuse,
                        "unfused_bound_gates_per_s": round(HBM_PEAK_GBS * 1e9 / (2 * 16 * 2 ** n), 1),
-                       "parallelism": "single GPU"},
+                       "schedule": {"kernel": args.kernel, "bq": args.bq, "btile": args.btile, "rtile": args.rtile,
+                                    "bthreads": args.bthreads},
+                       "autotune": tuned, "parallelism": "single GPU"},
             "roofline": roofline}
     if not args.no_adjoint:
         log("bench: adjoint section")

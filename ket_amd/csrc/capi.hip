@@ -49,6 +49,7 @@ struct ksv_state {
     int rtile = 0;                  // KSV_OPT_ROUND_TILE
     int bq = 0;                     // KSV_OPT_BLOCK_QUBITS
     int btile = 0;                  // KSV_OPT_BLOCK_TILE
+    int bthreads = 256;             // KSV_OPT_BLOCK_THREADS
     ksv_stats stats;
     std::vector<hipEvent_t> ev;
 };
@@ -146,9 +147,15 @@ int ksv_set_option(ksv_state* s, int option, double value) {
         s->bq = b;
         return KSV_OK;
     }
+    case KSV_OPT_BLOCK_THREADS: {
+        int t = (int)value;
+        if (t != 0 && t != 256 && t != 512) return fail(KSV_ERR_RANGE, "fused-block workgroup size must be 0, 256 or 512");
+        s->bthreads = t ? t : 256;
+        return KSV_OK;
+    }
     case KSV_OPT_BLOCK_TILE: {
         int t = (int)value;
-        if (t != 0 && t != 12 && t != 13) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0, 12 or 13");
+        if (t != 0 && (t < 11 || t > 13)) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0 or 11..13");
         s->btile = t;
         return KSV_OK;
     }
@@ -188,6 +195,7 @@ static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops
     std::string err;
     int bqv = bq > 0 ? bq : 4, btv = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
     if (kernel == 3 && bqv == 5 && btv != 13) return fail(KSV_ERR_RANGE, "5-qubit blocks need 13-qubit tiles");
+    if (kernel == 3 && btv == 11 && bqv != 4) return fail(KSV_ERR_RANGE, "11-qubit tiles take 4-qubit blocks");
     int rc = plan_program(p, n, prec, ops, nops, mats, nmats, kernel, tile, rtile, fuse, prune, err, btile, bq);
     if (rc) return fail(rc, err);
     return KSV_OK;
@@ -200,6 +208,7 @@ static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const dou
                        prune, s->btile, s->bq);
     if (rc) return rc;
     p->dev = s->dev;
+    p->bthreads = s->bthreads;
     if (!p->rops.empty()) {
         HIPCHK(hipMalloc(&p->d_rops, sizeof(ROp) * p->rops.size()));
         HIPCHK(hipMemcpy(p->d_rops, p->rops.data(), sizeof(ROp) * p->rops.size(), hipMemcpyHostToDevice));
@@ -392,7 +401,7 @@ int ksv_plan_blocks(int n, int precision, const ksv_op* ops, size_t nops, const 
     if (precision != KSV_F64 && precision != KSV_F32) return fail(KSV_ERR_TYPE, "unknown precision");
     if (kernel < 0 || kernel > 3) return fail(KSV_ERR_RANGE, "pass kernel must be 0, 1, 2 or 3");
     if (bq != 0 && bq != 4 && bq != 5) return fail(KSV_ERR_RANGE, "fused-block width must be 0, 4 or 5");
-    if (btile != 0 && btile != 12 && btile != 13) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0, 12 or 13");
+    if (btile != 0 && (btile < 11 || btile > 13)) return fail(KSV_ERR_RANGE, "fused-block tile qubits must be 0 or 11..13");
     Program p;
     if (rtile != 0 && (rtile < 11 || rtile > 13)) return fail(KSV_ERR_RANGE, "register-round tile qubits must be 0 or 11..13");
     int rc = plan_into(p, n, precision, ops, nops, mats, nmats, kernel, tile, rtile, fuse != 0, prune, btile, bq);

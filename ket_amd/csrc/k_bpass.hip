@@ -60,17 +60,18 @@ __device__ __forceinline__ uint32_t swz_part(uint32_t x, uint64_t w, int lo, int
     return g;
 }
 
-template <typename T, int BQ, int K>
-__global__ __launch_bounds__(256) void k_bpass(BPassArgs a) {
+template <typename T, int BQ, int K, int NT>
+__global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
     using V = typename BVec2<T>::type;
     using M = Mfma<T>;
     using acc_t = typename M::acc;
-    constexpr int NT = 256;
+    constexpr int NW = NT / 64;         // waves
+    constexpr int LNT = NT == 512 ? 9 : 8;
     constexpr int R = 1 << BQ;          // complex rows
     constexpr int MB = 2 * R / 16;      // output 16-row tiles (real embedding)
     constexpr int KS = 2 * R / 4;       // k-steps
     constexpr int NC = 1 << (K - BQ);   // columns
-    constexpr int NBW = NC / 64;        // 16-column slices per wave
+    constexpr int NBW = NC / 16 / NW;   // 16-column slices per wave
     constexpr int PER = (1 << K) / NT;  // amplitudes per thread at load / store
     static_assert(NBW >= 1, "tile too small for the block width");
     __shared__ V lds[1 << K];
@@ -89,16 +90,16 @@ __global__ __launch_bounds__(256) void k_bpass(BPassArgs a) {
     // physical offset and of the swizzled LDS address
     uint64_t g_t = 0;
 #pragma unroll
-    for (int b = 0; b < 8; b++) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(a.tq_lo, b);
-    const uint32_t l_t = tid ^ swz_part(tid, a.swz, 4, 8);
+    for (int b = 0; b < LNT; b++) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(b < 8 ? a.tq_lo : a.tq_hi, b & 7);
+    const uint32_t l_t = tid ^ swz_part(tid, a.swz, 4, LNT);
 #pragma unroll
     for (int i = 0; i < PER; i++) {
         const uint32_t xh = (uint32_t)i * NT;
         uint64_t g_h = 0;
 #pragma unroll
-        for (int b = 8; b < K; b++)
+        for (int b = LNT; b < K; b++)
             if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
-        lds[l_t ^ xh ^ swz_part(xh, a.swz, 8, K)] = psi[base | g_t | g_h];
+        lds[l_t ^ xh ^ swz_part(xh, a.swz, LNT, K)] = psi[base | g_t | g_h];
     }
     __syncthreads();
 
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void k_bpass(BPassArgs a) {
                                ((cl & 8) ? ac[3] : 0u);
 #pragma unroll
         for (int m = 0; m < NBW; m++) {
-            const uint32_t nb = wave + 4 * m;
+            const uint32_t nb = wave + NW * m;
             uint32_t col = col_l;
 #pragma unroll
             for (int b = 0; b < K - BQ - 4; b++)
@@ -178,9 +179,9 @@ __global__ __launch_bounds__(256) void k_bpass(BPassArgs a) {
         const uint32_t xh = (uint32_t)i * NT;
         uint64_t g_h = 0;
 #pragma unroll
-        for (int b = 8; b < K; b++)
+        for (int b = LNT; b < K; b++)
             if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
-        psi[base | g_t | g_h] = lds[l_t ^ xh ^ swz_part(xh, a.swz, 8, K)];
+        psi[base | g_t | g_h] = lds[l_t ^ xh ^ swz_part(xh, a.swz, LNT, K)];
     }
 }
 
@@ -198,10 +199,13 @@ static hipError_t launch_bpass_t(const Program& p, const Step& s, void* psi, hip
         a.tq_hi |= (uint64_t)(uint8_t)s.tq[8 + i] << (8 * i);
     }
     a.swz = s.swz;
-    const dim3 grid((unsigned)(1ull << (p.n - s.k))), block(256);
-    if (s.bq == 4 && s.k == 12) hipLaunchKernelGGL((k_bpass<T, 4, 12>), grid, block, 0, st, a);
-    else if (s.bq == 4 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 4, 13>), grid, block, 0, st, a);
-    else if (s.bq == 5 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 5, 13>), grid, block, 0, st, a);
+    const dim3 grid((unsigned)(1ull << (p.n - s.k)));
+    const bool wide = p.bthreads == 512;
+    if (s.bq == 4 && s.k == 11) hipLaunchKernelGGL((k_bpass<T, 4, 11, 256>), grid, dim3(256), 0, st, a);
+    else if (s.bq == 4 && s.k == 12 && !wide) hipLaunchKernelGGL((k_bpass<T, 4, 12, 256>), grid, dim3(256), 0, st, a);
+    else if (s.bq == 4 && s.k == 12) hipLaunchKernelGGL((k_bpass<T, 4, 12, 512>), grid, dim3(512), 0, st, a);
+    else if (s.bq == 4 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 4, 13, 256>), grid, dim3(256), 0, st, a);
+    else if (s.bq == 5 && s.k == 13) hipLaunchKernelGGL((k_bpass<T, 5, 13, 256>), grid, dim3(256), 0, st, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -198,6 +198,7 @@ struct Program {
     std::vector<int64_t> bmat_off;   // per BRec: offset of its matrix in bmats
     BRec* d_brecs = nullptr;
     void* d_frags = nullptr;
+    int bthreads = 256;              // fused-block passes: workgroup size (256 or 512)
     std::vector<cd> mats;
     // re-parameterisation (ksv_program_set_matrix): the tape, its lowered
     // ops and where each lowered op landed

@@ -20,6 +20,7 @@
 using namespace ksv;
 
 static std::string g_msg;
+static int g_bthreads = 256;     // fused-block workgroup size emulated
 
 static uint64_t ins0(uint64_t p, int q) { return ((p >> q) << (q + 1)) | (p & ((1ull << q) - 1)); }
 
@@ -311,7 +312,7 @@ static uint32_t emu_swz(uint64_t w, uint32_t x) {
 // lane j + 16k, D[i][j] in lane j + 16(i % 4), element i / 4.
 static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
     const int K = s.k, bq = s.bq, R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
-    const int NC = 1 << (K - bq), NBW = NC / 64, NT = 256, PER = (1 << K) / NT;
+    const int NT = g_bthreads, NW = NT / 64, NC = 1 << (K - bq), NBW = NC / 16 / NW, PER = (1 << K) / NT;
     const uint64_t N = 1ull << n;
     CHECK(NBW >= 1, "tile too small for the block width");
     std::vector<cd> lds((size_t)1 << K);
@@ -344,9 +345,9 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
             std::fill(rd.begin(), rd.end(), 0);
             std::fill(wr.begin(), wr.end(), 0);
             std::vector<cd> out(lds);
-            for (int wv = 0; wv < 4; wv++)
+            for (int wv = 0; wv < NW; wv++)
                 for (int m = 0; m < NBW; m++) {
-                    const uint32_t nb = (uint32_t)wv + 4u * (uint32_t)m;
+                    const uint32_t nb = (uint32_t)wv + (uint32_t)NW * (uint32_t)m;
                     // per-lane gathered B values and addresses
                     std::vector<std::vector<cd>> x(64, std::vector<cd>(R / 4));
                     std::vector<uint32_t> colv(64);
@@ -445,6 +446,8 @@ static int emu_big(const Program& p, const Step& s, cd* psi, int n) {
 }
 
 extern "C" {
+
+void emu_set_block_threads(int nt) { g_bthreads = nt; }
 
 int emu_last_error(char* buf, size_t len) {
     if (buf && len) {

@@ -21,9 +21,10 @@ TOL = 1e-12
 
 
 def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", rtile=0, bq=0, btile=0,
-              precision="f64"):
+              precision="f64", bthreads=0):
     c = build(n, calls)
-    st = State(n, precision, prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, bq=bq, btile=btile)
+    st = State(n, precision, prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, bq=bq, btile=btile,
+               bthreads=bthreads)
     if init:
         st.init_basis(init)
     st.apply(c.tape())
@@ -84,13 +85,24 @@ def test_register_round_passes(n, rtile, prune):
     assert np.max(np.abs(got - ref)) < TOL
 
 
-@pytest.mark.parametrize("n,btile,bq", [(12, 12, 4), (14, 12, 4), (15, 13, 4), (15, 13, 5), (17, 12, 4)])
-def test_fused_block_passes(n, btile, bq):
+@pytest.mark.parametrize("n,btile,bq,nt", [(12, 12, 4, 256), (14, 12, 4, 256), (15, 13, 4, 256), (15, 13, 5, 256),
+                                           (17, 12, 4, 256), (11, 11, 4, 256), (15, 11, 4, 256), (12, 12, 4, 512),
+                                           (16, 12, 4, 512)])
+def test_fused_block_passes(n, btile, bq, nt):
     """Dense fused blocks on MFMA (FP64): every op kind fused, wide unitaries as their own steps."""
     calls = random_calls(n, 140, seed=7 * n + bq, with_unitary=True)
     ref = dense_ref(n, calls, prune=False)
-    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq, bthreads=nt)
     assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("btile,nt", [(11, 256), (12, 512)])
+def test_fused_block_variants_f32(btile, nt):
+    n = 16
+    calls = workloads.random_layer_calls(n, 10, seed=9)
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=4, bthreads=nt, precision="f32")
+    assert np.max(np.abs(got - ref)) < 1e-6
 
 
 F32_TOL = 1e-6      # complex64 state, FP32 arithmetic: max |amp error| against the FP64 oracle

@@ -31,6 +31,7 @@ def emu():
                               ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.emu_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.emu_set_block_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -138,11 +139,17 @@ def test_thirty_qubit_schedule_shape():
     assert info["passes"] <= 70
 
 
-@pytest.mark.parametrize("n,btile,bq", [(12, 12, 4), (13, 12, 4), (14, 13, 4), (14, 13, 5), (15, 12, 4)])
-def test_fused_block_passes_every_op(n, btile, bq):
+@pytest.mark.parametrize("n,btile,bq,nt", [(12, 12, 4, 256), (13, 12, 4, 256), (14, 13, 4, 256), (14, 13, 5, 256),
+                                           (15, 12, 4, 256), (11, 11, 4, 256), (13, 11, 4, 256), (12, 12, 4, 512),
+                                           (14, 12, 4, 512)])
+def test_fused_block_passes_every_op(n, btile, bq, nt):
     calls = random_calls(n, 140, seed=7 * n + bq, with_unitary=True)
     ref = dense_ref(n, calls, prune=False)
-    got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    emu().emu_set_block_threads(nt)
+    try:
+        got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    finally:
+        emu().emu_set_block_threads(256)
     assert stats[4] > 0 and stats[5] > 0
     assert np.max(np.abs(got - ref)) < TOL
 
