@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6     # MI355X dense FP64 matrix peak
 METRIC = "gate-updates/s & % HBM peak at 30q FP64; adjoint vs param-shift grads/s"
 
 
@@ -375,14 +376,26 @@ def main():
     avg_ms = stats["pass_ms"] / launches
     per_launch = stats["pass_bytes"] / launches
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(
-                    f"k_bpass<double,{args.bq or 4},{args.btile or 12},{args.bthreads or 256}>@{n}q")
-                if info["blk_passes"] == info["passes"] else None,
-                "kernel": ("ksv::k_bpass" if info["blk_passes"] else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass"),
-                "bytes_per_launch": per_launch,
-                "avg_launch_ms": round(avg_ms, 4), "launches_per_step": launches // args.steps}
+    traffic = (traffic_from_profiles(f"k_bpass<double,{args.bq or 4},{args.btile or 12},{args.bthreads or 256}>@{n}q")
+               if info["blk_passes"] == info["passes"] else None)
+    kname = "ksv::k_bpass" if info["blk_passes"] else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass"
+    hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    if info["blk_passes"] == info["passes"] and info["blocks"]:
+        # a dense 2^b-qubit block is a real (2^(b+1))^2 x 2^(n-b) GEMM: 8 * 2^b flops per amplitude
+        bq = args.bq or 4
+        flops_per_launch = info["blocks"] * 8 * (1 << bq) * (1 << n) / info["passes"]
+        tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+        mfma = {"achieved": round(tflops, 2), "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(tflops / FP64_MFMA_PEAK_TFS, 4), "flops_per_launch": flops_per_launch}
+        # the bound is the resource whose time at its peak is the larger
+        mfma_bound = flops_per_launch / (FP64_MFMA_PEAK_TFS * 1e12) > per_launch / (HBM_PEAK_GBS * 1e9)
+    else:
+        mfma, mfma_bound = None, False
+    main_part = dict(mfma, bound="mfma") if mfma_bound else dict(hbm, bound="hbm")
+    main_part.pop("flops_per_launch", None)
+    roofline = {"bound": main_part.pop("bound"), **main_part, "traffic": traffic, "kernel": kname,
+                "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
+                "launches_per_step": launches // args.steps, "hbm": hbm, "mfma": mfma}
     line = {"metric": METRIC, "value": round(value, 2), "unit": "gate-updates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
