@@ -224,6 +224,58 @@ def adjoint_section(n=24, reps=24, psr_sample=24, fd_sample=8):
             "realAmplitudes_adjoint_s": round(t_ra, 4), "realAmplitudes_params": n * (reps + 1)}
 
 
+def workload_noisy(args):
+    """run() with device noise: the 12-qubit, 20-layer random circuit (830 gates) plus
+    a measurement of every qubit into a register, 'harmony' error rates, 10 000
+    shots.  Batched trajectories (all shots as copies of one device state) against
+    the shot-by-shot driver on a sample of shots; both give identical counts."""
+    import ket_amd
+    from ket_amd import workloads
+    n, shots, seed = 12, args.shots or 10000, 42
+    c = workloads.random_layers(n, 20, seed=seed).creg("m", n)
+    for q in range(n):
+        c = c.measure(q, "m", q)
+    c.run(shots=64, seed=1, noise="harmony")                     # warm-up (library load, first launches)
+    t0 = time.perf_counter()
+    d = c.run(shots=shots, seed=seed, noise="harmony")
+    t_b = time.perf_counter() - t0
+    log(f"bench: batched {shots} shots in {t_b:.3f} s")
+    sample = 40
+    old = ket_amd.configure(batch_shots=False)
+    try:
+        t0 = time.perf_counter()
+        d1 = c.run(shots=sample, seed=seed, noise="harmony")
+        t_p = (time.perf_counter() - t0) / sample
+    finally:
+        ket_amd.configure(**old)
+    old = ket_amd.configure(batch_shots=True)
+    d2 = c.run(shots=sample, seed=seed, noise="harmony")
+    ket_amd.configure(**old)
+    same = list(d1.probs.items()) == list(d2.probs.items()) and d1.cregs == d2.cregs
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ketsv_oracle as O
+        from ket_amd.noise import resolve_noise
+        calls = workloads.random_layer_calls(n, 20, seed) + [("measure", (q, "m", q)) for q in range(n)]
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 15.0):
+            O.run_shots(n, calls, 1, seed + k, resolve_noise("harmony"), cregs={"m": n})
+            k += 1
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(k / dt, 3), "unit": "shots/s", "cores": 1, "kind": "port",
+               "sample": f"{k} shots of the same circuit through the oracle's per-shot restatement, {dt:.1f} s"}
+    return {"metric": "noisy run() shots/s, 12 qubits, 830 gates + 12 measurements, harmony noise",
+            "value": round(shots / t_b, 1), "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(t_b * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "random_layers(12, 20, seed 42) + measure all, noise 'harmony', "
+                                   f"{shots} shots", "qubits": n, "shots": shots},
+            "per_shot_driver_s_per_shot": round(t_p, 5), "speedup_vs_per_shot_driver": round(t_p * shots / t_b, 1),
+            "per_shot_sample": sample, "identical_counts_on_sample": same,
+            "most_likely": d.most, "entropy_bits": round(d.entropy, 4), "cpu_baseline": cpu}
+
+
 def workload_qaoa(args):
     """512 seeded Erdos-Renyi Max-Cut instances (n uniform in 14..22, edge
     probability 0.3, p = 4, gamma and beta uniform in [0, pi)) in one batch:
@@ -304,10 +356,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default schedule")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with one 16 GiB device-to-device state copy (HBM counter calibration)")
-    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa"],
+    ap.add_argument("--shots", type=int, default=0, help="noisy workload: shots (default 10000)")
+    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
                          "with accuracy against FP64; qft: qft(n) on a seeded basis state; qaoa: 512 batched "
-                         "Max-Cut QAOA instances with adjoint gradients")
+                         "Max-Cut QAOA instances with adjoint gradients; noisy: run() with device noise, "
+                         "batched trajectories")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -315,7 +369,8 @@ def main():
         sys.exit(2)
 
     if args.workload != "random":
-        line = {"f32": workload_f32, "qft": workload_qft, "qaoa": workload_qaoa}[args.workload](args)
+        line = {"f32": workload_f32, "qft": workload_qft, "qaoa": workload_qaoa,
+                "noisy": workload_noisy}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
 

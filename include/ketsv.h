@@ -246,6 +246,22 @@ int ksv_qaoa_batch_destroy(ksv_batch* b);
 int ksv_batch_last_error(char* buf, size_t len);
 
 /* ---- instrumentation -------------------------------------------------------------- */
+/* ---- batched trajectories -------------------------------------------- *
+ * Replaces run()'s per-shot loop (src/circuit.ts:3455-3481) over applyOps
+ * (:164-207), dep1 / dep2 (:289-304), collapseQubit (:55-70) and the
+ * final draw (:3425-3450), for circuits whose shots consume a fixed number
+ * of PRNG draws (no `if`).  The B = 2^(s->n - n) shots live in one state,
+ * copy t holding amplitudes [t 2^n, (t + 1) 2^n); gates reach every copy
+ * through ksv_apply / programs (their qubits are < n).  r[] and outcome[]
+ * have one entry per copy.  Outcomes, projections and draws repeat the
+ * single-state arithmetic of ksv_collapse / ksv_sample copy by copy.      */
+int ksv_shots_init(ksv_state* s, int n, uint64_t index);                 /* every copy := |index>       */
+int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy,   /* pauli[i] (1 X, 2 Y, 3 Z) on */
+                    const uint8_t* pauli, size_t count);                 /* qubit q of copy[i]          */
+int ksv_shots_measure(ksv_state* s, int n, int q, const double* r,      /* outcome = r < P(q = 1);     */
+                      const uint8_t* reset, uint8_t* outcome);           /* project; reset[t]: to |0>   */
+int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out); /* one draw per copy        */
+
 int ksv_get_stats(ksv_state* s, ksv_stats* out);
 int ksv_reset_stats(ksv_state* s);
 

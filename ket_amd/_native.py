@@ -74,6 +74,10 @@ SIGNATURES = [
     ("ksv_expect_pauli", c_int, [_P, _P, _P, _P, c_size_t, c_int, _P, _DP]),
     ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
     ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
+    ("ksv_shots_init", c_int, [_P, c_int, c_uint64]),
+    ("ksv_shots_pauli", c_int, [_P, c_int, c_int, _P, _P, c_size_t]),
+    ("ksv_shots_measure", c_int, [_P, c_int, c_int, _P, _P, _P]),
+    ("ksv_shots_sample", c_int, [_P, c_int, _P, _P]),
     ("ksv_qaoa_batch_create", c_int, [c_int, _P, _P, _P, c_int, c_int, POINTER(c_void_p)]),
     ("ksv_qaoa_batch_run", c_int, [_P, _P, _P, c_int, _P, _P, _P]),
     ("ksv_qaoa_batch_amplitudes", c_int, [_P, c_int, c_uint64, c_uint64, _P]),
@@ -288,6 +292,35 @@ class State:
         o = c_int()
         check(lib().ksv_collapse(self.handle, int(q), float(r), ctypes.byref(o)))
         return o.value
+
+    # batched trajectories: copies of an n-qubit state, copy t = amplitudes [t 2^n, (t+1) 2^n)
+    def shots_init(self, n, index=0):
+        check(lib().ksv_shots_init(self.handle, int(n), int(index)))
+
+    def shots_pauli(self, n, q, copies, paulis):
+        c = np.ascontiguousarray(copies, dtype=np.uint32)
+        p = np.ascontiguousarray(paulis, dtype=np.uint8)
+        if len(c) != len(p):
+            raise TypeError("copies and paulis differ in length")
+        check(lib().ksv_shots_pauli(self.handle, int(n), int(q), ptr(c), ptr(p), len(c)))
+
+    def shots_measure(self, n, q, r, reset=None):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if len(r) != 1 << (self.n - n):
+            raise TypeError("one draw per copy")
+        rs = None if reset is None else np.ascontiguousarray(reset, dtype=np.uint8)
+        out = np.empty(len(r), dtype=np.uint8)
+        check(lib().ksv_shots_measure(self.handle, int(n), int(q), ptr(r), ptr(rs) if rs is not None else None,
+                                      ptr(out)))
+        return out
+
+    def shots_sample(self, n, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if len(r) != 1 << (self.n - n):
+            raise TypeError("one draw per copy")
+        out = np.empty(len(r), dtype=np.uint64)
+        check(lib().ksv_shots_sample(self.handle, int(n), ptr(r), ptr(out)))
+        return out
 
     def copy_from(self, other):
         check(lib().ksv_copy_state(self.handle, other.handle))

@@ -26,6 +26,7 @@ OPTIONS = {
     "precision": "f64",  # 'f64' complex128 or 'f32' complex64
     "tile": 0,           # LDS tile qubits (0 = library default)
     "fuse": False,       # exact mode only: multiply runs of 1-qubit gates
+    "batch_shots": True,  # run() without `if`: all shots as copies of one device state
 }
 
 
@@ -793,7 +794,7 @@ class Circuit:
     # ---- run ------------------------------------------------------------------------
     def run(self, shots=1024, seed=None, noise=None, initialState=None):
         """src/circuit.ts:3406-3482"""
-        from .noise import resolve_noise, run_per_shot
+        from .noise import batchable, resolve_noise, run_batched, run_per_shot
         rng = make_prng(seed)
         init = _svfrom(initialState, self.qubits) if initialState is not None else 0
         params = resolve_noise(noise)
@@ -812,7 +813,10 @@ class Circuit:
             cmap = dict(zip(u.tolist(), cnt.tolist()))
             counts = {i: cmap[i] for i in first}
             return Distribution(self.qubits, shots, counts, creg_counts)
-        counts, creg_counts = run_per_shot(self, shots, rng, params, init, creg_counts)
+        if OPTIONS["batch_shots"] and batchable(self):
+            counts, creg_counts = run_batched(self, shots, rng, params, init, creg_counts)
+        else:
+            counts, creg_counts = run_per_shot(self, shots, rng, params, init, creg_counts)
         return Distribution(self.qubits, shots, counts, creg_counts)
 
 

@@ -746,6 +746,147 @@ int ksv_collapse(ksv_state* s, int q, double r, int* outcome) {
 }
 
 
+}  // extern "C"
+
+// ---- batched trajectories -------------------------------------------------
+// Device scratch freed on every exit path.
+struct DevBufs {
+    std::vector<void*> p;
+    ~DevBufs() {
+        for (void* x : p) hipFree(x);
+    }
+    template <typename T> hipError_t get(T** out, size_t count) {
+        void* x = nullptr;
+        hipError_t e = hipMalloc(&x, std::max<size_t>(1, count) * sizeof(T));
+        if (e == hipSuccess) p.push_back(x);
+        *out = (T*)x;
+        return e;
+    }
+};
+
+static int shots_check(ksv_state* s, int n, uint32_t* copies) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (n < 1 || n > s->n) return fail(KSV_ERR_RANGE, "copy width must be 1.." + std::to_string(s->n) + " qubits");
+    if (s->n - n > 16) return fail(KSV_ERR_RANGE, "at most 65536 copies");
+    *copies = 1u << (s->n - n);
+    return KSV_OK;
+}
+
+extern "C" {
+
+int ksv_shots_init(ksv_state* s, int n, uint64_t index) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (index >> n) return fail(KSV_ERR_RANGE, "basis index out of range for a " + std::to_string(n) + "-qubit copy");
+    HIPCHK(shots_basis(s->n, n, s->prec, s->psi, index, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy, const uint8_t* pauli, size_t count) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (q < 0 || q >= n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(n) + "-qubit circuit");
+    if (!count) return KSV_OK;
+    if (!copy || !pauli) return fail(KSV_ERR_TYPE, "null buffer");
+    if (count > 65535) return fail(KSV_ERR_RANGE, "at most 65535 Pauli entries per call");
+    for (size_t i = 0; i < count; i++) {
+        if (copy[i] >= copies) return fail(KSV_ERR_RANGE, "copy index out of range");
+        if (pauli[i] < 1 || pauli[i] > 3) return fail(KSV_ERR_TYPE, "Pauli code must be 1 (X), 2 (Y) or 3 (Z)");
+    }
+    DevBufs b;
+    uint32_t* d_copy = nullptr;
+    uint8_t* d_pauli = nullptr;
+    HIPCHK(b.get(&d_copy, count));
+    HIPCHK(b.get(&d_pauli, count));
+    HIPCHK(hipMemcpyAsync(d_copy, copy, 4 * count, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(d_pauli, pauli, count, hipMemcpyHostToDevice, s->st));
+    HIPCHK(shots_pauli(n, s->prec, s->psi, q, d_copy, d_pauli, (uint32_t)count, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+int ksv_shots_measure(ksv_state* s, int n, int q, const double* r, const uint8_t* reset, uint8_t* outcome) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (q < 0 || q >= n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(n) + "-qubit circuit");
+    if (!r || !outcome) return fail(KSV_ERR_TYPE, "null buffer");
+    DevBufs b;
+    double *d_part = nullptr, *d_p1 = nullptr, *d_scale = nullptr;
+    uint8_t *d_o = nullptr, *d_reset = nullptr;
+    const int gmax = (int)std::min<uint64_t>(NBLK, ((1ull << (n - 1)) + 255) / 256);
+    HIPCHK(b.get(&d_part, (size_t)std::max(1, gmax) * copies));
+    HIPCHK(b.get(&d_p1, copies));
+    HIPCHK(b.get(&d_scale, copies));
+    HIPCHK(b.get(&d_o, copies));
+    HIPCHK(b.get(&d_reset, copies));
+    int g = 0;
+    HIPCHK(shots_bit1(n, s->prec, s->psi, q, copies, NBLK, d_part, &g, s->st));
+    HIPCHK(shots_final(d_part, g, copies, d_p1, s->st));
+    std::vector<double> p1(copies), scale(copies);
+    HIPCHK(hipMemcpyAsync(p1.data(), d_p1, 8 * copies, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    std::vector<uint8_t> rs(copies, 0);
+    for (uint32_t t = 0; t < copies; t++) {       // collapseQubit's rule, as ksv_collapse
+        const int o = r[t] < p1[t] ? 1 : 0;
+        outcome[t] = (uint8_t)o;
+        scale[t] = 1 / std::sqrt(o ? p1[t] : 1 - p1[t]);
+        if (reset) rs[t] = reset[t] ? 1 : 0;
+    }
+    HIPCHK(hipMemcpyAsync(d_o, outcome, copies, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(d_scale, scale.data(), 8 * copies, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(d_reset, rs.data(), copies, hipMemcpyHostToDevice, s->st));
+    HIPCHK(shots_project(s->n, n, s->prec, s->psi, q, d_o, d_scale, d_reset, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (!r || !out) return fail(KSV_ERR_TYPE, "null buffer");
+    const double thresh = 1e-14;
+    const int64_t nch = (int64_t)(((1ull << n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG);
+    DevBufs b;
+    double *d_sum = nullptr, *d_r = nullptr;
+    int64_t* d_lastc = nullptr;
+    unsigned long long* d_last = nullptr;
+    uint64_t* d_out = nullptr;
+    HIPCHK(b.get(&d_sum, (size_t)nch * copies));
+    HIPCHK(b.get(&d_r, copies));
+    HIPCHK(b.get(&d_lastc, copies));
+    HIPCHK(b.get(&d_last, copies));
+    HIPCHK(b.get(&d_out, copies));
+    int64_t nc = 0;
+    HIPCHK(shots_chunk_sums(n, s->prec, s->psi, copies, thresh, d_sum, &nc, s->st));
+    std::vector<double> cum((size_t)nch * copies);
+    HIPCHK(hipMemcpyAsync(cum.data(), d_sum, 8 * cum.size(), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    std::vector<int64_t> lastc(copies, -1);
+    for (uint32_t t = 0; t < copies; t++) {       // ksv_sample's host scan, copy by copy
+        double acc = 0;
+        double* c = cum.data() + (size_t)t * nch;
+        for (int64_t k = 0; k < nch; k++) {
+            if (c[k] > 0) lastc[t] = k;
+            acc += c[k];
+            c[k] = acc;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_sum, cum.data(), 8 * cum.size(), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(d_lastc, lastc.data(), 8 * copies, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(d_r, r, 8 * copies, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemsetAsync(d_last, 0, 8 * copies, s->st));
+    HIPCHK(shots_lastnz(n, s->prec, s->psi, copies, thresh, d_lastc, d_last, s->st));
+    HIPCHK(shots_sample(n, s->prec, s->psi, copies, thresh, d_sum, nch, d_r, d_lastc, d_last, d_out, s->st));
+    HIPCHK(hipMemcpyAsync(out, d_out, 8 * copies, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
 int ksv_get_stats(ksv_state* s, ksv_stats* out) {
     if (!s || !out) return fail(KSV_ERR_HANDLE, "null state");
     *out = s->stats;

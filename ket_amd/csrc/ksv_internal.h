@@ -266,6 +266,24 @@ hipError_t k_init_basis(int n, int prec, void* psi, uint64_t idx, hipStream_t st
 hipError_t k_project(int n, int prec, void* psi, int q, int outcome, double scale, hipStream_t st);
 hipError_t k_copy(int n, int prec, void* dst, const void* src, hipStream_t st);
 
+// batched trajectories (k_shots.hip): copies of an n-qubit state inside an
+// ntot-qubit buffer
+hipError_t shots_basis(int ntot, int n, int prec, void* psi, uint64_t idx, hipStream_t st);
+hipError_t shots_pauli(int n, int prec, void* psi, int q, const uint32_t* d_copy, const uint8_t* d_pauli,
+                       uint32_t count, hipStream_t st);
+hipError_t shots_bit1(int n, int prec, const void* psi, int q, uint32_t copies, int nblk, double* d_part,
+                      int* grid_x, hipStream_t st);
+hipError_t shots_final(const double* d_part, int nblk, uint32_t copies, double* d_out, hipStream_t st);
+hipError_t shots_project(int ntot, int n, int prec, void* psi, int q, const uint8_t* d_out, const double* d_scale,
+                         const uint8_t* d_reset, hipStream_t st);
+hipError_t shots_chunk_sums(int n, int prec, const void* psi, uint32_t copies, double thresh, double* d_sums,
+                            int64_t* nch, hipStream_t st);
+hipError_t shots_lastnz(int n, int prec, const void* psi, uint32_t copies, double thresh, const int64_t* d_lastc,
+                        unsigned long long* d_last, hipStream_t st);
+hipError_t shots_sample(int n, int prec, const void* psi, uint32_t copies, double thresh, const double* d_ccum,
+                        int64_t nch, const double* d_r, const int64_t* d_lastc, const unsigned long long* d_last,
+                        uint64_t* d_out, hipStream_t st);
+
 // probability compaction / sampling / top-k
 constexpr int CHUNK_LOG = 12;     // amplitudes per block chunk in scans
 hipError_t chunk_counts(int n, int prec, const void* psi, double thresh, uint32_t* d_cnt,

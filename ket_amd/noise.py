@@ -1,6 +1,10 @@
 """run()'s per-shot path: noise, mid-circuit measurement, reset and `if`
-(src/circuit.ts:169-207, :209-304, :3455-3481), driven from the host over
-the device state.
+(src/circuit.ts:169-207, :209-304, :3455-3481).
+
+Two drivers.  `run_batched` (circuits without `if`) holds up to 4096 shots
+as copies of one device state and walks the circuit once for all of them;
+`run_per_shot` re-simulates shot by shot and serves circuits with `if`,
+whose draw count depends on measured values.
 
 Each shot re-simulates the circuit, consuming the seeded xorshift32 stream
 in ket's order: one draw per 1-qubit gate when p1 > 0, one per cnot /
@@ -146,3 +150,142 @@ def run_per_shot(circ, shots, rng, noise, init, creg_counts):
                     acc[j] += 1
     st.free()
     return counts, creg_counts
+
+
+# ---------------------------------------------------------------------------
+# Batched trajectories
+# ---------------------------------------------------------------------------
+MAX_BATCH_AMPS = 1 << 28          # amplitudes per batched state (4 GiB FP64)
+MAX_COPIES = 4096
+
+
+def batchable(circ):
+    """True when every shot consumes the same number of draws: no `if`."""
+    from .circuit import _flatten
+    return not any(op[0] == "if" for op in _flatten(circ._ops()))
+
+
+def draws_per_shot(flat, n, p1, p2, pm):
+    """Draws one shot consumes, in ket's order (src/circuit.ts:169-207, :3455-3481)."""
+    d = 0
+    for op in flat:
+        k = op[0]
+        if k == "single":
+            d += 1 if p1 else 0
+        elif k in ("cnot", "controlled", "swap", "two"):
+            d += 1 if p2 else 0
+        elif k == "measure":
+            d += 2 if pm else 1
+        elif k == "reset":
+            d += 1
+    return d + 1 + (n if pm else 0)
+
+
+def run_batched(circ, shots, rng, noise, init, creg_counts):
+    """All shots of a circuit without `if` as copies of one device state.
+
+    Shot s consumes draws [s D, (s + 1) D) of the stream, so they are drawn
+    up front; gates go to every copy at once, Pauli errors, measurements,
+    resets and the final draw copy by copy (ksv_shots_*).  Gives the same
+    counts as `run_per_shot` for the same seed.
+    """
+    from .circuit import OPTIONS, _flatten
+    n = circ.qubits
+    flat = _flatten(circ._ops())
+    p1 = (noise or {}).get("p1", 0) or 0
+    p2 = (noise or {}).get("p2", 0) or 0
+    pm = (noise or {}).get("pMeas", 0) or 0
+    D = draws_per_shot(flat, n, p1, p2, pm)
+    draws = np.fromiter((rng() for _ in range(shots * D)), dtype=np.float64, count=shots * D).reshape(shots, D)
+    b = 0
+    while (1 << (b + 1)) <= max(1, shots) and (1 << (b + 1)) <= MAX_COPIES and (2 << b) << n <= MAX_BATCH_AMPS:
+        b += 1
+    if (1 << b) < shots and (2 << b) << n <= MAX_BATCH_AMPS and (1 << (b + 1)) <= MAX_COPIES:
+        b += 1                                    # round up: one batch instead of two
+    B = 1 << b
+    counts = {}
+    st = State(n + b, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+    try:
+        for s0 in range(0, shots, B):
+            cnt = min(B, shots - s0)
+            dr = np.full((B, D), 0.5)
+            dr[:cnt] = draws[s0:s0 + cnt]
+            regs = {name: np.zeros((B, size), dtype=bool) for name, size in circ._cregs.items()}
+            idx = _run_copies(st, n, flat, dr, p1, p2, pm, init, regs)
+            for t in range(cnt):
+                i = int(idx[t])
+                counts[i] = counts.get(i, 0) + 1
+            for name, bits in regs.items():
+                acc = creg_counts[name]
+                for j in range(bits.shape[1]):
+                    acc[j] += int(bits[:cnt, j].sum())
+    finally:
+        st.free()
+    return counts, creg_counts
+
+
+# Pauli codes of the ksv_shots_pauli ABI (1 X, 2 Y, 3 Z; 0 none) for the 15
+# two-qubit Paulis of _TWO_PAULI, first and second qubit
+_PAIR_A = np.array([0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3], dtype=np.uint8)
+_PAIR_B = np.array([1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3], dtype=np.uint8)
+
+
+def _run_copies(st, n, flat, dr, p1, p2, pm, init, regs):
+    B = dr.shape[0]
+    st.shots_init(n, init)
+    pending = []
+    k = 0
+
+    def flush():
+        if pending:
+            st.apply(pack(pending))
+            pending.clear()
+
+    for op in flat:
+        kind = op[0]
+        if kind in ("single", "cnot", "controlled", "swap", "two", "toffoli", "cswap", "csrswap", "unitary"):
+            pending.append(op)
+        if kind == "single" and p1:                     # dep1: X, Y, Z by thirds of rand / p1
+            rand = dr[:, k]
+            k += 1
+            t = np.nonzero(rand < p1)[0]
+            if len(t):
+                flush()
+                r = rand[t] / p1
+                codes = (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8)
+                st.shots_pauli(n, op[1], t, codes)
+        elif kind in ("cnot", "controlled", "swap", "two") and p2:   # dep2: one of 15 Pauli pairs
+            rand = dr[:, k]
+            k += 1
+            t = np.nonzero(rand < p2)[0]
+            if len(t):
+                flush()
+                pick = np.minimum(np.floor(rand[t] / p2 * 15).astype(np.int64), 14)
+                for q, table in ((op[1], _PAIR_A), (op[2], _PAIR_B)):
+                    codes = table[pick]
+                    m = codes > 0
+                    if m.any():
+                        st.shots_pauli(n, q, t[m], codes[m])
+        elif kind == "measure":
+            flush()
+            o = st.shots_measure(n, op[1], dr[:, k])
+            k += 1
+            if pm:
+                o = o ^ (dr[:, k] < pm)
+                k += 1
+            reg = regs.get(op[2])
+            if reg is not None:
+                reg[:, op[3]] = o == 1
+        elif kind == "reset":
+            flush()
+            st.shots_measure(n, op[1], dr[:, k], reset=np.ones(B, dtype=np.uint8))
+            k += 1
+    flush()
+    idx = st.shots_sample(n, dr[:, k]).astype(np.uint64)
+    k += 1
+    if pm:
+        for q in range(n):
+            idx ^= (dr[:, k] < pm).astype(np.uint64) << np.uint64(q)
+            k += 1
+    assert k == dr.shape[1], "draw layout mismatch"
+    return idx

@@ -184,3 +184,79 @@ def expect_direct(n, psi, x, z):
 
 def collapse(n, psi, q, r):
     return lib().orc_collapse(n, _p(psi), q, r)
+
+
+# ---------------------------------------------------------------------------
+# run()'s per-shot path, restated (src/circuit.ts:164-207 applyOps,
+# :289-304 dep1/dep2, :55-81 collapseQubit/sampleSV, :3455-3481 run's
+# per-shot loop, :813-819 makePrng).  Calls are ket Circuit-method tuples
+# plus ("measure", (q, creg, bit)) and ("reset", (q,)).
+# ---------------------------------------------------------------------------
+_SINGLE = {m for m, k in NQ.items() if k == 1}
+_TWO_KIND = {"cnot", "cx", "swap", "xx", "yy", "zz", "xy", "iswap", "srswap", "ms", "cy", "cz", "ch", "crx",
+             "cry", "crz", "cr2", "cr4", "cr8", "cu1", "cu2", "cu3", "cs", "ct", "csdg", "ctdg", "csrn"}
+# the 15 non-identity two-qubit Paulis in ket's order (src/circuit.ts:273-278)
+_PAIRS = [(None, "x"), (None, "y"), (None, "z"), ("x", None), ("x", "x"), ("x", "y"), ("x", "z"),
+          ("y", None), ("y", "x"), ("y", "y"), ("y", "z"), ("z", None), ("z", "x"), ("z", "y"), ("z", "z")]
+
+
+def xorshift(seed):
+    """makePrng (src/circuit.ts:813-819): xorshift32 <<13 >>>17 <<5, r = s / 2^32, seed 0 -> 1."""
+    s = (int(seed) & 0xFFFFFFFF) or 1
+    while True:
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        yield s / 4294967296.0
+
+
+def run_shots(n, calls, shots, seed, noise=None, init=0, cregs=None, prune=True):
+    """Per-shot simulation; returns (counts {index: n} in first-seen order, creg bit counts)."""
+    rng = xorshift(seed)
+    p1 = (noise or {}).get("p1", 0) or 0
+    p2 = (noise or {}).get("p2", 0) or 0
+    pm = (noise or {}).get("pMeas", 0) or 0
+    cregs = dict(cregs or {})
+    counts, creg_counts = {}, {k: [0] * v for k, v in cregs.items()}
+    for _ in range(shots):
+        psi = zero_state(n, init)
+        bits = {k: [False] * v for k, v in cregs.items()}
+        for name, args in calls:
+            if name == "measure":
+                q, reg, b = args
+                o = collapse(n, psi, q, next(rng))
+                if pm and next(rng) < pm:
+                    o = 1 - o
+                if reg in bits:
+                    bits[reg][b] = o == 1
+                continue
+            if name == "reset":
+                (q,) = args
+                if collapse(n, psi, q, next(rng)) == 1:
+                    apply_named(n, psi, [("x", (q,))], prune)
+                continue
+            apply_named(n, psi, [(name, args)], prune)
+            qs = args[len(args) - NQ[name]:] if name in NQ else ()
+            if name in _SINGLE and p1:                      # dep1
+                rand = next(rng)
+                if rand < p1:
+                    r = rand / p1
+                    apply_named(n, psi, [("x" if r < 1 / 3 else "y" if r < 2 / 3 else "z", (qs[0],))], prune)
+            elif name in _TWO_KIND and p2:                  # dep2
+                rand = next(rng)
+                if rand < p2:
+                    pa, pb = _PAIRS[min(int(rand / p2 * 15), 14)]
+                    if pa:
+                        apply_named(n, psi, [(pa, (qs[0],))], prune)
+                    if pb:
+                        apply_named(n, psi, [(pb, (qs[1],))], prune)
+        idx = int(sample(n, psi, np.array([next(rng)]))[0])
+        if pm:
+            for q in range(n):
+                if next(rng) < pm:
+                    idx ^= 1 << q
+        counts[idx] = counts.get(idx, 0) + 1
+        for k, v in bits.items():
+            for j, b in enumerate(v):
+                creg_counts[k][j] += 1 if b else 0
+    return counts, creg_counts

@@ -54,8 +54,10 @@ def random_unitary(dim, rnd):
     return [[complex(v) for v in row] for row in q]
 
 
-def build(n, calls):
+def build(n, calls, cregs=None):
     c = ket_amd.Circuit(n)
+    for name, size in (cregs or {}).items():
+        c = c.creg(name, size)
     for name, args in calls:
         if name == "unitary":
             c = c.unitary(args[0], *args[1:])
@@ -69,3 +71,16 @@ def dense_ref(n, calls, prune=True, init=0):
     psi = O.zero_state(n, init)
     O.apply_named(n, psi, calls, prune=prune)
     return psi
+
+
+def shot_calls(n, count, seed, measures=2, resets=1):
+    """Seeded random gates with mid-circuit measurements and resets (no `if`)."""
+    rng = np.random.default_rng(seed)
+    calls = random_calls(n, count, seed=seed)
+    for j in range(measures):
+        at = int(rng.integers(1, len(calls)))
+        calls.insert(at, ("measure", (int(rng.integers(0, n)), "c", j)))
+    for _ in range(resets):
+        at = int(rng.integers(1, len(calls)))
+        calls.insert(at, ("reset", (int(rng.integers(0, n)),)))
+    return calls

@@ -13,7 +13,7 @@ from ket_amd import Circuit, workloads
 from ket_amd._native import State
 
 import ketsv_oracle as O
-from helpers import build, dense_ref, random_calls
+from helpers import build, dense_ref, random_calls, shot_calls
 
 pytestmark = pytest.mark.gpu
 
@@ -266,6 +266,72 @@ def test_noise_run_reproducible():
     a = c.run(shots=100, seed=5, noise="harmony")
     b = c.run(shots=100, seed=5, noise="harmony")
     assert a.probs == b.probs
+
+
+def _expected_dist(n, shots, counts, cr):
+    from ket_amd.circuit import Distribution
+    return Distribution(n, shots, counts, cr)
+
+
+@pytest.mark.parametrize("n,noise,seed", [(3, "harmony", 1), (5, {"p1": 0.02, "p2": 0.05, "pMeas": 0.03}, 2),
+                                          (6, None, 3), (9, "aria-1", 4)])
+def test_batched_shots_equal_oracle(n, noise, seed):
+    """run() without `if`: all shots as copies of one device state (ksv_shots_*)
+    give the oracle's per-shot counts, creg frequencies and first-seen order."""
+    from ket_amd.noise import resolve_noise
+    calls = shot_calls(n, 40, seed)
+    d = build(n, calls, cregs={"c": 2}).run(shots=300, seed=seed, noise=noise)
+    counts, cr = O.run_shots(n, calls, 300, seed, resolve_noise(noise), cregs={"c": 2})
+    e = _expected_dist(n, 300, counts, cr)
+    assert list(d.probs.items()) == list(e.probs.items())
+    assert d.cregs == e.cregs
+
+
+def test_batched_shots_equal_per_shot_path():
+    """The batched and the shot-by-shot drivers agree exactly (12 qubits, noise)."""
+    n = 12
+    calls = shot_calls(n, 60, 8, measures=2, resets=2)
+    c = build(n, calls, cregs={"c": 2})
+    a = c.run(shots=200, seed=77, noise="harmony")
+    old = ket_amd.configure(batch_shots=False)
+    try:
+        b = c.run(shots=200, seed=77, noise="harmony")
+    finally:
+        ket_amd.configure(**old)
+    assert list(a.probs.items()) == list(b.probs.items())
+    assert a.cregs == b.cregs
+
+
+def test_batched_shots_in_several_batches(monkeypatch):
+    """More shots than copies per batch: the draw stream continues across batches."""
+    import ket_amd.noise as N
+    monkeypatch.setattr(N, "MAX_COPIES", 16)
+    n = 4
+    calls = shot_calls(n, 25, 5)
+    d = build(n, calls, cregs={"c": 2}).run(shots=100, seed=5, noise={"p1": 0.05, "p2": 0.1, "pMeas": 0.02})
+    counts, cr = O.run_shots(n, calls, 100, 5, {"p1": 0.05, "p2": 0.1, "pMeas": 0.02}, cregs={"c": 2})
+    e = _expected_dist(n, 100, counts, cr)
+    assert list(d.probs.items()) == list(e.probs.items())
+    assert d.cregs == e.cregs
+
+
+def test_batched_shots_entry_points():
+    """ksv_shots_* directly: per-copy Paulis, measurement with reset, one draw per copy."""
+    n, b = 3, 2
+    st = State(n + b, "f64")
+    st.shots_init(n, 0b001)
+    st.shots_pauli(n, 2, [1, 3], [1, 2])          # X on copy 1, Y on copy 3 (qubit 2)
+    amps = st.amplitudes().reshape(4, 8)
+    assert amps[0, 1] == 1 and amps[2, 1] == 1
+    assert amps[1, 0b101] == 1 and amps[3, 0b101] == 1j
+    o = st.shots_measure(n, 2, np.array([0.5, 0.5, 0.5, 0.5]), reset=np.array([0, 1, 0, 0], dtype=np.uint8))
+    assert o.tolist() == [0, 1, 0, 1]
+    amps = st.amplitudes().reshape(4, 8)
+    assert amps[1, 0b001] == 1                      # reset moved copy 1 back to q2 = 0
+    assert abs(amps[3, 0b101] - 1j) < 1e-15
+    idx = st.shots_sample(n, np.array([0.1, 0.2, 0.3, 0.4]))
+    assert idx.tolist() == [1, 1, 1, 5]
+    st.free()
 
 
 def test_errors_map_to_ket_classes():

@@ -161,3 +161,18 @@ def test_product_path_fails_loudly_without_device():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError, match="no HIP device"):
         Circuit(2).h(0).statevector()
+
+
+def test_batched_shot_draw_layout():
+    """Draws per shot follow ket's per-shot loop: one per noisy 1q / 2q gate, one
+    per measure (+1 with pMeas), one per reset, the final draw, n SPAM draws."""
+    from ket_amd.noise import batchable, draws_per_shot
+    from ket_amd.circuit import _flatten
+    c = Circuit(3).creg("c", 2).h(0).cnot(0, 1).ccx(0, 1, 2).measure(1, "c", 0).reset(2).rz(0.3, 2)
+    flat = _flatten(c._ops())
+    assert draws_per_shot(flat, 3, 0, 0, 0) == 1 + 1 + 1
+    assert draws_per_shot(flat, 3, 1e-3, 0, 0) == 2 + 1 + 1 + 1
+    assert draws_per_shot(flat, 3, 1e-3, 1e-2, 0) == 2 + 1 + 1 + 1 + 1
+    assert draws_per_shot(flat, 3, 1e-3, 1e-2, 1e-2) == 2 + 1 + 2 + 1 + 1 + 3
+    assert batchable(c)
+    assert not batchable(c.if_("c", 1, lambda k: k.x(0)))

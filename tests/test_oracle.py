@@ -318,3 +318,50 @@ def test_marginals_bloch_collapse():
     O.apply_named(2, bell, [("h", (0,)), ("cnot", (0, 1))])
     out = O.collapse(2, bell, 0, 0.3)     # p1 = 0.5 > 0.3 -> outcome 1
     assert out == 1 and abs(bell[3] - 1) < 1e-12 and bell[0] == 0
+
+
+# ---- run()'s per-shot path (noise, measure, reset) ------------------------
+
+def _chi2(obs, exp):
+    return sum((o - e) ** 2 / e for o, e in zip(obs, exp) if e > 0)
+
+
+def test_shots_spam_flip_closed_form():
+    """pMeas = 1 flips every reported bit (src/circuit.ts:3472-3478)."""
+    counts, cr = O.run_shots(3, [("x", (0,)), ("measure", (0, "c", 0))], 64, 7, {"pMeas": 1}, cregs={"c": 1})
+    assert counts == {0b110: 64}
+    assert cr == {"c": [0]}
+
+
+def test_shots_dep1_uniform_paulis():
+    """p1 = 1: every 1-qubit gate is followed by X, Y or Z with probability 1/3 each (src/circuit.ts:289-295)."""
+    counts, _ = O.run_shots(1, [("id", (0,))], 6000, 11, {"p1": 1})
+    assert _chi2([counts.get(0, 0), counts.get(1, 0)], [2000, 4000]) < 10.8      # 1 dof, p = 0.001
+
+
+def test_shots_dep2_fifteen_paulis():
+    """p2 = 1 on cnot from |00>: the 15 two-qubit Paulis are equally likely (src/circuit.ts:273-278, :296-304)."""
+    counts, _ = O.run_shots(2, [("cnot", (0, 1))], 7500, 12, {"p2": 1})
+    # index bit 0 flips when the control's Pauli is X or Y, bit 1 likewise for the target
+    exp = [3 / 15 * 7500, 4 / 15 * 7500, 4 / 15 * 7500, 4 / 15 * 7500]
+    assert _chi2([counts.get(i, 0) for i in range(4)], exp) < 16.3              # 3 dof, p = 0.001
+
+
+def test_shots_measure_and_reset_semantics():
+    counts, cr = O.run_shots(2, [("h", (0,)), ("measure", (0, "c", 0)), ("cnot", (0, 1)), ("reset", (0,))], 2000, 3,
+                             cregs={"c": 1})
+    # after the reset q0 is 0; q1 copies the measured value
+    assert set(counts) <= {0b00, 0b10}
+    assert counts.get(0b10, 0) == cr["c"][0]
+    assert abs(cr["c"][0] / 2000 - 0.5) < 0.05
+
+
+def test_shots_noise_free_matches_born_rule():
+    calls = random_calls(4, 30, seed=5)
+    psi = O.zero_state(4)
+    O.apply_named(4, psi, calls)
+    p = np.abs(psi) ** 2
+    counts, _ = O.run_shots(4, calls, 4000, 21)
+    obs = [counts.get(i, 0) for i in range(16)]
+    keep = p * 4000 >= 5
+    assert _chi2(np.array(obs)[keep], (p * 4000)[keep]) < 40
