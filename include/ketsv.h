@@ -262,6 +262,30 @@ int ksv_shots_measure(ksv_state* s, int n, int q, const double* r,      /* outco
                       const uint8_t* reset, uint8_t* outcome);           /* project; reset[t]: to |0>   */
 int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out); /* one draw per copy        */
 
+/* makePrng's stream (src/circuit.ts:813-819: xorshift32 <<13 >>>17 <<5,
+ * r = s / 2^32) as `count` draws; *state advances.  Host-only: the batched
+ * path draws every shot's numbers up front.                               */
+int ksv_xorshift32(uint32_t* state, size_t count, double* out);
+
+/* A whole batch in one call: the gate tape with events between its ops.
+ * Event i happens after ops [0, at): a Pauli list (copy[off..off+count),
+ * pauli[...]) on qubit q, or a measurement / reset of qubit q with draws
+ * r[off..off+B) and outcomes written to outcome[off..off+B).  Positions
+ * ascend; everything is validated before the state changes.              */
+#define KSV_SHOT_PAULI   0
+#define KSV_SHOT_MEASURE 1
+#define KSV_SHOT_RESET   2
+typedef struct ksv_shot_event {
+    uint64_t at;        /* tape position                                  */
+    int32_t  kind;      /* KSV_SHOT_*                                     */
+    int32_t  q;         /* qubit (< n)                                    */
+    uint64_t off;       /* Pauli: first list entry; measure/reset: first draw */
+    uint64_t count;     /* Pauli: list entries                             */
+} ksv_shot_event;
+int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                  const ksv_shot_event* ev, size_t nev, const uint32_t* copy, const uint8_t* pauli,
+                  size_t npauli, const double* r, size_t nr, uint8_t* outcome);
+
 int ksv_get_stats(ksv_state* s, ksv_stats* out);
 int ksv_reset_stats(ksv_state* s);
 

@@ -78,6 +78,9 @@ SIGNATURES = [
     ("ksv_shots_pauli", c_int, [_P, c_int, c_int, _P, _P, c_size_t]),
     ("ksv_shots_measure", c_int, [_P, c_int, c_int, _P, _P, _P]),
     ("ksv_shots_sample", c_int, [_P, c_int, _P, _P]),
+    ("ksv_xorshift32", c_int, [_P, c_size_t, _P]),
+    ("ksv_shots_run", c_int, [_P, c_int, _P, c_size_t, _P, c_size_t, _P, c_size_t, _P, _P, c_size_t, _P, c_size_t,
+                              _P]),
     ("ksv_qaoa_batch_create", c_int, [c_int, _P, _P, _P, c_int, c_int, POINTER(c_void_p)]),
     ("ksv_qaoa_batch_run", c_int, [_P, _P, _P, c_int, _P, _P, _P]),
     ("ksv_qaoa_batch_amplitudes", c_int, [_P, c_int, c_uint64, c_uint64, _P]),
@@ -143,6 +146,11 @@ def plan(n, tape, precision="f64", kernel="auto", tile=0, rtile=0, fuse=False, p
                                 len(mats), KERNELS[kernel], int(tile), int(rtile), int(btile), int(bq),
                                 1 if fuse else 0, 1 if prune else 0, ctypes.byref(s)))
     return {f: getattr(s, f) for f, _ in KsvProgramSummary._fields_}
+
+
+SHOT_EVENT_DTYPE = np.dtype([("at", np.uint64), ("kind", np.int32), ("q", np.int32), ("off", np.uint64),
+                             ("count", np.uint64)])
+KSV_SHOT_PAULI, KSV_SHOT_MEASURE, KSV_SHOT_RESET = 0, 1, 2
 
 
 def ptr(a):
@@ -322,6 +330,21 @@ class State:
         check(lib().ksv_shots_sample(self.handle, int(n), ptr(r), ptr(out)))
         return out
 
+    def shots_run(self, n, tape, events, copies, paulis, r):
+        """One batch: the gate tape with SHOT_EVENT_DTYPE events; returns the outcomes array (len(r))."""
+        ops, mats = tape
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        mats = np.ascontiguousarray(mats, dtype=np.float64)
+        ev = np.ascontiguousarray(events, dtype=SHOT_EVENT_DTYPE)
+        c = np.ascontiguousarray(copies, dtype=np.uint32)
+        pa = np.ascontiguousarray(paulis, dtype=np.uint8)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(max(1, len(r)), dtype=np.uint8)
+        m = mats if len(mats) else np.zeros(1)
+        check(lib().ksv_shots_run(self.handle, int(n), ptr(ops), len(ops), ptr(m), len(mats), ptr(ev), len(ev),
+                                  ptr(c), ptr(pa), len(c), ptr(r), len(r), ptr(out)))
+        return out[:len(r)]
+
     def copy_from(self, other):
         check(lib().ksv_copy_state(self.handle, other.handle))
 
@@ -436,3 +459,10 @@ class QaoaBatch:
             self.free()
         except Exception:
             pass
+
+
+def xorshift_fill(state, count):
+    """`count` draws of the xorshift32 stream from uint32 `state` (advanced in place: a 1-element uint32 array)."""
+    out = np.empty(count, dtype=np.float64)
+    check(lib().ksv_xorshift32(ptr(state), count, ptr(out)))
+    return out

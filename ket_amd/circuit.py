@@ -65,6 +65,15 @@ def make_prng(seed=None):
         state[0] = v
         return v / 4294967296.0
 
+    def fill(count):
+        """The next `count` draws as a float64 array (same stream as `count` calls)."""
+        from ._native import xorshift_fill
+        st = np.array([state[0]], dtype=np.uint32)
+        out = xorshift_fill(st, count)
+        state[0] = int(st[0])
+        return out
+
+    rng.fill = fill
     return rng
 
 

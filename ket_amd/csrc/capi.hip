@@ -54,7 +54,10 @@ struct ksv_state {
     std::vector<hipEvent_t> ev;
 };
 
-struct ksv_program : public Program {};
+struct ksv_program : public Program {
+    void* d_blob = nullptr;     // one device allocation holding every array below (owned)
+    bool borrowed = false;      // arrays live in a caller's blob: nothing to free
+};
 
 static size_t amp_bytes(const ksv_state* s) { return s->prec == KSV_F64 ? 16 : 8; }
 static size_t state_bytes(const ksv_state* s) { return amp_bytes(s) << s->n; }
@@ -201,6 +204,65 @@ static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops
     return KSV_OK;
 }
 
+// Serialise a planned program's device arrays (ROps, DOps, BRecs, block
+// fragments, matrices in the state's precision) into `blob` at 256-byte
+// aligned offsets; off[] receives the five offsets (SIZE_MAX = absent).
+static void program_blob(const ksv_state* s, const Program& p, std::vector<char>& blob, std::vector<size_t>& off) {
+    auto put = [&](const void* src, size_t bytes) -> size_t {
+        if (!bytes) return SIZE_MAX;
+        size_t at = (blob.size() + 255) & ~(size_t)255;
+        blob.resize(at + bytes);
+        std::memcpy(blob.data() + at, src, bytes);
+        return at;
+    };
+    off.assign(5, SIZE_MAX);
+    off[0] = put(p.rops.data(), sizeof(ROp) * p.rops.size());
+    off[1] = put(p.dops.data(), sizeof(DOp) * p.dops.size());
+    off[2] = put(p.brecs.data(), sizeof(BRec) * p.brecs.size());
+    if (!p.brecs.empty()) {
+        size_t total = 0;
+        std::vector<int> bqs(p.brecs.size(), 4);
+        for (const Step& st : p.steps)
+            if (st.blk)
+                for (int i = 0; i < st.op_count; i++) bqs[st.op_begin + i] = st.bq;
+        for (size_t i = 0; i < p.brecs.size(); i++) {
+            const int R = 1 << bqs[i];
+            total = std::max(total, (size_t)p.brecs[i].frag + (size_t)(2 * R / 16) * (2 * R / 4) * 64);
+        }
+        std::vector<double> fr(total, 0.0);
+        for (size_t i = 0; i < p.brecs.size(); i++)
+            block_fragments(p.bmats.data() + p.bmat_off[i], bqs[i], fr.data() + p.brecs[i].frag);
+        if (s->prec == KSV_F64) {
+            off[3] = put(fr.data(), 8 * total);
+        } else {
+            std::vector<float> ff(total);
+            for (size_t i = 0; i < total; i++) ff[i] = (float)fr[i];
+            off[3] = put(ff.data(), 4 * total);
+        }
+    }
+    if (!p.mats.empty()) {
+        if (s->prec == KSV_F64) {
+            off[4] = put(p.mats.data(), 16 * p.mats.size());
+        } else {
+            std::vector<float> f(2 * p.mats.size());
+            for (size_t i = 0; i < p.mats.size(); i++) {
+                f[2 * i] = (float)p.mats[i].real();
+                f[2 * i + 1] = (float)p.mats[i].imag();
+            }
+            off[4] = put(f.data(), 8 * p.mats.size());
+        }
+    }
+}
+
+static void bind_blob(Program& p, char* base, const std::vector<size_t>& off) {
+    auto at = [&](size_t o) -> void* { return o == SIZE_MAX ? nullptr : base + o; };
+    p.d_rops = (ROp*)at(off[0]);
+    p.d_ops = (DOp*)at(off[1]);
+    p.d_brecs = (BRec*)at(off[2]);
+    p.d_frags = at(off[3]);
+    p.d_mats = at(off[4]);
+}
+
 static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                          int prune, bool fuse, ksv_program** out) {
     std::unique_ptr<ksv_program> p(new ksv_program());
@@ -209,58 +271,19 @@ static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const dou
     if (rc) return rc;
     p->dev = s->dev;
     p->bthreads = s->bthreads;
-    if (!p->rops.empty()) {
-        HIPCHK(hipMalloc(&p->d_rops, sizeof(ROp) * p->rops.size()));
-        HIPCHK(hipMemcpy(p->d_rops, p->rops.data(), sizeof(ROp) * p->rops.size(), hipMemcpyHostToDevice));
-    }
-    if (!p->dops.empty()) {
-        HIPCHK(hipMalloc(&p->d_ops, sizeof(DOp) * p->dops.size()));
-        HIPCHK(hipMemcpy(p->d_ops, p->dops.data(), sizeof(DOp) * p->dops.size(), hipMemcpyHostToDevice));
-    }
-    if (!p->brecs.empty()) {
-        HIPCHK(hipMalloc(&p->d_brecs, sizeof(BRec) * p->brecs.size()));
-        HIPCHK(hipMemcpy(p->d_brecs, p->brecs.data(), sizeof(BRec) * p->brecs.size(), hipMemcpyHostToDevice));
-        size_t total = 0;
-        std::vector<int> bqs(p->brecs.size(), 4);
-        for (const Step& st : p->steps)
-            if (st.blk)
-                for (int i = 0; i < st.op_count; i++) bqs[st.op_begin + i] = st.bq;
-        for (size_t i = 0; i < p->brecs.size(); i++) {
-            const int R = 1 << bqs[i];
-            total = std::max(total, (size_t)p->brecs[i].frag + (size_t)(2 * R / 16) * (2 * R / 4) * 64);
-        }
-        std::vector<double> fr(total, 0.0);
-        for (size_t i = 0; i < p->brecs.size(); i++)
-            block_fragments(p->bmats.data() + p->bmat_off[i], bqs[i], fr.data() + p->brecs[i].frag);
-        if (s->prec == KSV_F64) {
-            HIPCHK(hipMalloc(&p->d_frags, 8 * total));
-            HIPCHK(hipMemcpy(p->d_frags, fr.data(), 8 * total, hipMemcpyHostToDevice));
-        } else {
-            std::vector<float> ff(total);
-            for (size_t i = 0; i < total; i++) ff[i] = (float)fr[i];
-            HIPCHK(hipMalloc(&p->d_frags, 4 * total));
-            HIPCHK(hipMemcpy(p->d_frags, ff.data(), 4 * total, hipMemcpyHostToDevice));
-        }
-    }
-    if (!p->mats.empty()) {
-        if (s->prec == KSV_F64) {
-            HIPCHK(hipMalloc(&p->d_mats, 16 * p->mats.size()));
-            HIPCHK(hipMemcpy(p->d_mats, p->mats.data(), 16 * p->mats.size(), hipMemcpyHostToDevice));
-        } else {
-            std::vector<float> f(2 * p->mats.size());
-            for (size_t i = 0; i < p->mats.size(); i++) {
-                f[2 * i] = (float)p->mats[i].real();
-                f[2 * i + 1] = (float)p->mats[i].imag();
-            }
-            HIPCHK(hipMalloc(&p->d_mats, 8 * p->mats.size()));
-            HIPCHK(hipMemcpy(p->d_mats, f.data(), 8 * p->mats.size(), hipMemcpyHostToDevice));
-        }
+    std::vector<char> blob;
+    std::vector<size_t> off;
+    program_blob(s, *p, blob, off);
+    if (!blob.empty()) {
+        HIPCHK(hipMalloc(&p->d_blob, blob.size()));
+        HIPCHK(hipMemcpy(p->d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
+        bind_blob(*p, (char*)p->d_blob, off);
     }
     *out = p.release();
     return KSV_OK;
 }
 
-static int run_program_on(ksv_state* s, ksv_program* p, void* psi) {
+static int run_program_on(ksv_state* s, ksv_program* p, void* psi, bool sync = true) {
     const bool timing = s->timing != 0;
     const size_t nst = p->steps.size();
     if (timing) {
@@ -278,6 +301,12 @@ static int run_program_on(ksv_state* s, ksv_program* p, void* psi) {
         else if (st.blk) HIPCHK(launch_bpass(*p, st, psi, s->st));
         else HIPCHK(launch_pass(*p, st, psi, s->st));
         if (timing) HIPCHK(hipEventRecord(s->ev[i + 1], s->st));
+    }
+    if (!sync && !timing) {
+        s->stats.pass_launches += nst;
+        s->stats.pass_bytes += 2.0 * (double)state_bytes(s) * nst;
+        s->stats.gates_applied += p->tape_ops;
+        return KSV_OK;
     }
     HIPCHK(hipStreamSynchronize(s->st));
     const double bytes = 2.0 * (double)state_bytes(s);
@@ -380,11 +409,7 @@ int ksv_program_info(const ksv_program* p, ksv_program_summary* out) {
 
 int ksv_program_destroy(ksv_program* p) {
     if (!p) return fail(KSV_ERR_HANDLE, "null program");
-    if (p->d_ops) hipFree(p->d_ops);
-    if (p->d_rops) hipFree(p->d_rops);
-    if (p->d_brecs) hipFree(p->d_brecs);
-    if (p->d_frags) hipFree(p->d_frags);
-    if (p->d_mats) hipFree(p->d_mats);
+    if (p->d_blob && !p->borrowed) (void)hipFree(p->d_blob);
     delete p;
     return KSV_OK;
 }
@@ -774,6 +799,19 @@ static int shots_check(ksv_state* s, int n, uint32_t* copies) {
 
 extern "C" {
 
+int ksv_xorshift32(uint32_t* state, size_t count, double* out) {
+    if (!state || (count && !out)) return fail(KSV_ERR_TYPE, "null buffer");
+    uint32_t v = *state ? *state : 1u;
+    for (size_t i = 0; i < count; i++) {
+        v ^= v << 13;
+        v ^= v >> 17;
+        v ^= v << 5;
+        out[i] = v / 4294967296.0;
+    }
+    *state = v;
+    return KSV_OK;
+}
+
 int ksv_shots_init(ksv_state* s, int n, uint64_t index) {
     uint32_t copies = 0;
     int rc = shots_check(s, n, &copies);
@@ -808,12 +846,12 @@ int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy, const uint
     return KSV_OK;
 }
 
-int ksv_shots_measure(ksv_state* s, int n, int q, const double* r, const uint8_t* reset, uint8_t* outcome) {
-    uint32_t copies = 0;
-    int rc = shots_check(s, n, &copies);
-    if (rc) return rc;
-    if (q < 0 || q >= n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(n) + "-qubit circuit");
-    if (!r || !outcome) return fail(KSV_ERR_TYPE, "null buffer");
+}  // extern "C"
+
+// Per-copy P(q = 1), collapseQubit's outcome rule and the projection (with
+// the reset flip); reset == nullptr: no copy resets, reset_all: every copy.
+static int shots_measure_impl(ksv_state* s, int n, uint32_t copies, int q, const double* r, const uint8_t* reset,
+                              bool reset_all, uint8_t* outcome) {
     DevBufs b;
     double *d_part = nullptr, *d_p1 = nullptr, *d_scale = nullptr;
     uint8_t *d_o = nullptr, *d_reset = nullptr;
@@ -834,12 +872,117 @@ int ksv_shots_measure(ksv_state* s, int n, int q, const double* r, const uint8_t
         const int o = r[t] < p1[t] ? 1 : 0;
         outcome[t] = (uint8_t)o;
         scale[t] = 1 / std::sqrt(o ? p1[t] : 1 - p1[t]);
-        if (reset) rs[t] = reset[t] ? 1 : 0;
+        rs[t] = reset_all ? 1 : (reset && reset[t]) ? 1 : 0;
     }
     HIPCHK(hipMemcpyAsync(d_o, outcome, copies, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(d_scale, scale.data(), 8 * copies, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(d_reset, rs.data(), copies, hipMemcpyHostToDevice, s->st));
     HIPCHK(shots_project(s->n, n, s->prec, s->psi, q, d_o, d_scale, d_reset, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
+extern "C" {
+
+int ksv_shots_measure(ksv_state* s, int n, int q, const double* r, const uint8_t* reset, uint8_t* outcome) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (q < 0 || q >= n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(q) + " is out of range for a " + std::to_string(n) + "-qubit circuit");
+    if (!r || !outcome) return fail(KSV_ERR_TYPE, "null buffer");
+    return shots_measure_impl(s, n, copies, q, r, reset, false, outcome);
+}
+
+int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                  const ksv_shot_event* ev, size_t nev, const uint32_t* copy, const uint8_t* pauli, size_t npauli,
+                  const double* r, size_t nr, uint8_t* outcome) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if ((nops && !ops) || (nev && !ev)) return fail(KSV_ERR_TYPE, "null buffer");
+    // validate everything before touching the state
+    uint64_t at = 0;
+    for (size_t i = 0; i < nev; i++) {
+        const ksv_shot_event& e = ev[i];
+        if (e.at < at || e.at > nops) return fail(KSV_ERR_RANGE, "event positions must be ascending and within the tape");
+        at = e.at;
+        if (e.q < 0 || e.q >= n) return fail(KSV_ERR_RANGE, "qubit index " + std::to_string(e.q) + " is out of range for a " + std::to_string(n) + "-qubit circuit");
+        if (e.kind == KSV_SHOT_PAULI) {
+            if (!copy || !pauli || e.off > npauli || e.count > npauli - e.off || e.count > 65535)
+                return fail(KSV_ERR_RANGE, "Pauli event range out of bounds");
+            for (uint64_t j = e.off; j < e.off + e.count; j++) {
+                if (copy[j] >= copies) return fail(KSV_ERR_RANGE, "copy index out of range");
+                if (pauli[j] < 1 || pauli[j] > 3) return fail(KSV_ERR_TYPE, "Pauli code must be 1 (X), 2 (Y) or 3 (Z)");
+            }
+        } else if (e.kind == KSV_SHOT_MEASURE || e.kind == KSV_SHOT_RESET) {
+            if (!r || !outcome || e.off > nr || copies > nr - e.off) return fail(KSV_ERR_RANGE, "draw range out of bounds");
+        } else {
+            return fail(KSV_ERR_TYPE, "unknown shot event kind");
+        }
+    }
+    // plan every gate segment between events on the host, then one upload
+    std::vector<std::unique_ptr<ksv_program>> seg;
+    std::vector<std::vector<size_t>> segoff;
+    std::vector<char> blob;
+    size_t cur = 0;
+    for (size_t i = 0; i <= nev; i++) {
+        const size_t upto = i < nev ? (size_t)ev[i].at : nops;
+        if (upto > cur) {
+            std::unique_ptr<ksv_program> p(new ksv_program());
+            rc = plan_into(*p, s->n, s->prec, ops + cur, upto - cur, mats, nmats, s->kernel, default_tile(s), s->rtile,
+                           s->fuse != 0, s->prune, s->btile, s->bq);
+            if (rc) return rc;
+            p->dev = s->dev;
+            p->bthreads = s->bthreads;
+            p->borrowed = true;
+            std::vector<size_t> off;
+            std::vector<char> one;
+            program_blob(s, *p, one, off);
+            const size_t base = (blob.size() + 255) & ~(size_t)255;
+            blob.resize(base + one.size());
+            std::memcpy(blob.data() + base, one.data(), one.size());
+            for (size_t& o : off)
+                if (o != SIZE_MAX) o += base;
+            seg.push_back(std::move(p));
+            segoff.push_back(off);
+            cur = upto;
+        }
+    }
+    DevBufs b;
+    char* d_blob = nullptr;
+    uint32_t* d_copy = nullptr;
+    uint8_t* d_pauli = nullptr;
+    if (!blob.empty()) {
+        HIPCHK(b.get(&d_blob, blob.size()));
+        HIPCHK(hipMemcpy(d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    }
+    for (size_t i = 0; i < seg.size(); i++) bind_blob(*seg[i], d_blob, segoff[i]);
+    if (npauli) {
+        HIPCHK(b.get(&d_copy, npauli));
+        HIPCHK(b.get(&d_pauli, npauli));
+        HIPCHK(hipMemcpy(d_copy, copy, 4 * npauli, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_pauli, pauli, npauli, hipMemcpyHostToDevice));
+    }
+    size_t si = 0;
+    cur = 0;
+    for (size_t i = 0; i <= nev; i++) {
+        const size_t upto = i < nev ? (size_t)ev[i].at : nops;
+        if (upto > cur) {
+            rc = run_program_on(s, seg[si].get(), s->psi, false);
+            if (rc) return rc;
+            si++;
+            cur = upto;
+        }
+        if (i == nev) break;
+        const ksv_shot_event& e = ev[i];
+        if (e.kind == KSV_SHOT_PAULI) {
+            if (e.count)
+                HIPCHK(shots_pauli(n, s->prec, s->psi, e.q, d_copy + e.off, d_pauli + e.off, (uint32_t)e.count, s->st));
+        } else {
+            rc = shots_measure_impl(s, n, copies, e.q, r + e.off, nullptr, e.kind == KSV_SHOT_RESET, outcome + e.off);
+            if (rc) return rc;
+        }
+    }
     HIPCHK(hipStreamSynchronize(s->st));
     return KSV_OK;
 }

@@ -16,7 +16,7 @@ reset) are applied as one tape.
 import numpy as np
 
 from . import gates as G
-from ._native import State
+from ._native import KSV_SHOT_MEASURE, KSV_SHOT_PAULI, KSV_SHOT_RESET, SHOT_EVENT_DTYPE, State
 from .tape import pack
 
 # Published device error rates (src/circuit.ts:249-261).
@@ -196,7 +196,10 @@ def run_batched(circ, shots, rng, noise, init, creg_counts):
     p2 = (noise or {}).get("p2", 0) or 0
     pm = (noise or {}).get("pMeas", 0) or 0
     D = draws_per_shot(flat, n, p1, p2, pm)
-    draws = np.fromiter((rng() for _ in range(shots * D)), dtype=np.float64, count=shots * D).reshape(shots, D)
+    if hasattr(rng, "fill"):
+        draws = rng.fill(shots * D).reshape(shots, D)
+    else:
+        draws = np.fromiter((rng() for _ in range(shots * D)), dtype=np.float64, count=shots * D).reshape(shots, D)
     b = 0
     while (1 << (b + 1)) <= max(1, shots) and (1 << (b + 1)) <= MAX_COPIES and (2 << b) << n <= MAX_BATCH_AMPS:
         b += 1
@@ -231,56 +234,62 @@ _PAIR_B = np.array([1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3], dtype=np.uint8
 
 
 def _run_copies(st, n, flat, dr, p1, p2, pm, init, regs):
+    """One batch: events are collected on the host, then the whole batch runs in
+    one ksv_shots_run call; returns the drawn index of every copy."""
     B = dr.shape[0]
     st.shots_init(n, init)
-    pending = []
+    gates, events, copies, codes_all, draws, meas = [], [], [], [], [], []
     k = 0
 
-    def flush():
-        if pending:
-            st.apply(pack(pending))
-            pending.clear()
+    def paulis(q, t, codes):
+        events.append((len(gates), KSV_SHOT_PAULI, q, len(copies), len(t)))
+        copies.extend(t.tolist())
+        codes_all.extend(codes.tolist())
 
     for op in flat:
         kind = op[0]
         if kind in ("single", "cnot", "controlled", "swap", "two", "toffoli", "cswap", "csrswap", "unitary"):
-            pending.append(op)
+            gates.append(op)
         if kind == "single" and p1:                     # dep1: X, Y, Z by thirds of rand / p1
             rand = dr[:, k]
             k += 1
             t = np.nonzero(rand < p1)[0]
             if len(t):
-                flush()
                 r = rand[t] / p1
-                codes = (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8)
-                st.shots_pauli(n, op[1], t, codes)
+                paulis(op[1], t, (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8))
         elif kind in ("cnot", "controlled", "swap", "two") and p2:   # dep2: one of 15 Pauli pairs
             rand = dr[:, k]
             k += 1
             t = np.nonzero(rand < p2)[0]
             if len(t):
-                flush()
                 pick = np.minimum(np.floor(rand[t] / p2 * 15).astype(np.int64), 14)
                 for q, table in ((op[1], _PAIR_A), (op[2], _PAIR_B)):
                     codes = table[pick]
                     m = codes > 0
                     if m.any():
-                        st.shots_pauli(n, q, t[m], codes[m])
-        elif kind == "measure":
-            flush()
-            o = st.shots_measure(n, op[1], dr[:, k])
+                        paulis(q, t[m], codes[m])
+        elif kind in ("measure", "reset"):
+            off = len(draws) * B
+            events.append((len(gates), KSV_SHOT_MEASURE if kind == "measure" else KSV_SHOT_RESET, op[1], off, 0))
+            draws.append(dr[:, k])
             k += 1
-            if pm:
-                o = o ^ (dr[:, k] < pm)
-                k += 1
-            reg = regs.get(op[2])
-            if reg is not None:
-                reg[:, op[3]] = o == 1
-        elif kind == "reset":
-            flush()
-            st.shots_measure(n, op[1], dr[:, k], reset=np.ones(B, dtype=np.uint8))
-            k += 1
-    flush()
+            if kind == "measure":
+                flip = None
+                if pm:
+                    flip = dr[:, k] < pm
+                    k += 1
+                meas.append((op, off, flip))
+    ev = np.array(events, dtype=SHOT_EVENT_DTYPE) if events else np.zeros(0, dtype=SHOT_EVENT_DTYPE)
+    r = np.concatenate(draws) if draws else np.zeros(0)
+    out = st.shots_run(n, pack(gates), ev, np.asarray(copies, dtype=np.uint32),
+                       np.asarray(codes_all, dtype=np.uint8), r)
+    for op, off, flip in meas:
+        o = out[off:off + B]
+        if flip is not None:
+            o = o ^ flip
+        reg = regs.get(op[2])
+        if reg is not None:
+            reg[:, op[3]] = o == 1
     idx = st.shots_sample(n, dr[:, k]).astype(np.uint64)
     k += 1
     if pm:

@@ -176,3 +176,15 @@ def test_batched_shot_draw_layout():
     assert draws_per_shot(flat, 3, 1e-3, 1e-2, 1e-2) == 2 + 1 + 2 + 1 + 1 + 3
     assert batchable(c)
     assert not batchable(c.if_("c", 1, lambda k: k.x(0)))
+
+
+def test_prng_fill_continues_the_stream():
+    """rng.fill(k) (ksv_xorshift32) gives the next k draws of the same stream."""
+    a = make_prng(12345)
+    b = make_prng(12345)
+    first = [a() for _ in range(5)]
+    assert first == [b() for _ in range(5)]
+    filled = b.fill(1000)
+    assert filled.tolist() == [a() for _ in range(1000)]
+    assert a() == b()
+    assert make_prng(0).fill(3).tolist() == make_prng(1).fill(3).tolist()     # seed 0 -> 1
