@@ -256,6 +256,10 @@ int ksv_batch_last_error(char* buf, size_t len);
  * have one entry per copy.  Outcomes, projections and draws repeat the
  * single-state arithmetic of ksv_collapse / ksv_sample copy by copy.      */
 int ksv_shots_init(ksv_state* s, int n, uint64_t index);                 /* every copy := |index>       */
+/* copy t := |t> with 2^n copies (s->n = 2n): one run of a circuit leaves
+ * column t of its matrix in copy t.  Replaces circuitMatrix()'s loop of 2^n
+ * simulations (src/circuit.ts:1463-1485).                                 */
+int ksv_shots_init_identity(ksv_state* s, int n);
 int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy,   /* pauli[i] (1 X, 2 Y, 3 Z) on */
                     const uint8_t* pauli, size_t count);                 /* qubit q of copy[i]          */
 int ksv_shots_measure(ksv_state* s, int n, int q, const double* r,      /* outcome = r < P(q = 1);     */

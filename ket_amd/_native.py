@@ -75,6 +75,7 @@ SIGNATURES = [
     ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
     ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
     ("ksv_shots_init", c_int, [_P, c_int, c_uint64]),
+    ("ksv_shots_init_identity", c_int, [_P, c_int]),
     ("ksv_shots_pauli", c_int, [_P, c_int, c_int, _P, _P, c_size_t]),
     ("ksv_shots_measure", c_int, [_P, c_int, c_int, _P, _P, _P]),
     ("ksv_shots_sample", c_int, [_P, c_int, _P, _P]),
@@ -304,6 +305,9 @@ class State:
     # batched trajectories: copies of an n-qubit state, copy t = amplitudes [t 2^n, (t+1) 2^n)
     def shots_init(self, n, index=0):
         check(lib().ksv_shots_init(self.handle, int(n), int(index)))
+
+    def shots_init_identity(self, n):
+        check(lib().ksv_shots_init_identity(self.handle, int(n)))
 
     def shots_pauli(self, n, q, copies, paulis):
         c = np.ascontiguousarray(copies, dtype=np.uint32)

@@ -677,19 +677,17 @@ class Circuit:
         if n > 12:
             raise RangeError(f"circuitMatrix(): circuit too large ({n} qubits = {dim}×{dim} matrix)")
         rev = np.array([int(format(i, "b").zfill(n)[::-1], 2) for i in range(dim)], dtype=np.int64)
-        tape = self.tape()
-        st = State(n, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
-        prog = st.program(tape)
-        mat = np.zeros((dim, dim), dtype=np.complex128)
-        for col in range(dim):
-            st.init_basis(int(rev[col]))
-            prog.run()
-            a = st.amplitudes()
-            p = a.real ** 2 + a.imag ** 2
-            a = np.where(p >= 1e-14, a, 0)
-            mat[rev, col] = a
-        prog.free()
-        st.free()
+        # all 2^n columns at once: a 2n-qubit state whose copy t starts as |t>
+        st = State(2 * n, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+        try:
+            st.shots_init_identity(n)
+            st.apply(self.tape())
+            a = st.amplitudes().reshape(dim, dim)           # a[t, i] = <i| U |t>
+        finally:
+            st.free()
+        p = a.real ** 2 + a.imag ** 2
+        a = np.where(p >= 1e-14, a, 0)
+        mat = a.T[np.ix_(rev, rev)]                         # q0 = MSB on both indices
         return [[complex(v) for v in row] for row in mat]
 
     def amplitude(self, bitstring):

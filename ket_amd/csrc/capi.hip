@@ -822,6 +822,16 @@ int ksv_shots_init(ksv_state* s, int n, uint64_t index) {
     return KSV_OK;
 }
 
+int ksv_shots_init_identity(ksv_state* s, int n) {
+    uint32_t copies = 0;
+    int rc = shots_check(s, n, &copies);
+    if (rc) return rc;
+    if (s->n != 2 * n) return fail(KSV_ERR_RANGE, "an identity batch of " + std::to_string(n) + "-qubit copies needs a " + std::to_string(2 * n) + "-qubit state");
+    HIPCHK(shots_identity(s->n, n, s->prec, s->psi, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
 int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy, const uint8_t* pauli, size_t count) {
     uint32_t copies = 0;
     int rc = shots_check(s, n, &copies);

@@ -44,6 +44,19 @@ __global__ void k_shots_basis(V* psi, uint64_t total, int n, uint64_t idx) {
     }
 }
 
+// copy t := |t> (2^n copies): applying a circuit then leaves column t of
+// its matrix in copy t
+template <typename V>
+__global__ void k_shots_identity(V* psi, uint64_t total, int n) {
+    const uint64_t mask = (1ull << n) - 1;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        V v;
+        v.x = (i & mask) == (i >> n) ? 1 : 0;
+        v.y = 0;
+        psi[i] = v;
+    }
+}
+
 // Pauli `pauli[e]` (1 X, 2 Y, 3 Z) on qubit q of copy `copy[e]`; blockIdx.y = e.
 // The entries are exact sign flips and swaps, as ket's applySingle gives
 // them for these matrices.
@@ -244,6 +257,13 @@ hipError_t shots_basis(int ntot, int n, int prec, void* psi, uint64_t idx, hipSt
     const uint64_t total = 1ull << ntot;
     SDISPATCH(prec, hipLaunchKernelGGL((k_shots_basis<V>), dim3(sgrid(total, 8192)), dim3(256), 0, st, (V*)psi,
                                        total, n, idx));
+    return hipGetLastError();
+}
+
+hipError_t shots_identity(int ntot, int n, int prec, void* psi, hipStream_t st) {
+    const uint64_t total = 1ull << ntot;
+    SDISPATCH(prec, hipLaunchKernelGGL((k_shots_identity<V>), dim3(sgrid(total, 8192)), dim3(256), 0, st, (V*)psi,
+                                       total, n));
     return hipGetLastError();
 }
 

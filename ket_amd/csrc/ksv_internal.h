@@ -269,6 +269,7 @@ hipError_t k_copy(int n, int prec, void* dst, const void* src, hipStream_t st);
 // batched trajectories (k_shots.hip): copies of an n-qubit state inside an
 // ntot-qubit buffer
 hipError_t shots_basis(int ntot, int n, int prec, void* psi, uint64_t idx, hipStream_t st);
+hipError_t shots_identity(int ntot, int n, int prec, void* psi, hipStream_t st);
 hipError_t shots_pauli(int n, int prec, void* psi, int q, const uint32_t* d_copy, const uint8_t* d_pauli,
                        uint32_t count, hipStream_t st);
 hipError_t shots_bit1(int n, int prec, const void* psi, int q, uint32_t copies, int nblk, double* d_part,

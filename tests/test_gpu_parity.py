@@ -334,6 +334,39 @@ def test_batched_shots_entry_points():
     st.free()
 
 
+def test_circuit_matrix_known_answers():
+    """ket's circuitMatrix() cases (src/circuit.test.ts:6425-6511), all 2^n columns
+    from one run on a 2n-qubit state whose copy t starts as |t>."""
+    sq2 = 1 / math.sqrt(2)
+    I = np.array(Circuit(2).circuitMatrix())
+    assert np.allclose(I, np.eye(4), atol=1e-12)
+    assert np.allclose(np.array(Circuit(1).h(0).circuitMatrix()), [[sq2, sq2], [sq2, -sq2]], atol=1e-12)
+    assert np.allclose(np.array(Circuit(1).x(0).circuitMatrix()), [[0, 1], [1, 0]], atol=1e-12)
+    U = np.array(Circuit(2).h(0).cnot(0, 1).circuitMatrix())
+    assert np.allclose(U.conj().T @ U, np.eye(4), atol=1e-12)
+    cnot = [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]]
+    assert np.allclose(np.array(Circuit(2).unitary(cnot, 0, 1).circuitMatrix()), cnot, atol=1e-12)
+    S = np.array(Circuit(1).s(0).circuitMatrix())
+    assert np.allclose(S, [[1, 0], [0, 1j]], atol=1e-12)
+    with pytest.raises(TypeError):
+        Circuit(1).h(0).creg("c", 1).measure(0, "c", 0).circuitMatrix()
+    with pytest.raises(ket_amd.RangeError):
+        Circuit(13).circuitMatrix()
+
+
+@pytest.mark.parametrize("n", [3, 6, 10])
+def test_circuit_matrix_equals_oracle_columns(n):
+    """Every column against the oracle simulating that basis state alone (q0 = MSB indices)."""
+    calls = random_calls(n, 60, seed=40 + n)
+    got = np.array(build(n, calls).circuitMatrix())
+    dim = 1 << n
+    rev = np.array([int(format(i, "b").zfill(n)[::-1], 2) for i in range(dim)])
+    for col in range(0, dim, max(1, dim // 64)):
+        psi = dense_ref(n, calls, init=int(rev[col]))
+        psi = np.where(np.abs(psi) ** 2 >= 1e-14, psi, 0)
+        assert np.max(np.abs(got[rev, col] - psi)) < TOL
+
+
 def test_errors_map_to_ket_classes():
     with pytest.raises(ket_amd.RangeError):
         Circuit(2).h(2)
