@@ -690,6 +690,19 @@ class Circuit:
         mat = a.T[np.ix_(rev, rev)]                         # q0 = MSB on both indices
         return [[complex(v) for v in row] for row in mat]
 
+    def dm(self, noise=None):
+        """src/circuit.ts:4549-4579: exact density matrix with optional per-gate depolarising noise."""
+        from .density import DM_DEVICE_NOISE, run_dm
+        if not self._is_pure():
+            raise TypeError("dm() requires a pure circuit — remove measure/reset/if ops")
+        if isinstance(noise, str):
+            params = DM_DEVICE_NOISE.get(noise)
+            if params is None:
+                raise TypeError(f"Unknown device profile '{noise}'. Known: {', '.join(DM_DEVICE_NOISE)}")
+        else:
+            params = noise
+        return run_dm(self.qubits, _flatten(self._ops()), params, OPTIONS["precision"], OPTIONS["prune"])
+
     def amplitude(self, bitstring):
         if not isinstance(bitstring, str) or len(bitstring) != self.qubits or any(ch not in "01" for ch in bitstring) or not bitstring:
             raise TypeError(f"bitstring '{bitstring}' must be a {self.qubits}-character binary string")

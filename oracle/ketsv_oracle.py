@@ -260,3 +260,85 @@ def run_shots(n, calls, shots, seed, noise=None, init=0, cregs=None, prune=True)
             for j, b in enumerate(v):
                 creg_counts[k][j] += 1 if b else 0
     return counts, creg_counts
+
+
+# ---------------------------------------------------------------------------
+# Density matrix, restated (src/density.ts:475-588 runDM, :193-272 the
+# depolarising channels, :274-380 the read-outs) on a dense 2^n x 2^n rho.
+# Every op ends with dmSet's prune: entries with |v|^2 < 1e-14 become 0.
+# ---------------------------------------------------------------------------
+_SRISW = np.array([[1, 0, 0, 0], [0, 2 ** -0.5, 1j * 2 ** -0.5, 0], [0, 1j * 2 ** -0.5, 2 ** -0.5, 0], [0, 0, 0, 1]])
+# [flip_a, flip_b, z_a, z_b] of the 15 non-identity two-qubit Paulis (src/density.ts:226-233)
+_PAULI15 = [(0, 1, 0, 0), (0, 1, 0, 1), (0, 0, 0, 1), (1, 0, 0, 0), (1, 1, 0, 0), (1, 1, 0, 1), (1, 0, 0, 1),
+            (1, 0, 1, 0), (1, 1, 1, 0), (1, 1, 1, 1), (1, 0, 1, 1), (0, 0, 1, 0), (0, 1, 1, 0), (0, 1, 1, 1),
+            (0, 0, 1, 1)]
+
+
+def _full_operator(n, name, args):
+    """The 2^n x 2^n matrix of one named call, column by column (no pruning)."""
+    dim = 1 << n
+    U = np.zeros((dim, dim), dtype=np.complex128)
+    for j in range(dim):
+        psi = zero_state(n, j)
+        apply_named(n, psi, [(name, args)], prune=False)
+        U[:, j] = psi
+    return U
+
+
+def _dm_prune(rho):
+    return np.where(rho.real ** 2 + rho.imag ** 2 < 1e-14, 0, rho)
+
+
+def _dm_depolarize1(n, rho, q, p):
+    sa, sb, cf = 1 - 2 * p / 3, 2 * p / 3, 1 - 4 * p / 3
+    idx = np.arange(1 << n)
+    b = (idx >> q) & 1
+    same = b[:, None] == b[None, :]
+    flipped = rho[np.ix_(idx ^ (1 << q), idx ^ (1 << q))]
+    return _dm_prune(np.where(same, sa * rho + sb * flipped, cf * rho))
+
+
+def _dm_depolarize2(n, rho, a, b, p):
+    w1, w2 = 1 - p, p / 15
+    idx = np.arange(1 << n)
+    ba, bb = (idx >> a) & 1, (idx >> b) & 1
+    new = w1 * rho
+    for fa, fb, za, zb in _PAULI15:
+        perm = idx ^ ((fa << a) | (fb << b))
+        par = (za * (ba[:, None] ^ ba[None, :])) ^ (zb * (bb[:, None] ^ bb[None, :]))
+        new = new + np.where(par == 1, -w2, w2) * rho[np.ix_(perm, perm)]
+    return _dm_prune(new)
+
+
+def _dm_csrswap(n, rho, c, a, b):
+    """ket's block-wise controlled-sqrt(iSWAP) (src/density.ts:521-570)."""
+    W = _full_operator(n, "unitary", (_SRISW, a, b))
+    ctl = ((np.arange(1 << n) >> c) & 1).astype(bool)
+    B = {(i, j): np.where(np.outer(ctl == bool(i), ctl == bool(j)), rho, 0) for i in (0, 1) for j in (0, 1)}
+    out = B[0, 0].copy()
+    out = out + W @ B[1, 1] @ W.conj().T
+    out = out + W @ B[1, 0] @ W.conj().T
+    out = out + (W @ B[0, 1].T @ W.conj().T).T
+    return _dm_prune(out)
+
+
+def run_dm(n, calls, noise=None):
+    """runDM: rho of the calls from |0..0><0..0| with optional p1 / p2 depolarising noise."""
+    p1 = (noise or {}).get("p1", 0) or 0
+    p2 = (noise or {}).get("p2", 0) or 0
+    rho = np.zeros((1 << n, 1 << n), dtype=np.complex128)
+    rho[0, 0] = 1
+    for name, args in calls:
+        if name == "barrier":
+            continue
+        if name == "csrswap":
+            rho = _dm_csrswap(n, rho, *args)
+            continue
+        U = _full_operator(n, name, args)
+        rho = _dm_prune(U @ rho @ U.conj().T)
+        qs = args[len(args) - NQ[name]:] if name in NQ else ()
+        if name in _SINGLE and p1:
+            rho = _dm_depolarize1(n, rho, qs[0], p1)
+        elif name in _TWO_KIND and p2:
+            rho = _dm_depolarize2(n, rho, qs[0], qs[1], p2)
+    return rho

@@ -367,6 +367,59 @@ def test_circuit_matrix_equals_oracle_columns(n):
         assert np.max(np.abs(got[rev, col] - psi)) < TOL
 
 
+def test_dm_known_answers():
+    """ket's dm() cases (src/circuit.test.ts:5017-5250): rho as a 2n-qubit device state."""
+    sq2 = 1 / math.sqrt(2)
+    for c in (Circuit(1), Circuit(1).x(0), Circuit(1).h(0), Circuit(2).h(0).cnot(0, 1)):
+        assert abs(c.dm().purity() - 1) < 1e-10
+    assert Circuit(1).dm().probabilities() == {"0": 1.0}
+    assert Circuit(1).x(0).dm().probabilities() == {"1": 1.0}
+    p = Circuit(2).h(0).cnot(0, 1).dm().probabilities()
+    assert set(p) == {"00", "11"} and abs(p["00"] - 0.5) < 1e-10
+    c = Circuit(3).h(0).h(1).h(2)
+    ex, pr = c.exactProbs(), c.dm().probabilities()
+    assert all(abs(pr[k] - v) < 1e-10 for k, v in ex.items())
+    for c in (Circuit(1), Circuit(1).h(0), Circuit(2).h(0).cnot(0, 1)):
+        assert abs(c.dm().entropy()) < 1e-10
+    assert abs(Circuit(1).x(0).dm().blochAngles(0)["theta"] - math.pi) < 1e-10
+    b = Circuit(1).h(0).dm().blochAngles(0)
+    assert abs(b["theta"] - math.pi / 2) < 1e-10 and abs(b["phi"]) < 1e-10
+    c = Circuit(1).rx(math.pi / 3, 0)
+    sv, d = c.blochAngles(0), c.dm().blochAngles(0)
+    assert abs(sv["theta"] - d["theta"]) < 1e-8 and abs(sv["phi"] - d["phi"]) < 1e-8
+    assert Circuit(1).dm().get(0, 1) == 0
+    with pytest.raises(TypeError):
+        Circuit(1).creg("c", 1).measure(0, "c", 0).dm()
+    with pytest.raises(TypeError):
+        Circuit(1).h(0).dm(noise="nonexistent-device")
+    pv = [Circuit(1).h(0).dm(noise={"p1": x}).purity() for x in (0.01, 0.1, 0.3)]
+    assert pv[0] > pv[1] > pv[2]
+    assert abs(Circuit(1).h(0).dm(noise={"p1": 0.1}).get(0, 1).real - (1 - 0.4 / 3) * 0.5) < 1e-8
+    c = Circuit(2).h(0).cnot(0, 1)
+    assert c.dm(noise="forte-1").purity() > c.dm(noise="harmony").purity()
+    assert c.dm(noise={"p2": 0.05}).purity() < 1
+    assert abs(Circuit(1).h(0).dm(noise={"p1": 0.75}).entropy() - 1) < 1e-6
+    assert 0 < Circuit(1).h(0).dm(noise={"p1": 0.2}).entropy() < 1
+    assert abs(sum(c.dm(noise={"p1": 0.05, "p2": 0.05}).probabilities().values()) - 1) < 1e-8
+    assert abs(Circuit(3).x(0).x(1).ccx(0, 1, 2).dm().probabilities()["111"] - 1) < 1e-10
+    assert abs(Circuit(3).x(0).x(2).cswap(0, 1, 2).dm().probabilities()["110"] - 1) < 1e-10
+
+
+@pytest.mark.parametrize("n,noise,seed", [(3, None, 1), (3, {"p1": 0.05, "p2": 0.1}, 2), (4, "harmony", 3),
+                                          (5, {"p1": 0.02, "p2": 0.03}, 4)])
+def test_dm_equals_oracle(n, noise, seed):
+    """rho of seeded circuits over every op kind (csrswap, 2- and 4-qubit unitaries,
+    toffoli, cswap) with depolarising noise, against the oracle's runDM."""
+    from ket_amd.density import DM_DEVICE_NOISE
+    calls = random_calls(n, 30, seed=seed, with_unitary=True)
+    calls.append(("csrswap", (0, 1, 2)))
+    d = build(n, calls).dm(noise=noise)
+    params = DM_DEVICE_NOISE[noise] if isinstance(noise, str) else noise
+    ref = O.run_dm(n, calls, params)
+    assert np.max(np.abs(d._rho - ref)) < TOL
+    assert abs(d.purity() - float(np.sum(np.abs(ref) ** 2))) < 1e-12
+
+
 def test_errors_map_to_ket_classes():
     with pytest.raises(ket_amd.RangeError):
         Circuit(2).h(2)

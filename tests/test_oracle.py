@@ -365,3 +365,54 @@ def test_shots_noise_free_matches_born_rule():
     obs = [counts.get(i, 0) for i in range(16)]
     keep = p * 4000 >= 5
     assert _chi2(np.array(obs)[keep], (p * 4000)[keep]) < 40
+
+
+# ---- density matrix (runDM) --------------------------------------------------
+
+def test_dm_pure_circuits_are_projectors():
+    """Without noise rho = |psi><psi| of the statevector oracle (prune after every op aside)."""
+    for seed in range(3):
+        calls = [c for c in random_calls(3, 25, seed=seed, with_unitary=True) if c[0] != "csrswap"]
+        psi = O.zero_state(3)
+        O.apply_named(3, psi, calls, prune=False)
+        rho = O.run_dm(3, calls)
+        assert np.max(np.abs(rho - np.outer(psi, psi.conj()))) < 1e-12
+
+
+def test_dm_depolarize1_closed_forms():
+    """src/circuit.test.ts:5164-5170, :5199-5203: off-diagonal (1 - 4p/3)/2; p = 3/4 maximally mixed."""
+    p = 0.1
+    rho = O.run_dm(1, [("h", (0,))], {"p1": p})
+    assert abs(rho[0, 1] - (1 - 4 * p / 3) * 0.5) < 1e-15
+    assert abs(rho[0, 0] - 0.5) < 1e-15
+    rho = O.run_dm(1, [("h", (0,))], {"p1": 0.75})
+    assert np.max(np.abs(rho - 0.5 * np.eye(2))) < 1e-15
+
+
+def test_dm_depolarize2_is_the_pauli_twirl():
+    """(1 - p) rho + p/15 sum_{P != II} P rho P^dagger, built from Pauli matrices directly."""
+    X = np.array([[0, 1], [1, 0]]); Y = np.array([[0, -1j], [1j, 0]]); Z = np.diag([1, -1]); I = np.eye(2)
+    p = 0.2
+    pure = O.run_dm(2, [("h", (0,)), ("cnot", (0, 1))])
+    got = O.run_dm(2, [("h", (0,)), ("cnot", (0, 1))], {"p2": p})
+    want = (1 - p) * pure
+    for A in (I, X, Y, Z):
+        for B in (I, X, Y, Z):
+            if A is I and B is I:
+                continue
+            P = np.kron(B, A)          # index bit 0 = qubit 0 (control), bit 1 = qubit 1
+            want = want + p / 15 * P @ pure @ P.conj().T
+    assert np.max(np.abs(got - want)) < 1e-14
+    assert abs(np.trace(got) - 1) < 1e-14
+
+
+def test_dm_csrswap_with_definite_control_is_unitary():
+    """Control definitely 1 (or 0): ket's block form reduces to V rho V^dagger (or identity)."""
+    calls = [("x", (0,)), ("h", (1,)), ("ry", (0.7, 2)), ("csrswap", (0, 1, 2))]
+    psi = O.zero_state(3)
+    O.apply_named(3, psi, calls, prune=False)
+    assert np.max(np.abs(O.run_dm(3, calls) - np.outer(psi, psi.conj()))) < 1e-12
+    calls[0] = ("id", (0,))
+    psi = O.zero_state(3)
+    O.apply_named(3, psi, calls, prune=False)
+    assert np.max(np.abs(O.run_dm(3, calls) - np.outer(psi, psi.conj()))) < 1e-12
