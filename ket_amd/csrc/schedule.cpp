@@ -859,6 +859,97 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
     }
 }
 
+// Front-driven fusion.  Repeatedly take the first op not yet placed and grow
+// its qubit set to bq qubits, each added qubit chosen to maximise how many of
+// the next `look` ops the block could absorb; then absorb every op that acts
+// inside the set and is not preceded, on one of its qubits, by an op left
+// out (so moving the block ahead of everything still unplaced keeps every
+// dependence).  On brick-wall circuits this finds the staggered windows that
+// op-order greedy fusion misses (157 instead of 220 blocks for 30 qubits x 40
+// layers).  Blocks come out in a valid execution order.
+static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::vector<FBlock>& blocks) {
+    const int look = 400;
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    std::vector<int> rem(lops.size());
+    for (size_t i = 0; i < lops.size(); i++) rem[i] = (int)i;
+    std::vector<char> done(lops.size(), 0);
+    size_t head = 0;
+    auto fits = [&](const LOp& o) { return !o.big && popc(o.touch) <= bq; };
+    auto scan = [&](uint64_t Q, int limit, std::vector<int>* out) {
+        uint64_t blocked = 0;
+        int cnt = 0, seen = 0;
+        for (size_t j = head; j < rem.size(); j++) {
+            const int i = rem[j];
+            if (done[i]) continue;
+            if (limit >= 0 && seen++ >= limit) break;
+            const LOp& o = lops[i];
+            if (fits(o) && (o.touch & ~Q) == 0 && (o.touch & blocked) == 0) {
+                cnt++;
+                if (out) out->push_back(i);
+            } else {
+                blocked |= o.touch & Q;
+                if (blocked == Q) break;
+            }
+        }
+        return cnt;
+    };
+    for (;;) {
+        while (head < rem.size() && done[rem[head]]) head++;
+        if (head == rem.size()) break;
+        if (head > 4096 && head * 2 > rem.size()) {       // drop the placed prefix
+            rem.erase(rem.begin(), rem.begin() + (long)head);
+            head = 0;
+        }
+        const int first = rem[head];
+        const LOp& f0 = lops[first];
+        if (!fits(f0)) {
+            FBlock f;
+            f.lop = first;
+            for (int q = 0; q < n; q++)
+                if (f0.touch >> q & 1) f.q.push_back(q);
+            f.touch = f0.touch;
+            blocks.push_back(f);
+            done[first] = 1;
+            continue;
+        }
+        uint64_t Q = f0.touch;
+        while (popc(Q) < bq && popc(Q) < n) {
+            uint64_t cand = 0;
+            int seen = 0;
+            for (size_t j = head; j < rem.size() && seen < look; j++) {
+                const int i = rem[j];
+                if (done[i]) continue;
+                seen++;
+                if (lops[i].touch & Q) cand |= lops[i].touch;
+            }
+            cand &= all & ~Q;
+            if (!cand) cand = all & ~Q;
+            int bestq = -1, bestc = -1;
+            for (int q = 0; q < n; q++) {
+                if (!(cand >> q & 1)) continue;
+                const int c = scan(Q | (1ull << q), look, nullptr);
+                if (c > bestc) { bestc = c; bestq = q; }
+            }
+            Q |= 1ull << bestq;
+        }
+        std::vector<int> got;
+        scan(Q, -1, &got);
+        FBlock f;
+        f.u.assign(1, cd(1, 0));
+        for (int i : got) {
+            const LOp& o = lops[i];
+            std::vector<int> add;
+            for (int q = 0; q < n; q++)
+                if ((o.touch >> q & 1) && !(f.touch >> q & 1)) add.push_back(q);
+            grow_block(f, add);
+            absorb(f, o);
+            f.ops.push_back(i);
+            done[i] = 1;
+        }
+        blocks.push_back(f);
+    }
+}
+
 void block_fragments(const cd* u, int bq, double* out) {
     const int R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
     auto ue = [&](int i, int k) -> double {     // real embedding [[Ur, -Ui], [Ui, Ur]]
@@ -973,8 +1064,11 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
         schedule_rounds(n, prog.precision == KSV_F64 ? 11 : 12, false, lops, prog);
         return;
     }
-    std::vector<FBlock> blocks;
+    // two packings; the one with fewer blocks does less MFMA work
+    std::vector<FBlock> blocks, front;
     fuse_blocks(n, bq, lops, blocks);
+    fuse_blocks_front(n, bq, lops, front);
+    if (front.size() < blocks.size()) blocks.swap(front);
     // pack blocks into passes (a standalone op is a step of its own)
     std::vector<LOp> proxies(blocks.size());
     for (size_t i = 0; i < blocks.size(); i++) {
