@@ -861,14 +861,13 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
 
 // Front-driven fusion.  Repeatedly take the first op not yet placed and grow
 // its qubit set to bq qubits, each added qubit chosen to maximise how many of
-// the next `look` ops the block could absorb; then absorb every op that acts
+// the next `look` unplaced ops the block could absorb; then absorb every op that acts
 // inside the set and is not preceded, on one of its qubits, by an op left
 // out (so moving the block ahead of everything still unplaced keeps every
 // dependence).  On brick-wall circuits this finds the staggered windows that
-// op-order greedy fusion misses (157 instead of 220 blocks for 30 qubits x 40
-// layers).  Blocks come out in a valid execution order.
-static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::vector<FBlock>& blocks) {
-    const int look = 400;
+// op-order greedy fusion misses (148 instead of 220 blocks for 30 qubits x 40
+// layers, best lookahead).  Blocks come out in a valid execution order.
+static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::vector<FBlock>& blocks, int look) {
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     std::vector<int> rem(lops.size());
     for (size_t i = 0; i < lops.size(); i++) rem[i] = (int)i;
@@ -1064,11 +1063,15 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
         schedule_rounds(n, prog.precision == KSV_F64 ? 11 : 12, false, lops, prog);
         return;
     }
-    // two packings; the one with fewer blocks does less MFMA work
-    std::vector<FBlock> blocks, front;
+    // several packings; the one with fewest blocks does the least MFMA work
+    // (the best lookahead depends on the circuit's width and layering)
+    std::vector<FBlock> blocks;
     fuse_blocks(n, bq, lops, blocks);
-    fuse_blocks_front(n, bq, lops, front);
-    if (front.size() < blocks.size()) blocks.swap(front);
+    for (int look : {48, 64, 96, 400}) {
+        std::vector<FBlock> front;
+        fuse_blocks_front(n, bq, lops, front, look);
+        if (front.size() < blocks.size()) blocks.swap(front);
+    }
     // pack blocks into passes (a standalone op is a step of its own)
     std::vector<LOp> proxies(blocks.size());
     for (size_t i = 0; i < blocks.size(); i++) {
