@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6     # MI355X dense FP64 matrix peak
+FP32_MFMA_PEAK_TFS = 157.3    # MI355X dense FP32 matrix peak
 METRIC = "gate-updates/s & % HBM peak at 30q FP64; adjoint vs param-shift grads/s"
 
 
@@ -88,44 +89,65 @@ def compare_states(a, b, n, chunk_log=26):
 
 def workload_f32(args):
     """Complex64 state at 32 qubits (32 GiB), the 40-layer random circuit fused
-    into 5-qubit blocks on FP32 MFMA; accuracy against FP64 at 30 qubits."""
+    into dense blocks on FP32 MFMA (the library default: 4-qubit blocks on
+    12-qubit tiles; the 5-qubit-block schedule is timed beside it); accuracy
+    against FP64 at 30 qubits."""
     from ket_amd import workloads
     from ket_amd._native import State
     n = args.qubits if args.qubits != 30 else 32
     circ = workloads.random_layers(n, args.layers, seed=42)
     ops, mats = circ.tape()
-    st = State(n, "f32", prune=False, kernel="mfma", bq=args.bq or 5, btile=args.btile or 13)
-    prog = st.program((ops, mats))
-    info = prog.info()
-    log(f"bench f32: {n} qubits, {len(ops)} gates -> {info['blocks']} blocks in {info['passes']} passes")
-    dt, stats = run_program_steps(st, prog, args.steps, args.warmup)
-    norm = st.norm2()
-    prog.free()
-    st.free()
+    gates = len(ops)
+
+    def timed(bq, btile):
+        st = State(n, "f32", prune=False, kernel="mfma", bq=bq, btile=btile)
+        prog = st.program((ops, mats))
+        info = prog.info()
+        log(f"bench f32: {n} qubits, {gates} gates -> {info['blocks']} {bq}-qubit blocks in {info['passes']} passes")
+        dt, stats = run_program_steps(st, prog, args.steps, args.warmup)
+        norm = st.norm2()
+        prog.free()
+        st.free()
+        return dt, stats, info, norm
+
+    bq, bt = args.bq or 4, args.btile or 12
+    dt, stats, info, norm = timed(bq, bt)
+    five = None
+    if not (args.bq or args.btile):
+        dt5, _, info5, _ = timed(5, 13)
+        five = {"gate_updates_per_s": round(gates * args.steps / dt5, 2), "blocks": info5["blocks"],
+                "passes": info5["passes"], "ms_per_step": round(dt5 / args.steps * 1e3, 3)}
     launches = stats["pass_launches"]
     avg_ms = stats["pass_ms"] / launches
-    achieved = stats["pass_bytes"] / launches / (avg_ms * 1e-3) / 1e9
+    per_launch = stats["pass_bytes"] / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    flops = info["blocks"] * 8 * (1 << bq) * (1 << n) / info["passes"]
+    tflops = flops / (avg_ms * 1e-3) / 1e12
+    hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    mfma = {"achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+            "frac": round(tflops / FP32_MFMA_PEAK_TFS, 4)}
+    mfma_bound = flops / (FP32_MFMA_PEAK_TFS * 1e12) > per_launch / (HBM_PEAK_GBS * 1e9)
+    main = dict(mfma if mfma_bound else hbm)
+    roofline = {"bound": "mfma" if mfma_bound else "hbm", **main, "traffic": None, "kernel": "ksv::k_bpass",
+                "avg_launch_ms": round(avg_ms, 4), "hbm": hbm, "mfma": mfma}
     # accuracy at 30 qubits: the same seeded circuit in FP64 and in FP32
     na = 30
     ca = workloads.random_layers(na, args.layers, seed=42).tape()
     s64 = State(na, "f64", prune=False)
     s64.apply(ca)
-    s32 = State(na, "f32", prune=False, kernel="mfma", bq=args.bq or 5, btile=args.btile or 13)
+    s32 = State(na, "f32", prune=False, kernel="mfma", bq=bq, btile=bt)
     s32.apply(ca)
     maxd, tvd = compare_states(s64, s32, na)
     s64.free()
     s32.free()
-    gates = len(ops)
-    return {"metric": "gate-updates/s & % HBM peak, complex64 statevector with FP32 MFMA blocks",
+    return {"metric": "gate-updates/s & roofline, complex64 statevector with FP32 MFMA blocks",
             "value": round(gates * args.steps / dt, 2), "unit": "gate-updates/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"random Rx/Ry/Rz + CNOT-brick circuit, {n} qubits, {args.layers} layers, "
-                                   f"complex64, fused {args.bq or 5}-qubit blocks",
+                                   f"complex64, fused {bq}-qubit blocks on {bt}-qubit tiles",
                        "qubits": n, "gates": gates, "blocks": info["blocks"], "passes": info["passes"]},
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "ksv::k_bpass",
-                         "avg_launch_ms": round(avg_ms, 4)},
+            "roofline": roofline, "five_qubit_blocks": five,
             "norm2_fp64_accumulated": norm,
             "accuracy_vs_fp64_30q": {"max_abs_amp_err": maxd, "tvd": tvd}}
 
@@ -140,7 +162,14 @@ def workload_qft(args):
     n = args.qubits
     c, bs, x = workloads.qft_input(n, seed=7)
     ops, mats = c.tape()
-    st = State(n, "f64", prune=False)
+    tuned = None
+    if not args.no_autotune:
+        from ket_amd.autotune import autotune, state_for
+        best, cands = autotune(n, (ops, mats), precision="f64", prune=False, reps=2, verbose=log)
+        tuned = {"chosen": best, "candidates_ms": [[c_, round(ms, 2)] for c_, ms in cands]}
+        st = state_for(n, best, "f64", prune=False)
+    else:
+        st = State(n, "f64", prune=False)
     prog = st.program((ops, mats))
     info = prog.info()
     for _ in range(args.warmup):
@@ -174,7 +203,8 @@ def workload_qft(args):
             "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"qft({n}) on |x>, x = {x}", "qubits": n, "gates": len(ops),
-                       "passes": info["passes"], "blocks": info["blocks"]},
+                       "passes": info["passes"], "blocks": info["blocks"], "reg_passes": info["reg_passes"],
+                       "autotune": tuned},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(ms, 4)},
             "checks": {"max_abs_amp_err_256_samples": err, "top1024_max_rel_p_err": prel, "norm2": norm}}

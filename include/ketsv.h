@@ -95,7 +95,7 @@ typedef struct ksv_program ksv_program;
 #define KSV_OPT_ROUND_TILE   5  /* tile qubits of register-round passes: 0 = default (11 FP64,
                                    12 FP32) or 11..13; 2^(tile-4) threads x 16 amplitudes     */
 #define KSV_OPT_BLOCK_QUBITS 6  /* fused-block width: 0 = default (4) or 4, 5                  */
-#define KSV_OPT_BLOCK_TILE   7  /* fused-block tile qubits: 0 = default (12 FP64, 13 FP32) or
+#define KSV_OPT_BLOCK_TILE   7  /* fused-block tile qubits: 0 = default (12) or
                                    11..13 (5-qubit blocks need 13, 11 takes 4-qubit blocks)   */
 #define KSV_OPT_BLOCK_THREADS 8 /* fused-block workgroup size: 0 = default (256) or 256, 512
                                    (512 only with 4-qubit blocks on 12-qubit tiles)            */

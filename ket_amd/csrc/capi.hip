@@ -196,7 +196,7 @@ int ksv_copy_state(ksv_state* dst, const ksv_state* src) {
 static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                      int kernel, int tile, int rtile, bool fuse, int prune, int btile = 0, int bq = 0) {
     std::string err;
-    int bqv = bq > 0 ? bq : 4, btv = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
+    int bqv = bq > 0 ? bq : 4, btv = btile > 0 ? btile : 12;
     if (kernel == 3 && bqv == 5 && btv != 13) return fail(KSV_ERR_RANGE, "5-qubit blocks need 13-qubit tiles");
     if (kernel == 3 && btv == 11 && bqv != 4) return fail(KSV_ERR_RANGE, "11-qubit tiles take 4-qubit blocks");
     int rc = plan_program(p, n, prec, ops, nops, mats, nmats, kernel, tile, rtile, fuse, prune, err, btile, bq);

@@ -1121,7 +1121,7 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
     p.mat_of_lop.assign(lops.size(), -1);
     const int rt = rtile > 0 ? rtile : (prec == KSV_F64 ? 11 : 12);
     // auto: fused MFMA blocks in exact mode, ordered register rounds under ket's pruning
-    const int bt = btile > 0 ? btile : (prec == KSV_F64 ? 12 : 13);
+    const int bt = btile > 0 ? btile : 12;
     if (kernel == 0) kernel = prune ? 2 : 3;
     if (kernel == 1) schedule(n, p.tile, fuse && !prune, lops, p);
     else if (kernel == 3) schedule_blocks(n, bt, bq > 0 ? bq : 4, lops, p);
