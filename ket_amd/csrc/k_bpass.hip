@@ -14,7 +14,9 @@
 // 2^(BQ+1)/16 output tiles over 2^(BQ+1)/4 k-steps, and writes the result
 // back in place (lane l, element v: row 4v + l/16 for f64, 4(l/16) + v
 // for f32).  A fragments come pre-arranged
-// per lane from the host.  Columns differ between waves, so blocks need a
+// per lane from the host.  A block whose ops are all diagonal is instead one
+// phase per row: its fragments hold the diagonal and each lane multiplies
+// the amplitudes it reads in place.  Columns differ between waves, so blocks need a
 // barrier only between them.  The LDS address of tile index x is x with its
 // low four bits XORed by a per-pass word of its higher bits, chosen so that
 // each 16-lane group hits 16 distinct 16-byte bank groups.  HBM sees one
@@ -120,6 +122,7 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
             }
         }
         const int64_t foff = (int64_t)w[6];
+        const bool diag = (w[7] & BREC_DIAG) != 0;     // a phase per row (fragments hold the diagonal)
         T af[MB * KS];
 #pragma unroll
         for (int f = 0; f < MB * KS; f++) af[f] = frags[foff + f * 64 + lane];
@@ -140,13 +143,25 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
             for (int b = 0; b < K - BQ - 4; b++)
                 if ((nb >> b) & 1) col ^= ac[4 + b];
             V x[R / 4];
+            uint32_t ra[R / 4];
 #pragma unroll
             for (int t = 0; t < R / 4; t++) {
-                uint32_t ra = rd_l;
+                ra[t] = rd_l;
 #pragma unroll
                 for (int b = 0; b < BQ - 2; b++)
-                    if ((t >> b) & 1) ra ^= ar[2 + b];
-                x[t] = lds[ra ^ col];
+                    if ((t >> b) & 1) ra[t] ^= ar[2 + b];
+                x[t] = lds[ra[t] ^ col];
+            }
+            if (diag) {
+#pragma unroll
+                for (int t = 0; t < R / 4; t++) {
+                    const T pr = af[2 * t], pi = af[2 * t + 1];
+                    V y;
+                    y.x = x[t].x * pr - x[t].y * pi;
+                    y.y = x[t].x * pi + x[t].y * pr;
+                    lds[ra[t] ^ col] = y;
+                }
+                continue;
             }
             acc_t d[MB];
 #pragma unroll

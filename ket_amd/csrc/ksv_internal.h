@@ -104,8 +104,10 @@ struct BRec {
     uint16_t ar[8];    // LDS address (swizzled) contribution of row bit i, i < BQ
     uint16_t ac[16];   // ... of column bit i, i < K - BQ
     int64_t frag;      // offset (elements) of the block's A fragments: [mb][kk][lane]
-    int64_t pad;
+    int64_t flags;     // BREC_DIAG: every absorbed op is diagonal; the fragments hold the
+                       // diagonal and the block is a phase per row instead of a GEMM
 };
+constexpr int64_t BREC_DIAG = 1;
 static_assert(sizeof(BRec) == 64, "BRec layout");
 
 struct BPassArgs {
@@ -231,7 +233,7 @@ void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Progra
 void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& prog);
 // A fragments of one block for MFMA 16x16x4: [mb][kk][lane] real values of
 // the real embedding of the 2^bq x 2^bq matrix u (row-major).
-void block_fragments(const cd* u, int bq, double* out);
+void block_fragments(const cd* u, int bq, bool diag, double* out);
 // Lower and schedule a tape into `p` (host side only).  kernel: KSV_OPT_PASS_KERNEL
 // value; tile / rtile / btile: LDS, register-round and fused-block tile widths;
 // bq: fused-block width (0 = defaults).

@@ -726,6 +726,7 @@ struct FBlock {
     int lop = -1;             // >= 0: a standalone op too wide for a block
     uint64_t touch = 0;
     std::vector<int> ops;     // lowered ops absorbed, in application order
+    bool diag = true;         // every absorbed op is diagonal (by kind, whatever its angles)
 };
 
 // Apply a logical op to a small dense vector over qubits Q (bit i = Q[i]).
@@ -855,6 +856,7 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
         grow_block(f, add);
         absorb(f, o);
         f.ops.push_back((int)i);
+        f.diag = f.diag && o.cls == L_DIAG;
         for (int q : S) last[q] = best;
     }
 }
@@ -943,14 +945,27 @@ static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::
             grow_block(f, add);
             absorb(f, o);
             f.ops.push_back(i);
+            f.diag = f.diag && o.cls == L_DIAG;
             done[i] = 1;
         }
         blocks.push_back(f);
     }
 }
 
-void block_fragments(const cd* u, int bq, double* out) {
+void block_fragments(const cd* u, int bq, bool diag, double* out) {
     const int R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
+    if (diag) {
+        // fragment 2t / 2t + 1, lane l: Re / Im of U[r][r], r = 4t + l / 16 (the rows a
+        // lane reads in the kernel); the remaining fragments are unused
+        std::fill(out, out + (size_t)MB * KS * 64, 0.0);
+        for (int t = 0; t < R / 4; t++)
+            for (int l = 0; l < 64; l++) {
+                const int r = 4 * t + l / 16;
+                out[(size_t)(2 * t) * 64 + l] = u[(size_t)r * R + r].real();
+                out[(size_t)(2 * t + 1) * 64 + l] = u[(size_t)r * R + r].imag();
+            }
+        return;
+    }
     auto ue = [&](int i, int k) -> double {     // real embedding [[Ur, -Ui], [Ui, Ur]]
         const int r = i % R, c = k % R;
         const cd v = u[(size_t)r * R + c];
@@ -1044,6 +1059,7 @@ static void block_pass(int n, int K, int bq, uint64_t T, std::vector<FBlock>& bl
         for (int i = 0; i < bq; i++) r.ar[i] = (uint16_t)swz_addr(st.swz, 1u << rowpos[b][i]);
         for (int i = 0; i < K - bq; i++) r.ac[i] = (uint16_t)swz_addr(st.swz, 1u << colpos[b][i]);
         r.frag = (int64_t)prog.brecs.size() * nfrag;
+        r.flags = f.diag ? BREC_DIAG : 0;
         prog.bmat_off.push_back((int64_t)prog.bmats.size());
         prog.bmats.insert(prog.bmats.end(), f.u.begin(), f.u.end());
         for (int id : f.ops)

@@ -321,7 +321,8 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
     const size_t nf = (size_t)MB * KS * 64;
     for (int i = 0; i < s.op_count; i++) {
         fr[i].resize(nf);
-        block_fragments(p.bmats.data() + p.bmat_off[s.op_begin + i], bq, fr[i].data());
+        block_fragments(p.bmats.data() + p.bmat_off[s.op_begin + i], bq,
+                        (p.brecs[s.op_begin + i].flags & BREC_DIAG) != 0, fr[i].data());
     }
     for (uint64_t b = 0; b < (N >> K); b++) {
         uint64_t base = b;
@@ -345,6 +346,38 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
             std::fill(rd.begin(), rd.end(), 0);
             std::fill(wr.begin(), wr.end(), 0);
             std::vector<cd> out(lds);
+            if (r.flags & BREC_DIAG) {
+                // phase per row: lane l of wave w, slice m reads rows 4t + l/16 of column
+                // (l % 16, slice bits) and writes them back times the row's phase
+                for (int wv = 0; wv < NW; wv++)
+                    for (int m = 0; m < NBW; m++) {
+                        const uint32_t nb = (uint32_t)wv + (uint32_t)NW * (uint32_t)m;
+                        for (int l = 0; l < 64; l++) {
+                            const uint32_t rl = l >> 4, cl = l & 15;
+                            uint32_t col = 0;
+                            for (int bb = 0; bb < 4; bb++)
+                                if ((cl >> bb) & 1) col ^= r.ac[bb];
+                            for (int bb = 0; bb < K - bq - 4; bb++)
+                                if ((nb >> bb) & 1) col ^= r.ac[4 + bb];
+                            for (int t = 0; t < R / 4; t++) {
+                                const uint32_t row = 4u * (uint32_t)t + rl;
+                                uint32_t ra = 0;
+                                for (int bb = 0; bb < bq; bb++)
+                                    if ((row >> bb) & 1) ra ^= r.ar[bb];
+                                const uint32_t a = ra ^ col;
+                                CHECK(a < (1u << K), "diagonal block address out of range");
+                                rd[a]++;
+                                wr[a]++;
+                                const cd ph(af[(size_t)(2 * t) * 64 + l], af[(size_t)(2 * t + 1) * 64 + l]);
+                                out[a] = lds[a] * ph;
+                            }
+                        }
+                    }
+                for (uint32_t a = 0; a < (1u << K); a++)
+                    CHECK(rd[a] == 1 && wr[a] == 1, "diagonal block LDS slot not read and written exactly once");
+                lds.swap(out);
+                continue;
+            }
             for (int wv = 0; wv < NW; wv++)
                 for (int m = 0; m < NBW; m++) {
                     const uint32_t nb = (uint32_t)wv + (uint32_t)NW * (uint32_t)m;

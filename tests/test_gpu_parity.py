@@ -420,6 +420,21 @@ def test_dm_equals_oracle(n, noise, seed):
     assert abs(d.purity() - float(np.sum(np.abs(ref) ** 2))) < 1e-12
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", TOL), ("f32", 1e-6)])
+def test_qft_diagonal_blocks(precision, tol):
+    """QFT on 16 qubits: blocks of controlled phases run as a phase per row;
+    against the oracle and against set_matrix re-parameterising a phase."""
+    n = 16
+    calls = [("x", (q,)) for q in (0, 3, 7, 12)]
+    for j in range(n):
+        calls.append(("h", (j,)))
+        for k in range(j + 1, n):
+            calls.append(("cu1", (math.pi / (1 << (k - j)), k, j)))
+    ref = dense_ref(n, calls, prune=False)
+    got = gpu_state(n, calls, prune=False, kernel="mfma", btile=12, bq=4, precision=precision)
+    assert np.max(np.abs(got - ref)) < tol
+
+
 def test_errors_map_to_ket_classes():
     with pytest.raises(ket_amd.RangeError):
         Circuit(2).h(2)

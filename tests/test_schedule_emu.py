@@ -3,6 +3,7 @@ oracle: register-round passes (LDS transposes, slot / thread / global
 controls), LDS passes and wide unitaries, with every HBM index and LDS
 address checked.  Runs without a GPU."""
 import ctypes
+import math
 import os
 import subprocess
 
@@ -151,6 +152,21 @@ def test_fused_block_passes_every_op(n, btile, bq, nt):
     finally:
         emu().emu_set_block_threads(256)
     assert stats[4] > 0 and stats[5] > 0
+    assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("n,bq,btile", [(13, 4, 12), (14, 5, 13)])
+def test_fused_diagonal_blocks(n, bq, btile):
+    """QFT: most blocks hold only controlled phases and run as a phase per row."""
+    calls = [("x", (q,)) for q in range(0, n, 3)] + [("h", (0,)), ("ry", (0.4, n - 1))]
+    for j in range(n):
+        calls.append(("h", (j,)))
+        for k in range(j + 1, n):
+            calls.append(("cu1", (math.pi / (1 << (k - j)), k, j)))
+    calls += [("rz", (0.3, 2)), ("cz", (1, 5)), ("crz", (0.7, 3, 6)), ("s", (4,))]
+    ref = dense_ref(n, calls, prune=False)
+    got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    assert stats[4] > 0
     assert np.max(np.abs(got - ref)) < TOL
 
 
