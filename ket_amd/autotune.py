@@ -2,9 +2,9 @@
 a circuit by timing the candidates on the device.
 
 Candidates (exact mode): fused MFMA blocks of 4 qubits on 11-, 12- and
-13-qubit tiles (12-qubit tiles also with 512-thread workgroups), of 5
-qubits on 13-qubit tiles, and register-round passes on 11-, 12- and
-13-qubit tiles.  Under ket's per-gate pruning only register-round
+13-qubit tiles, of 5 qubits on 13-qubit tiles, and register-round passes
+on 11-, 12- and 13-qubit tiles.  (512-thread workgroups,
+KSV_OPT_BLOCK_THREADS, time within noise of 256 and are left out.)  Under ket's per-gate pruning only register-round
 passes keep ket's gate order, so only their tile width is tuned.  Every
 candidate is first scheduled without a device (ksv_plan_blocks) so invalid
 or degenerate ones are skipped; each valid one is then built once and run
@@ -23,7 +23,7 @@ _CACHE = {}
 def _candidates(prune):
     if prune:
         return [{"kernel": "reg", "rtile": r} for r in (11, 12, 13)]
-    return ([{"kernel": "mfma", "bq": 4, "btile": 12}, {"kernel": "mfma", "bq": 4, "btile": 12, "bthreads": 512},
+    return ([{"kernel": "mfma", "bq": 4, "btile": 12},
              {"kernel": "mfma", "bq": 4, "btile": 11}, {"kernel": "mfma", "bq": 4, "btile": 13},
              {"kernel": "mfma", "bq": 5, "btile": 13}] +
             [{"kernel": "reg", "rtile": r} for r in (11, 12, 13)])
