@@ -188,3 +188,97 @@ def test_prng_fill_continues_the_stream():
     assert filled.tolist() == [a() for _ in range(1000)]
     assert a() == b()
     assert make_prng(0).fill(3).tolist() == make_prng(1).fill(3).tolist()     # seed 0 -> 1
+
+
+@pytest.mark.parametrize("noise,seed", [(None, 1), ({"p1": 0.05, "p2": 0.1}, 2), ({"p1": 0.02, "p2": 0.03}, 3)])
+def test_dm_tape_on_the_oracle(noise, seed):
+    """The 2n-qubit tape dm() sends to the device (U (x) conj(U) per gate on row and
+    column qubits, depolarising maps, ket's csrswap block form), run through the
+    oracle's tape engine, equals the oracle's runDM restatement."""
+    import ketsv_oracle as O
+    from helpers import build, random_calls
+    from ket_amd.circuit import _flatten
+    from ket_amd.density import dm_ops
+    from ket_amd.tape import pack
+    n = 3
+    calls = random_calls(n, 25, seed=seed, with_unitary=True) + [("csrswap", (0, 1, 2))]
+    c = build(n, calls)
+    p1 = (noise or {}).get("p1", 0)
+    p2 = (noise or {}).get("p2", 0)
+    ops, mats = pack(dm_ops(n, _flatten(c._ops()), p1, p2))
+    vec = O.zero_state(2 * n)
+    O.apply_tape(2 * n, vec, np.ascontiguousarray(ops, dtype=OP_DTYPE), mats, prune=True)
+    rho = vec.reshape(1 << n, 1 << n)
+    ref = O.run_dm(n, calls, noise)
+    assert np.max(np.abs(rho - ref)) < 1e-12
+
+
+class _OracleCopies:
+    """CPU stand-in for a batched device state (the ksv_shots_* methods), every copy
+    simulated by the oracle; lets the host driver's event logic run without a GPU."""
+
+    def __init__(self, n_total, precision="f64", prune=True, tile=0):
+        self.n = n_total
+        self.prune = prune
+
+    def shots_init(self, n, index=0):
+        import ketsv_oracle as O
+        self.k = n
+        self.psi = np.array([O.zero_state(n, index) for _ in range(1 << (self.n - n))])
+
+    def _apply(self, ops, mats):
+        import ketsv_oracle as O
+        if len(ops):
+            for t in range(len(self.psi)):
+                O.apply_tape(self.k, self.psi[t], np.ascontiguousarray(ops, dtype=OP_DTYPE), mats, prune=self.prune)
+
+    def shots_run(self, n, tape, events, copies, paulis, r):
+        import ketsv_oracle as O
+        ops, mats = tape
+        out = np.zeros(len(r), dtype=np.uint8)
+        cur = 0
+        for e in events:
+            self._apply(ops[cur:int(e["at"])], mats)
+            cur = int(e["at"])
+            q, off = int(e["q"]), int(e["off"])
+            if e["kind"] == 0:
+                for j in range(off, off + int(e["count"])):
+                    O.apply_named(n, self.psi[int(copies[j])], [("xyz"[int(paulis[j]) - 1], (q,))], self.prune)
+            else:
+                for t in range(len(self.psi)):
+                    o = O.collapse(n, self.psi[t], q, float(r[off + t]))
+                    out[off + t] = o
+                    if e["kind"] == 2 and o == 1:
+                        O.apply_named(n, self.psi[t], [("x", (q,))], self.prune)
+        self._apply(ops[cur:], mats)
+        return out
+
+    def shots_sample(self, n, r):
+        import ketsv_oracle as O
+        return np.array([int(O.sample(n, self.psi[t], np.array([r[t]]))[0]) for t in range(len(self.psi))],
+                        dtype=np.uint64)
+
+    def free(self):
+        pass
+
+
+@pytest.mark.parametrize("noise,seed", [("harmony", 1), ({"p1": 0.05, "p2": 0.1, "pMeas": 0.05}, 2), (None, 3)])
+def test_batched_shot_driver_on_the_oracle(monkeypatch, noise, seed):
+    """run_batched's draw layout and event list (Pauli errors after their gate,
+    measurements, resets, SPAM flips, batches) with every copy simulated by the
+    oracle give the counts of the oracle's per-shot loop."""
+    import ketsv_oracle as O
+    import ket_amd.noise as N
+    from helpers import build, shot_calls
+    from ket_amd.circuit import make_prng
+    monkeypatch.setattr(N, "State", _OracleCopies)
+    monkeypatch.setattr(N, "MAX_COPIES", 8)
+    n = 3
+    calls = shot_calls(n, 20, seed)
+    c = build(n, calls, cregs={"c": 2})
+    params = N.resolve_noise(noise)
+    cregs = {"c": [0, 0]}
+    counts, cr = N.run_batched(c, 40, make_prng(seed), params, 0, cregs)
+    ref, rcr = O.run_shots(n, calls, 40, seed, params, cregs={"c": 2})
+    assert list(counts.items()) == list(ref.items())
+    assert cr == rcr
