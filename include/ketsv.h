@@ -99,6 +99,11 @@ typedef struct ksv_program ksv_program;
                                    11..13 (5-qubit blocks need 13, 11 takes 4-qubit blocks)   */
 #define KSV_OPT_BLOCK_THREADS 8 /* fused-block workgroup size: 0 = default (256) or 256, 512
                                    (512 only with 4-qubit blocks on 12-qubit tiles)            */
+#define KSV_OPT_ADJ_TILE     9  /* ksv_adjoint backward-pass tile qubits (phi and lambda tiles in
+                                   LDS): 0 = default (11 FP64, 12 FP32), or 10, 11 (FP64),
+                                   10..12 (FP32)                                               */
+#define KSV_OPT_ADJ_THREADS 10  /* ksv_adjoint backward-pass workgroup size: 0 = default (256),
+                                   256 or 512 (512 on the default tile only)                   */
 
 /* Per-state launch statistics (KSV_OPT_TIMING). */
 typedef struct ksv_stats {

@@ -246,7 +246,8 @@ def _adjoint(flat, records, nparam, n, hamiltonian):
             mats.append(complex(v).imag)
     mats = np.asarray(mats, dtype=np.float64)
     x, z, c = _terms(hamiltonian, n)
-    st = State(n, OPTIONS["precision"], prune=False, tile=OPTIONS["tile"])
+    st = State(n, OPTIONS["precision"], prune=False, tile=OPTIONS["tile"], adj_tile=OPTIONS["adj_tile"],
+               adj_threads=OPTIONS["adj_threads"])
     grad, energy = st.adjoint((ops, mats), rec, nparam, x, z, c)
     st.free()
     return grad, energy

@@ -27,6 +27,8 @@ OPTIONS = {
     "tile": 0,           # LDS tile qubits (0 = library default)
     "fuse": False,       # exact mode only: multiply runs of 1-qubit gates
     "batch_shots": True,  # run() without `if`: all shots as copies of one device state
+    "adj_tile": 0,       # adjoint backward passes: tile qubits (0 = library default)
+    "adj_threads": 0,    # ... workgroup size (0 = library default)
 }
 
 

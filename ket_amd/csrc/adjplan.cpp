@@ -1,0 +1,353 @@
+// Host planner of the adjoint backward sweep (ksv_adjoint).
+//
+// The sweep applies the inverse tape G_N^dag, ..., G_1^dag to phi and to
+// lambda and, at every op with parameter records, adds
+// Re <lambda| dG |G^dag phi> (lambda taken before its own G^dag).  This file
+// splits that inverse tape into passes over K-qubit tiles: a pass takes, in
+// order, every op whose targets lie in the tile and whose qubits no earlier
+// skipped op touches.  Taking an op ahead of skipped ops on other qubits is
+// exact for the derivative terms too: such ops U act on phi and lambda alike
+// and commute with dG, so <U lambda| dG |U phi> = <lambda| dG |phi>.
+// Controls may lie outside the tile (the kernel tests them on the tile's base
+// index).  Unitaries wider than three qubits run alone through the general
+// engine, on phi and then on lambda.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "ksv_internal.h"
+
+namespace ksv {
+
+namespace {
+
+struct AItem {
+    int kind = AK_DENSE;        // AKind, or -1: wide unitary for the general engine
+    std::vector<int> tg, ctl;   // targets (MSB first), controls
+    std::vector<cd> m;          // G^dag (AK_DENSE and wide), row-major
+    std::vector<int> recs;      // parameter records of the op
+    uint64_t need = 0, touch = 0;
+    bool diag = false;          // AK_DENSE: G^dag and every record's dG diagonal
+};
+
+int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+uint64_t mask_of(const std::vector<int>& v) {
+    uint64_t m = 0;
+    for (int q : v) m |= 1ull << q;
+    return m;
+}
+
+std::vector<cd> dagger(const std::vector<cd>& m, int dim) {
+    std::vector<cd> a((size_t)dim * dim);
+    for (int r = 0; r < dim; r++)
+        for (int c = 0; c < dim; c++) a[(size_t)r * dim + c] = std::conj(m[(size_t)c * dim + r]);
+    return a;
+}
+
+bool read_mat(const double* mats, size_t nmats, int64_t off, int dim, std::vector<cd>& m) {
+    const size_t cnt = 2 * (size_t)dim * dim;
+    if (off < 0 || (size_t)off + cnt > nmats) return false;
+    m.resize((size_t)dim * dim);
+    for (size_t j = 0; j < m.size(); j++) m[j] = cd(mats[off + 2 * j], mats[off + 2 * j + 1]);
+    return true;
+}
+
+bool offdiag_zero(const std::vector<cd>& m, int dim) {
+    for (int r = 0; r < dim; r++)
+        for (int c = 0; c < dim; c++)
+            if (r != c && m[(size_t)r * dim + c] != cd(0, 0)) return false;
+    return true;
+}
+
+// Same element enumeration: dense ops on the same targets in the same order
+// under the same controls.
+bool same_support(const AItem& a, const AItem& b) {
+    if (a.kind != AK_DENSE || b.kind != AK_DENSE || a.tg != b.tg) return false;
+    return mask_of(a.ctl) == mask_of(b.ctl);
+}
+
+// Order the ops of a pass into chains: op j joins the chain of an earlier op
+// i with the same support when no op left between them touches those
+// qubits (j then commutes with everything it moves past).  run[x] = chain
+// length at a chain head, 0 inside a chain.
+void make_chains(const std::vector<AItem>& it, const std::vector<int>& chosen, std::vector<int>& order,
+                 std::vector<int>& run) {
+    order.clear();
+    run.clear();
+    std::vector<char> used(chosen.size(), 0);
+    for (size_t i = 0; i < chosen.size(); i++) {
+        if (used[i]) continue;
+        used[i] = 1;
+        const AItem& a = it[chosen[i]];
+        const size_t head = order.size();
+        order.push_back(chosen[i]);
+        run.push_back(1);
+        if (a.kind != AK_DENSE || a.tg.size() > 2) continue;
+        const int cap = a.tg.size() == 1 ? ACHAIN_1Q : ACHAIN_2Q;
+        uint64_t blk = 0;
+        for (size_t j = i + 1; j < chosen.size() && run[head] < cap; j++) {
+            if (used[j]) continue;
+            const AItem& b = it[chosen[j]];
+            if (!(b.touch & blk) && same_support(a, b)) {
+                used[j] = 1;
+                order.push_back(chosen[j]);
+                run.push_back(0);
+                run[head]++;
+            } else {
+                blk |= b.touch;
+                if (blk & a.touch) break;
+            }
+        }
+    }
+}
+
+// Ops of `rem` that a pass over tile T takes (see the file comment), within
+// ADJ_SLOTS parameter records.  Scans at most `window` ops when chosen == null.
+int take(const std::vector<AItem>& it, const std::vector<int>& rem, uint64_t T, uint64_t all, size_t window,
+         std::vector<int>* chosen, std::vector<int>* rest) {
+    uint64_t blocked = 0;
+    int nslot = 0, count = 0;
+    size_t i = 0;
+    const size_t lim = chosen ? rem.size() : std::min(rem.size(), window);
+    for (; i < lim; i++) {
+        const AItem& o = it[rem[i]];
+        const bool ok = o.kind >= 0 && !(o.touch & blocked) && !(o.need & ~T) &&
+                        nslot + (int)o.recs.size() <= ADJ_SLOTS;
+        if (ok) {
+            count++;
+            nslot += (int)o.recs.size();
+            if (chosen) chosen->push_back(rem[i]);
+        } else {
+            blocked |= o.touch;
+            if (rest) rest->push_back(rem[i]);
+        }
+        if ((blocked & all) == all) {
+            i++;
+            break;
+        }
+    }
+    if (rest)
+        for (; i < rem.size(); i++) rest->push_back(rem[i]);
+    return count;
+}
+
+// Grow T first-come: every op in order that is not blocked and still fits.
+uint64_t fill_first_come(const std::vector<AItem>& it, const std::vector<int>& rem, uint64_t T, int K,
+                         uint64_t all) {
+    uint64_t blocked = 0;
+    for (int id : rem) {
+        const AItem& o = it[id];
+        if (o.kind < 0 || (o.touch & blocked)) blocked |= o.touch;
+        else if (popc(T | o.need) <= K) T |= o.need;
+        else blocked |= o.touch;
+        if ((blocked & all) == all || popc(T) == K) break;
+    }
+    return T;
+}
+
+// Tile of the next pass: greedy forward selection (add the qubit that lets
+// the pass take the most ops) from the first op's targets, then first-come
+// fill; kept if it takes more ops than plain first-come growth.
+uint64_t choose_tile(const std::vector<AItem>& it, const std::vector<int>& rem, int n, int K, uint64_t forced,
+                     uint64_t all) {
+    constexpr size_t WINDOW = 2048;
+    uint64_t T = forced | it[rem[0]].need;
+    uint64_t cand = 0;
+    for (size_t i = 0; i < std::min(rem.size(), WINDOW); i++)
+        if (it[rem[i]].kind >= 0) cand |= it[rem[i]].need;
+    int cur = take(it, rem, T, all, WINDOW, nullptr, nullptr);
+    while (popc(T) < K) {
+        int best = cur, bq = -1;
+        for (int q = 0; q < n; q++) {
+            if (!((cand >> q) & 1) || ((T >> q) & 1)) continue;
+            const int s = take(it, rem, T | (1ull << q), all, WINDOW, nullptr, nullptr);
+            if (s > best) { best = s; bq = q; }
+        }
+        if (bq < 0) break;
+        T |= 1ull << bq;
+        cur = best;
+    }
+    T = fill_first_come(it, rem, T, K, all);
+    const uint64_t F = fill_first_come(it, rem, forced, K, all);
+    if (take(it, rem, F, all, WINDOW, nullptr, nullptr) > take(it, rem, T, all, WINDOW, nullptr, nullptr)) T = F;
+    for (int q = 0; q < n && popc(T) < K; q++) T |= 1ull << q;
+    return T;
+}
+
+}  // namespace
+
+int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err) {
+    plan = AdjPlan();
+    K = std::min(n, std::max(K, 8));
+    plan.k = K;
+    std::vector<std::vector<int>> recs_of(nops);
+    for (size_t r = 0; r < nrec; r++) recs_of[(size_t)params[r].op].push_back((int)r);
+    // inverse tape
+    std::vector<AItem> items;
+    items.reserve(nops);
+    for (size_t i = nops; i-- > 0;) {
+        const ksv_op& o = ops[i];
+        const int32_t* q = o.q;
+        AItem a;
+        std::vector<cd> m;
+        int dim = 0;
+        switch (o.kind) {
+        case KSV_BARRIER: continue;
+        case KSV_SINGLE: a.tg = {q[0]}; dim = 2; break;
+        case KSV_CONTROLLED: a.tg = {q[1]}; a.ctl = {q[0]}; dim = 2; break;
+        case KSV_TWO: a.tg = {q[0], q[1]}; dim = 4; break;
+        case KSV_CNOT: a.kind = AK_X; a.tg = {q[1]}; a.ctl = {q[0]}; break;
+        case KSV_TOFFOLI: a.kind = AK_X; a.tg = {q[2]}; a.ctl = {q[0], q[1]}; break;
+        case KSV_SWAP: a.kind = AK_SWAP; a.tg = {q[0], q[1]}; break;
+        case KSV_CSWAP: a.kind = AK_SWAP; a.tg = {q[1], q[2]}; a.ctl = {q[0]}; break;
+        case KSV_CSRSWAP: {
+            // src/statevector.ts:177-207: |c01>, |c10> mixed by [[1, i], [i, 1]] / sqrt(2)
+            const double s = 1 / std::sqrt(2.0);
+            a.tg = {q[1], q[2]};
+            a.ctl = {q[0]};
+            m.assign(16, cd(0, 0));
+            m[0] = m[15] = 1;
+            m[5] = m[10] = cd(s, 0);
+            m[6] = m[9] = cd(0, s);
+            a.m = dagger(m, 4);
+            break;
+        }
+        case KSV_UNITARY:
+            if (o.nq < 1 || o.nq > KSV_MAX_OPERANDS) { err = "unitary: bad operand count"; return KSV_ERR_TYPE; }
+            a.tg.assign(q, q + o.nq);
+            dim = 1 << o.nq;
+            if (o.nq > ADJ_DENSE_MAX) a.kind = -1;
+            break;
+        default: err = "op " + std::to_string(i) + ": unknown kind"; return KSV_ERR_TYPE;
+        }
+        for (int t : a.tg)
+            if (t < 0 || t >= n) { err = "op " + std::to_string(i) + ": qubit out of range"; return KSV_ERR_RANGE; }
+        for (int c : a.ctl)
+            if (c < 0 || c >= n) { err = "op " + std::to_string(i) + ": qubit out of range"; return KSV_ERR_RANGE; }
+        if (dim) {
+            if (!read_mat(mats, nmats, o.mat, dim, m)) { err = "op " + std::to_string(i) + ": bad matrix offset"; return KSV_ERR_TYPE; }
+            a.m = dagger(m, dim);
+        }
+        a.recs = recs_of[i];
+        if (a.kind == AK_DENSE) {
+            const int d = 1 << a.tg.size();
+            a.diag = offdiag_zero(a.m, d);
+            for (int r : a.recs) {
+                std::vector<cd> dm;
+                if (a.diag && read_mat(mats, nmats, params[r].dmat, d, dm)) a.diag = offdiag_zero(dm, d);
+            }
+        }
+        a.need = mask_of(a.tg);
+        a.touch = a.need | mask_of(a.ctl);
+        if (popc(a.touch) != (int)(a.tg.size() + a.ctl.size())) {
+            err = "op " + std::to_string(i) + ": operands must be distinct";
+            return KSV_ERR_TYPE;
+        }
+        items.push_back(std::move(a));
+    }
+
+    const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    // the tile always holds the lowest qubits: 128-byte runs of amplitudes
+    const uint64_t forced = (1ull << std::min(K, 3)) - 1;
+    std::vector<int> rem(items.size()), chosen, rest, order, run;
+    std::iota(rem.begin(), rem.end(), 0);
+    while (!rem.empty()) {
+        const AItem& first = items[rem[0]];
+        if (first.kind < 0) {
+            AdjStep s;
+            s.big = true;
+            s.bop = (int)plan.bops.size();
+            ksv_op w;
+            std::memset(&w, 0, sizeof w);
+            w.kind = KSV_UNITARY;
+            w.nq = (int32_t)first.tg.size();
+            for (size_t j = 0; j < first.tg.size(); j++) w.q[j] = first.tg[j];
+            w.mat = (int64_t)plan.bmats.size();
+            for (const cd& z : first.m) {
+                plan.bmats.push_back(z.real());
+                plan.bmats.push_back(z.imag());
+            }
+            plan.bops.push_back(w);
+            plan.steps.push_back(s);
+            rem.erase(rem.begin());
+            continue;
+        }
+        const uint64_t T = choose_tile(items, rem, n, K, forced, all);
+        chosen.clear();
+        rest.clear();
+        take(items, rem, T, all, 0, &chosen, &rest);
+        make_chains(items, chosen, order, run);
+        AdjStep s;
+        s.k = K;
+        int pos[64];
+        for (int& p : pos) p = -1;
+        int j = 0;
+        for (int qb = 0; qb < n; qb++)
+            if ((T >> qb) & 1) {
+                s.tq[j] = (int8_t)qb;
+                pos[qb] = j++;
+            }
+        s.op_begin = (int)plan.aops.size();
+        s.slot_begin = (int)plan.slot_rec.size();
+        for (size_t x = 0; x < order.size(); x++) {
+            const AItem& it = items[order[x]];
+            AOp a;
+            std::memset(&a, 0, sizeof a);
+            a.kind = it.kind;
+            a.run = (uint8_t)run[x];
+            a.flags = it.diag ? AOP_DIAG : 0;
+            a.k = (int32_t)it.tg.size();
+            std::vector<int> ins;
+            for (size_t b = 0; b < it.tg.size(); b++) {
+                a.t[b] = (int8_t)pos[it.tg[b]];
+                ins.push_back(pos[it.tg[b]]);
+            }
+            for (int c : it.ctl) {
+                if (pos[c] >= 0) {
+                    a.lctl |= 1u << pos[c];
+                    ins.push_back(pos[c]);
+                } else {
+                    a.gctl |= 1ull << c;
+                }
+            }
+            std::sort(ins.begin(), ins.end());
+            a.nins = (int32_t)ins.size();
+            for (size_t b = 0; b < ins.size(); b++) a.ins[b] = (int8_t)ins[b];
+            if (it.kind == AK_DENSE) {
+                a.mat = (int32_t)(plan.mats.size() / 2);
+                for (const cd& z : it.m) {
+                    plan.mats.push_back(z.real());
+                    plan.mats.push_back(z.imag());
+                }
+            }
+            a.nrec = (int32_t)it.recs.size();
+            const int dim = 1 << a.k;
+            for (size_t r = 0; r < it.recs.size(); r++) {
+                const ksv_param& p = params[it.recs[r]];
+                if (p.dmat < 0 || (size_t)p.dmat + 2 * (size_t)dim * dim > nmats) {
+                    err = "bad derivative matrix offset";
+                    return KSV_ERR_TYPE;
+                }
+                a.dmat[r] = (int32_t)(plan.mats.size() / 2);
+                plan.mats.insert(plan.mats.end(), mats + p.dmat, mats + p.dmat + 2 * (size_t)dim * dim);
+                a.slot[r] = (int16_t)((int)plan.slot_rec.size() - s.slot_begin);
+                plan.slot_rec.push_back(it.recs[r]);
+            }
+            plan.aops.push_back(a);
+        }
+        s.op_count = (int)plan.aops.size() - s.op_begin;
+        s.slot_count = (int)plan.slot_rec.size() - s.slot_begin;
+        plan.steps.push_back(s);
+        rem.swap(rest);
+    }
+    if (plan.mats.size() / 2 > (size_t)INT32_MAX) {
+        err = "adjoint tape too large";
+        return KSV_ERR_RANGE;
+    }
+    return KSV_OK;
+}
+
+}  // namespace ksv

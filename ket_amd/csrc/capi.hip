@@ -50,6 +50,8 @@ struct ksv_state {
     int bq = 0;                     // KSV_OPT_BLOCK_QUBITS
     int btile = 0;                  // KSV_OPT_BLOCK_TILE
     int bthreads = 256;             // KSV_OPT_BLOCK_THREADS
+    int atile = 0;                  // KSV_OPT_ADJ_TILE
+    int athreads = 256;             // KSV_OPT_ADJ_THREADS
     ksv_stats stats;
     std::vector<hipEvent_t> ev;
 };
@@ -154,6 +156,19 @@ int ksv_set_option(ksv_state* s, int option, double value) {
         int t = (int)value;
         if (t != 0 && t != 256 && t != 512) return fail(KSV_ERR_RANGE, "fused-block workgroup size must be 0, 256 or 512");
         s->bthreads = t ? t : 256;
+        return KSV_OK;
+    }
+    case KSV_OPT_ADJ_TILE: {
+        int t = (int)value;
+        const int hi = s->prec == KSV_F64 ? ADJ_TILE_F64 : ADJ_TILE_F32;
+        if (t != 0 && (t < 10 || t > hi)) return fail(KSV_ERR_RANGE, "adjoint tile qubits must be 0 or 10..default");
+        s->atile = t;
+        return KSV_OK;
+    }
+    case KSV_OPT_ADJ_THREADS: {
+        int t = (int)value;
+        if (t != 0 && t != 256 && t != 512) return fail(KSV_ERR_RANGE, "adjoint workgroup size must be 0, 256 or 512");
+        s->athreads = t ? t : 256;
         return KSV_OK;
     }
     case KSV_OPT_BLOCK_TILE: {
@@ -1060,7 +1075,7 @@ namespace ksv {
 int adj_fail(int code, const std::string& msg) { return fail(code, msg); }
 
 int adj_context(ksv_state* s, int* n, int* prec, void** psi, void** psi2, hipStream_t* st, double** part,
-                int* prune, int* tile) {
+                int* prune, int* tile, int* atile, int* athreads) {
     if (!s) return fail(KSV_ERR_HANDLE, "null state");
     int rc = ensure_psi2(s);
     if (rc) return rc;
@@ -1072,6 +1087,8 @@ int adj_context(ksv_state* s, int* n, int* prec, void** psi, void** psi2, hipStr
     *part = s->d_part;
     *prune = s->prune;
     *tile = default_tile(s);
+    *atile = s->atile ? s->atile : (s->prec == KSV_F64 ? ADJ_TILE_F64 : ADJ_TILE_F32);
+    *athreads = s->athreads;
     return KSV_OK;
 }
 

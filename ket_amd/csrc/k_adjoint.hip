@@ -15,118 +15,339 @@
 #include <cstring>
 #include <map>
 
-#include "ksv_internal.h"
+#include "kcommon.h"
 
 namespace ksv {
 
-template <typename V>
-__device__ __forceinline__ void cfma(double& ar, double& ai, V m, V x) {
-    ar += (double)m.x * x.x - (double)m.y * x.y;
-    ai += (double)m.x * x.y + (double)m.y * x.x;
+// Local index of element e of an op: zero bits inserted at the op's
+// ascending positions, then its local control bits set.
+__device__ __forceinline__ uint32_t aop_elem(uint32_t e, const AOp& op) {
+    uint32_t l = e;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if (i < op.nins) l = ins0(l, op.ins[i]);
+    return l | op.lctl;
 }
 
-struct AdjArgs {
+struct AdjPassArgs {
     void* phi;
     void* lam;
-    int k;              // targets (1 or 2)
-    int nrec;           // derivative records (<= 3)
-    int8_t t[2];        // targets, t[0] = MSB of the local index
-    int8_t ins[4];      // ascending bits to insert (targets + controls)
-    int nins;
-    uint64_t ctl;       // control bits (set after insertion)
-    double ud[32];      // U^dag, row-major interleaved (dim x dim)
-    double du[3][32];   // dU per record
+    const AOp* ops;
+    const double* mats;        // interleaved complex
+    const int32_t* slot_rec;   // global record of each slot
+    double* part;              // [record][gridDim.x]
+    int nops;
+    int nslot;
+    int k;
+    int pad;
+    uint64_t ntiles;
+    uint64_t tq_lo, tq_hi;     // physical bit of tile bit i, ascending
 };
 
-template <typename V>
-__global__ __launch_bounds__(256) void k_adj(AdjArgs a, uint64_t groups, double* part) {
-    __shared__ double sh[4];
-    V* phi = reinterpret_cast<V*>(a.phi);
-    V* lam = reinterpret_cast<V*>(a.lam);
-    const int dim = 1 << a.k;
-    double acc[3] = {0, 0, 0};
-    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < groups; g += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t b = g;
+// Fields of chain member c, read when its step starts (scalar loads), so
+// only one member's fields and matrices occupy SGPRs at a time.
+struct ChainOp {
+    int32_t mat, nrec, diag;
+    int32_t dmat[3];
+};
+__device__ __forceinline__ ChainOp chain_op(const KSV_CONST uint64_t* opw, int c) {
+    AOp m;
+    uint64_t w[sizeof(AOp) / 8];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (i < a.nins) b = ((b >> a.ins[i]) << (a.ins[i] + 1)) | (b & ((1ull << a.ins[i]) - 1));
-        b |= a.ctl;
-        uint64_t id[4];
-        V x[4], l[4];
+    for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = opw[(sizeof(AOp) / 8) * c + i];
+    __builtin_memcpy(&m, w, sizeof m);
+    ChainOp q;
+    q.mat = m.mat;
+    q.nrec = m.nrec;
+    q.diag = m.flags & AOP_DIAG;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (r < dim) {
-                uint64_t gi = b;
-                if (a.k == 1) { if (r & 1) gi |= 1ull << a.t[0]; }
-                else {
-                    if (r & 2) gi |= 1ull << a.t[0];
-                    if (r & 1) gi |= 1ull << a.t[1];
-                }
-                id[r] = gi;
-                x[r] = phi[gi];
-                l[r] = lam[gi];
+    for (int j = 0; j < 3; j++) q.dmat[j] = m.dmat[j];
+    return q;
+}
+
+// A chain of `run` dense ops on the same 2^KK-element groups: G groups per
+// thread and iteration (all their LDS reads issued before any arithmetic),
+// each op applies G^dag to phi and lambda in registers and, for each
+// record j, acc[c][j] += Re sum_r conj(lambda_r) (dG_j G^dag phi)_r with
+// lambda taken before its own G^dag.  Diagonal ops cost one complex
+// product per amplitude and state.
+template <typename V, int KK, int G, int CMAX, int NT>
+__device__ __forceinline__ void adj_chain(V* P, V* L, const AOp& op, const KSV_CONST uint64_t* opw, int run,
+                                          const KSV_CONST double* mats, uint32_t ne, uint32_t tid,
+                                          double (&acc)[ACHAIN_1Q][3]) {
+    constexpr int D = 1 << KK;
+    for (uint32_t e0 = tid; e0 < ne; e0 += G * NT) {
+        uint32_t id[G][D];
+        double xr[G][D], xi[G][D], yr[G][D], yi[G][D];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const uint32_t l = aop_elem(e0 + g * NT, op);
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                uint32_t q = l;
+#pragma unroll
+                for (int b = 0; b < KK; b++)
+                    if ((r >> (KK - 1 - b)) & 1) q |= 1u << op.t[b];
+                id[g][r] = q;
             }
         }
-        V xn[4], ln[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (r < dim) {
-                double xr = 0, xi = 0, lr = 0, li = 0;
+        for (int g = 0; g < G; g++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    if (c < dim) {
-                        V m;
-                        m.x = a.ud[2 * (r * dim + c)];
-                        m.y = a.ud[2 * (r * dim + c) + 1];
-                        cfma(xr, xi, m, x[c]);
-                        cfma(lr, li, m, l[c]);
+            for (int r = 0; r < D; r++) {
+                const V x = P[id[g][r]], y = L[id[g][r]];
+                xr[g][r] = x.x; xi[g][r] = x.y;
+                yr[g][r] = y.x; yi[g][r] = y.y;
+            }
+#pragma unroll
+        for (int c = 0; c < CMAX; c++) {
+            if (c >= run) break;
+            // keeps the scalar loads of this op's matrices from being hoisted
+            // above the step: one op's matrices live in SGPRs at a time
+            asm volatile("" ::: "memory");
+            const ChainOp q = chain_op(opw, c);
+            const KSV_CONST double* u = mats + 2 * (int64_t)q.mat;
+            double ar[G][D], ai[G][D], br[G][D], bi[G][D];
+            if (q.diag) {
+#pragma unroll
+                for (int r = 0; r < D; r++) {
+                    const double mr = u[2 * (r * D + r)], mi = u[2 * (r * D + r) + 1];
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        ar[g][r] = fma(mr, xr[g][r], -mi * xi[g][r]);
+                        ai[g][r] = fma(mr, xi[g][r], mi * xr[g][r]);
+                        br[g][r] = fma(mr, yr[g][r], -mi * yi[g][r]);
+                        bi[g][r] = fma(mr, yi[g][r], mi * yr[g][r]);
                     }
                 }
-                xn[r].x = xr; xn[r].y = xi;
-                ln[r].x = lr; ln[r].y = li;
+            } else {
+#pragma unroll
+                for (int r = 0; r < D; r++) {
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        double sr = 0, si = 0, tr = 0, ti = 0;
+#pragma unroll
+                        for (int cc = 0; cc < D; cc++) {
+                            const double mr = u[2 * (r * D + cc)], mi = u[2 * (r * D + cc) + 1];
+                            sr = fma(-mi, xi[g][cc], fma(mr, xr[g][cc], sr));
+                            si = fma(mi, xr[g][cc], fma(mr, xi[g][cc], si));
+                            tr = fma(-mi, yi[g][cc], fma(mr, yr[g][cc], tr));
+                            ti = fma(mi, yr[g][cc], fma(mr, yi[g][cc], ti));
+                        }
+                        ar[g][r] = sr; ai[g][r] = si;
+                        br[g][r] = tr; bi[g][r] = ti;
+                    }
+                }
             }
-        }
+            if (KK <= 2) {   // parameters sit on 1- and 2-qubit gates only
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            if (j < a.nrec) {
-                double s = 0;
+                for (int j = 0; j < 3; j++) {
+                    if (j >= q.nrec) break;
+                    asm volatile("" ::: "memory");
+                    const KSV_CONST double* d = mats + 2 * (int64_t)q.dmat[j];
+                    double s = 0;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    if (r < dim) {
-                        double tr = 0, ti = 0;
+                    for (int r = 0; r < D; r++) {
+                        if (q.diag) {
+                            const double mr = d[2 * (r * D + r)], mi = d[2 * (r * D + r) + 1];
 #pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            if (c < dim) {
-                                V m;
-                                m.x = a.du[j][2 * (r * dim + c)];
-                                m.y = a.du[j][2 * (r * dim + c) + 1];
-                                cfma(tr, ti, m, xn[c]);
+                            for (int g = 0; g < G; g++) {
+                                const double tr = fma(mr, ar[g][r], -mi * ai[g][r]);
+                                const double ti = fma(mr, ai[g][r], mi * ar[g][r]);
+                                s = fma(yi[g][r], ti, fma(yr[g][r], tr, s));
+                            }
+                        } else {
+#pragma unroll
+                            for (int g = 0; g < G; g++) {
+                                double tr = 0, ti = 0;
+#pragma unroll
+                                for (int cc = 0; cc < D; cc++) {
+                                    const double mr = d[2 * (r * D + cc)], mi = d[2 * (r * D + cc) + 1];
+                                    tr = fma(-mi, ai[g][cc], fma(mr, ar[g][cc], tr));
+                                    ti = fma(mi, ar[g][cc], fma(mr, ai[g][cc], ti));
+                                }
+                                s = fma(yi[g][r], ti, fma(yr[g][r], tr, s));
                             }
                         }
-                        s += (double)l[r].x * tr + (double)l[r].y * ti;   // Re(conj(l) t)
                     }
+                    acc[c][j] += s;
                 }
-                acc[j] += s;
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < D; r++) {
+                    xr[g][r] = ar[g][r]; xi[g][r] = ai[g][r];
+                    yr[g][r] = br[g][r]; yi[g][r] = bi[g][r];
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                V x, y;
+                x.x = xr[g][r]; x.y = xi[g][r];
+                y.x = yr[g][r]; y.y = yi[g][r];
+                P[id[g][r]] = x;
+                L[id[g][r]] = y;
+            }
+    }
+}
+
+// The widest group batch that divides the op's element count: G * NT
+// elements per iteration.
+template <typename V, int KK, int GMAX, int CMAX, int NT>
+__device__ __forceinline__ void adj_chain_g(V* P, V* L, const AOp& op, const KSV_CONST uint64_t* opw, int run,
+                                            const KSV_CONST double* mats, uint32_t ne, uint32_t tid,
+                                            double (&acc)[ACHAIN_1Q][3]) {
+    if constexpr (GMAX >= 4) {
+        if (ne >= 4 * NT) return adj_chain<V, KK, 4, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+    }
+    if constexpr (GMAX >= 2) {
+        if (ne >= 2 * NT) return adj_chain<V, KK, 2, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+    }
+    adj_chain<V, KK, 1, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+}
+
+// One backward pass: the workgroup walks tiles blockIdx.x, +gridDim.x, ...;
+// for each it stages phi and lambda in LDS, applies the pass's ops to both
+// and accumulates the records' inner products in red[slot][wave]; at the
+// end it writes one partial per record of the pass.
+template <typename V, int KT, int NT>
+__global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
+    constexpr int NW = NT / 64;
+    constexpr int LNT = NT == 512 ? 9 : 8;
+    __shared__ V P[1 << KT], L[1 << KT];
+    __shared__ double red[ADJ_SLOTS][NW];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t i = tid; i < ADJ_SLOTS * NW; i += NT) (&red[0][0])[i] = 0;
+    const int k = a.k;
+    const uint32_t tsize = 1u << k;
+    V* phi = reinterpret_cast<V*>(a.phi);
+    V* lam = reinterpret_cast<V*>(a.lam);
+    const KSV_CONST uint64_t* ops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
+    const KSV_CONST double* mats = to_const(a.mats);
+
+    uint64_t g_t = 0;   // physical offset of the thread bits
+#pragma unroll
+    for (int b = 0; b < LNT; b++)
+        if (b < k) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(b < 8 ? a.tq_lo : a.tq_hi, b & 7);
+
+    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+        uint64_t base = tile;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < k) base = ins0_64(base, byte_at(i < 8 ? a.tq_lo : a.tq_hi, i & 7));
+        for (uint32_t xh = 0; xh < tsize; xh += NT) {
+            const uint32_t l = tid + xh;
+            if (l < tsize) {
+                uint64_t g_h = 0;
+#pragma unroll
+                for (int b = LNT; b < KT; b++)
+                    if (b < k && ((xh >> b) & 1)) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
+                P[l] = phi[base | g_t | g_h];
+                L[l] = lam[base | g_t | g_h];
             }
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-            if (r < dim) { phi[id[r]] = xn[r]; lam[id[r]] = ln[r]; }
-    }
-    for (int j = 0; j < 3; j++) {
-        double v = acc[j];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        if (lane == 0) sh[w] = v;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = 0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); i++) r += sh[i];
-            part[(size_t)blockIdx.x * 3 + j] = r;
+        for (int o = 0; o < a.nops; o++) {
+            AOp op;
+            {
+                uint64_t w[sizeof(AOp) / 8];
+#pragma unroll
+                for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = ops[(sizeof(AOp) / 8) * o + i];
+                __builtin_memcpy(&op, w, sizeof op);
+            }
+            if ((base & op.gctl) != op.gctl) {   // a chain shares its head's controls
+                if (op.kind == AK_DENSE) o += op.run - 1;
+                continue;
+            }
+            const uint32_t ne = tsize >> op.nins;
+            if (op.kind == AK_X) {
+                const uint32_t tb = 1u << op.t[0];
+                for (uint32_t e = tid; e < ne; e += NT) {
+                    const uint32_t l0 = aop_elem(e, op), l1 = l0 | tb;
+                    const V x0 = P[l0], x1 = P[l1], y0 = L[l0], y1 = L[l1];
+                    P[l0] = x1; P[l1] = x0;
+                    L[l0] = y1; L[l1] = y0;
+                }
+            } else if (op.kind == AK_SWAP) {
+                const uint32_t ba = 1u << op.t[0], bb = 1u << op.t[1];
+                for (uint32_t e = tid; e < ne; e += NT) {
+                    const uint32_t l = aop_elem(e, op), la = l | ba, lb = l | bb;
+                    const V x0 = P[la], x1 = P[lb], y0 = L[la], y1 = L[lb];
+                    P[la] = x1; P[lb] = x0;
+                    L[la] = y1; L[lb] = y0;
+                }
+            } else {
+                // a chain of `run` dense ops from here
+                const int run = op.run;
+                const KSV_CONST uint64_t* opw = ops + (sizeof(AOp) / 8) * o;
+                double acc[ACHAIN_1Q][3];
+#pragma unroll
+                for (int c = 0; c < ACHAIN_1Q; c++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) acc[c][j] = 0;
+                constexpr int G1 = (1 << (KT - 1)) / NT >= 4 ? 4 : (1 << (KT - 1)) / NT;
+                constexpr int G2 = (1 << (KT - 2)) / NT >= 2 ? 2 : 1;
+                if (op.k == 1) adj_chain_g<V, 1, G1, ACHAIN_1Q, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+                else if (op.k == 2) adj_chain_g<V, 2, G2, ACHAIN_2Q, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+                else adj_chain<V, 3, 1, 1, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+#pragma unroll
+                for (int c = 0; c < ACHAIN_1Q; c++) {
+                    if (c >= run) break;
+                    AOp m;
+                    {
+                        uint64_t w[sizeof(AOp) / 8];
+#pragma unroll
+                        for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = opw[(sizeof(AOp) / 8) * c + i];
+                        __builtin_memcpy(&m, w, sizeof m);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        if (j >= m.nrec) break;
+                        double v = acc[c][j];
+#pragma unroll
+                        for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+                        if (lane == 0) red[m.slot[j]][wave] += v;
+                    }
+                }
+                o += run - 1;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        for (uint32_t xh = 0; xh < tsize; xh += NT) {
+            const uint32_t l = tid + xh;
+            if (l < tsize) {
+                uint64_t g_h = 0;
+#pragma unroll
+                for (int b = LNT; b < KT; b++)
+                    if (b < k && ((xh >> b) & 1)) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
+                phi[base | g_t | g_h] = P[l];
+                lam[base | g_t | g_h] = L[l];
+            }
+        }
     }
+    __syncthreads();
+    for (int s = (int)tid; s < a.nslot; s += NT) {
+        double v = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) v += red[s][w];
+        a.part[(size_t)a.slot_rec[s] * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// out[r] = sum_g part[r * G + g], one workgroup per record.
+__global__ __launch_bounds__(256) void k_recsum(const double* part, int G, double* out) {
+    __shared__ double sh[4];
+    const size_t r = blockIdx.x;
+    double v = 0;
+    for (int g = threadIdx.x; g < G; g += 256) v += part[r * G + g];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[r] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 // lambda = sum_t c_t P_t psi for up to 8 terms per launch, accumulated into
@@ -178,12 +399,31 @@ __global__ __launch_bounds__(256) void k_dotre(const V* a, const V* b, uint64_t 
 
 using namespace ksv;
 
+// LDS capacity (tile qubits) and workgroup size of the instantiated passes;
+// a plan's tile k is at most the capacity.
+static hipError_t launch_adjpass(int prec, int tile, int threads, int G, const AdjPassArgs& a, hipStream_t st) {
+    const dim3 grid((unsigned)G);
+    if (prec == KSV_F64) {
+        if (tile == 11 && threads == 512) hipLaunchKernelGGL((k_adjpass<double2, 11, 512>), grid, dim3(512), 0, st, a);
+        else if (tile == 11) hipLaunchKernelGGL((k_adjpass<double2, 11, 256>), grid, dim3(256), 0, st, a);
+        else if (tile == 10) hipLaunchKernelGGL((k_adjpass<double2, 10, 256>), grid, dim3(256), 0, st, a);
+        else return hipErrorInvalidValue;
+    } else {
+        if (tile == 12 && threads == 512) hipLaunchKernelGGL((k_adjpass<float2, 12, 512>), grid, dim3(512), 0, st, a);
+        else if (tile == 12) hipLaunchKernelGGL((k_adjpass<float2, 12, 256>), grid, dim3(256), 0, st, a);
+        else if (tile == 11) hipLaunchKernelGGL((k_adjpass<float2, 11, 256>), grid, dim3(256), 0, st, a);
+        else if (tile == 10) hipLaunchKernelGGL((k_adjpass<float2, 10, 256>), grid, dim3(256), 0, st, a);
+        else return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ---- host side --------------------------------------------------------------
 
 namespace ksv {
 // Entry points of capi.hip used here.
 int adj_context(ksv_state* s, int* n, int* prec, void** psi, void** psi2, hipStream_t* st, double** part,
-                int* prune, int* tile);
+                int* prune, int* tile, int* atile, int* athreads);
 int adj_fail(int code, const std::string& msg);
 int adj_run_tape(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int prune,
                  void* psi);
@@ -195,75 +435,28 @@ int adj_run_tape(ksv_state* s, const ksv_op* ops, size_t nops, const double* mat
         if (e_ != hipSuccess) return adj_fail(KSV_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-static void adjoint_of(const std::vector<cd>& m, int dim, std::vector<cd>& out) {
-    out.assign((size_t)dim * dim, cd(0, 0));
-    for (int r = 0; r < dim; r++)
-        for (int c = 0; c < dim; c++) out[(size_t)r * dim + c] = std::conj(m[(size_t)c * dim + r]);
-}
-
-// Inverse of a tape segment: reversed ops with adjoint matrices.
-static void invert_tape(const ksv_op* ops, size_t b, size_t e, const double* mats, std::vector<ksv_op>& out,
-                        std::vector<double>& om) {
-    out.clear();
-    om.clear();
-    static const int dims[] = {2, 0, 0, 4, 2, 0, 0, 0, 0, 0};
-    for (size_t i = e; i-- > b;) {
-        ksv_op o = ops[i];
-        if (o.kind == KSV_CSRSWAP) {
-            // inverse of the controlled sqrt(iSWAP): conj-transposed 4x4 with the control
-            const double s = 1 / std::sqrt(2.0);
-            double m[32] = {0};
-            m[0] = 1;
-            m[2 * 5] = s;
-            m[2 * 6 + 1] = -s;
-            m[2 * 9 + 1] = -s;
-            m[2 * 10] = s;
-            m[2 * 15] = 1;
-            // as a KSV_UNITARY on (control, a, b) is wider; express as controlled via an 8x8
-            double u[128] = {0};
-            for (int r = 0; r < 4; r++) { u[2 * (r * 8 + r)] = 1; }
-            for (int r = 0; r < 4; r++)
-                for (int c = 0; c < 4; c++) {
-                    u[2 * ((4 + r) * 8 + 4 + c)] = m[2 * (r * 4 + c)];
-                    u[2 * ((4 + r) * 8 + 4 + c) + 1] = m[2 * (r * 4 + c) + 1];
-                }
-            ksv_op w;
-            std::memset(&w, 0, sizeof w);
-            w.kind = KSV_UNITARY;
-            w.nq = 3;
-            w.q[0] = o.q[0];
-            w.q[1] = o.q[1];
-            w.q[2] = o.q[2];
-            w.mat = (int64_t)om.size();
-            om.insert(om.end(), u, u + 128);
-            out.push_back(w);
-            continue;
-        }
-        int dim = 0;
-        if (o.kind == KSV_UNITARY) dim = 1 << o.nq;
-        else dim = dims[o.kind];
-        if (dim) {
-            std::vector<cd> m((size_t)dim * dim), a;
-            for (size_t j = 0; j < m.size(); j++) m[j] = cd(mats[o.mat + 2 * j], mats[o.mat + 2 * j + 1]);
-            adjoint_of(m, dim, a);
-            o.mat = (int64_t)om.size();
-            for (auto& z : a) { om.push_back(z.real()); om.push_back(z.imag()); }
-        } else {
-            o.mat = -1;   // permutations (cnot, swap, toffoli, cswap) are self-inverse
-        }
-        out.push_back(o);
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 256)); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
     }
-}
+};
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
 
 extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                            const ksv_param* params, size_t nrec, size_t nparam, const uint64_t* xmask,
                            const uint64_t* zmask, const double* coeff, size_t nterms, double* grad, double* energy) {
-    int n, prec, prune, tile;
+    int n, prec, prune, tile, atile, athreads;
     void *psi, *psi2;
     hipStream_t st;
     double* part;
-    int rc0 = adj_context(s, &n, &prec, &psi, &psi2, &st, &part, &prune, &tile);
+    int rc0 = adj_context(s, &n, &prec, &psi, &psi2, &st, &part, &prune, &tile, &atile, &athreads);
     if (rc0) return rc0;
+    const int adef = prec == KSV_F64 ? ADJ_TILE_F64 : ADJ_TILE_F32;
+    if (athreads == 512 && atile != adef) return adj_fail(KSV_ERR_RANGE, "512-thread adjoint passes need the default tile");
     if (prec != KSV_F64 && prec != KSV_F32) return adj_fail(KSV_ERR_TYPE, "bad precision");
     if (nparam && !grad) return adj_fail(KSV_ERR_TYPE, "null gradient buffer");
     if (nterms && (!xmask || !zmask || !coeff)) return adj_fail(KSV_ERR_TYPE, "null term buffer");
@@ -342,88 +535,72 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         hipFree(d_cr);
         hipFree(d_ci);
     }
-    // backward sweep: every parametrised op writes its per-block partial
-    // sums into its own slice of `slots`; one read-back at the end
-    std::vector<ksv_op> inv;
-    std::vector<double> im;
-    size_t end = nops;
-    const int GMAX = 1024;
-    double* slots = nullptr;
-    const size_t nslots = std::max<size_t>(recs.size(), 1);
-    AHIP(hipMalloc(&slots, sizeof(double) * 3 * GMAX * nslots));
-    struct Pending { int op; int g; size_t slot; };
-    std::vector<Pending> pend;
-    while (end > 0) {
-        // maximal run of parameter-free ops ending at `end`
-        size_t b = end;
-        while (b > 0 && !recs.count((int)(b - 1))) b--;
-        if (b < end) {
-            invert_tape(ops, b, end, mats, inv, im);
-            rc = adj_run_tape(s, inv.data(), inv.size(), im.data(), im.size(), 0, psi);
-            if (!rc) rc = adj_run_tape(s, inv.data(), inv.size(), im.data(), im.size(), 0, psi2);
-            if (rc) {
-                hipFree(slots);
-                return rc;
-            }
-            end = b;
-            if (end == 0) break;
+    // backward sweep: the inverse tape in tile passes over phi (psi) and
+    // lambda (psi2); every workgroup writes one partial per record, summed
+    // per record at the end
+    AdjPlan plan;
+    {
+        std::string err;
+        rc = plan_adjoint(n, atile, ops, nops, mats, nmats, params, nrec, plan, err);
+        if (rc) return adj_fail(rc, err);
+    }
+    const uint64_t ntiles = 1ull << (n - plan.k);
+    const size_t nr = std::max<size_t>(nrec, 1);
+    // workgroups per pass: enough to fill the chip, partials kept under 1 GiB
+    const int G = (int)std::max<uint64_t>(1, std::min<uint64_t>({ntiles, 2048, (1ull << 27) / nr}));
+    DevBuf blob;
+    const size_t off_ops = 0;
+    const size_t off_mats = align256(off_ops + sizeof(AOp) * plan.aops.size());
+    const size_t off_slot = align256(off_mats + sizeof(double) * plan.mats.size());
+    const size_t off_part = align256(off_slot + sizeof(int32_t) * plan.slot_rec.size());
+    const size_t off_sum = align256(off_part + sizeof(double) * nr * G);
+    const size_t bytes = off_sum + sizeof(double) * nr;
+    AHIP(blob.alloc(bytes));
+    {
+        std::vector<char> h(off_part, 0);
+        if (!plan.aops.empty()) std::memcpy(h.data() + off_ops, plan.aops.data(), sizeof(AOp) * plan.aops.size());
+        if (!plan.mats.empty()) std::memcpy(h.data() + off_mats, plan.mats.data(), sizeof(double) * plan.mats.size());
+        if (!plan.slot_rec.empty())
+            std::memcpy(h.data() + off_slot, plan.slot_rec.data(), sizeof(int32_t) * plan.slot_rec.size());
+        AHIP(hipMemcpyAsync(blob.p, h.data(), h.size(), hipMemcpyHostToDevice, st));
+        AHIP(hipStreamSynchronize(st));   // the host copy goes out of scope
+    }
+    char* dev = static_cast<char*>(blob.p);
+    double* d_part = reinterpret_cast<double*>(dev + off_part);
+    double* d_sum = reinterpret_cast<double*>(dev + off_sum);
+    for (const AdjStep& sp : plan.steps) {
+        if (sp.big) {
+            const ksv_op& w = plan.bops[sp.bop];
+            rc = adj_run_tape(s, &w, 1, plan.bmats.data(), plan.bmats.size(), 0, psi);
+            if (!rc) rc = adj_run_tape(s, &w, 1, plan.bmats.data(), plan.bmats.size(), 0, psi2);
+            if (rc) return rc;
+            continue;
         }
-        // op end-1 carries parameters
-        const int oi = (int)(end - 1);
-        const ksv_op& o = ops[oi];
-        const auto& rv = recs[oi];
-        AdjArgs a;
+        AdjPassArgs a;
         std::memset(&a, 0, sizeof a);
         a.phi = psi;
         a.lam = psi2;
-        a.nrec = (int)rv.size();
-        int dim;
-        std::vector<int> insb;
-        if (o.kind == KSV_SINGLE) { a.k = 1; a.t[0] = (int8_t)o.q[0]; dim = 2; insb = {o.q[0]}; }
-        else if (o.kind == KSV_CONTROLLED) {
-            a.k = 1; a.t[0] = (int8_t)o.q[1]; dim = 2; insb = {o.q[0], o.q[1]};
-            a.ctl = 1ull << o.q[0];
-        } else {
-            if (o.q[0] == o.q[1]) {
-                hipFree(slots);
-                return adj_fail(KSV_ERR_TYPE, "two-qubit gate needs distinct qubits");
-            }
-            a.k = 2; a.t[0] = (int8_t)o.q[0]; a.t[1] = (int8_t)o.q[1]; dim = 4; insb = {o.q[0], o.q[1]};
+        a.ops = reinterpret_cast<const AOp*>(dev + off_ops) + sp.op_begin;
+        a.mats = reinterpret_cast<const double*>(dev + off_mats);
+        a.slot_rec = reinterpret_cast<const int32_t*>(dev + off_slot) + sp.slot_begin;
+        a.part = d_part;
+        a.nops = sp.op_count;
+        a.nslot = sp.slot_count;
+        a.k = sp.k;
+        a.ntiles = ntiles;
+        for (int i = 0; i < 8; i++) {
+            a.tq_lo |= (uint64_t)(uint8_t)sp.tq[i] << (8 * i);
+            a.tq_hi |= (uint64_t)(uint8_t)sp.tq[8 + i] << (8 * i);
         }
-        std::sort(insb.begin(), insb.end());
-        a.nins = (int)insb.size();
-        for (size_t i = 0; i < insb.size(); i++) a.ins[i] = (int8_t)insb[i];
-        std::vector<cd> m((size_t)dim * dim), ad;
-        for (size_t j = 0; j < m.size(); j++) m[j] = cd(mats[o.mat + 2 * j], mats[o.mat + 2 * j + 1]);
-        adjoint_of(m, dim, ad);
-        for (size_t j = 0; j < ad.size(); j++) { a.ud[2 * j] = ad[j].real(); a.ud[2 * j + 1] = ad[j].imag(); }
-        for (size_t r = 0; r < rv.size(); r++)
-            for (int j = 0; j < 2 * dim * dim; j++) a.du[r][j] = mats[rv[r]->dmat + j];
-        const uint64_t groups = N >> a.nins;
-        int g = (int)std::min<uint64_t>((groups + 255) / 256, GMAX);
-        if (g < 1) g = 1;
-        double* out = slots + pend.size() * 3 * GMAX;
-        if (prec == KSV_F64) hipLaunchKernelGGL((k_adj<double2>), dim3(g), dim3(256), 0, st, a, groups, out);
-        else hipLaunchKernelGGL((k_adj<float2>), dim3(g), dim3(256), 0, st, a, groups, out);
-        if (hipGetLastError() != hipSuccess) {
-            hipFree(slots);
-            return adj_fail(KSV_ERR_DEVICE, "adjoint kernel launch failed");
-        }
-        pend.push_back({oi, g, pend.size()});
-        end = end - 1;
+        AHIP(launch_adjpass(prec, atile, athreads, G, a, st));
     }
-    std::vector<double> hp(3 * GMAX * nslots);
-    AHIP(hipMemcpyAsync(hp.data(), slots, 8 * hp.size(), hipMemcpyDeviceToHost, st));
+    if (nrec) {
+        hipLaunchKernelGGL(k_recsum, dim3((unsigned)nrec), dim3(256), 0, st, d_part, G, d_sum);
+        AHIP(hipGetLastError());
+    }
+    std::vector<double> sums(nr, 0.0);
+    AHIP(hipMemcpyAsync(sums.data(), d_sum, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
     AHIP(hipStreamSynchronize(st));
-    hipFree(slots);
-    for (const Pending& pd : pend) {
-        const auto& rv = recs[pd.op];
-        const double* h = hp.data() + pd.slot * 3 * GMAX;
-        for (size_t r = 0; r < rv.size(); r++) {
-            double sum = 0;
-            for (int b2 = 0; b2 < pd.g; b2++) sum += h[(size_t)b2 * 3 + r];
-            grad[rv[r]->param] += 2.0 * rv[r]->coeff * sum;
-        }
-    }
+    for (size_t r = 0; r < nrec; r++) grad[params[r].param] += 2.0 * params[r].coeff * sums[r];
     return KSV_OK;
 }

@@ -241,6 +241,72 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
                  int kernel, int tile, int rtile, bool fuse, int prune, std::string& err, int btile = 0,
                  int bq = 0);
 
+// ---------------------------------------------------------------------------
+// Adjoint backward pass: a workgroup stages the same K-qubit tile of phi and
+// of lambda in LDS and walks a run of the inverse tape.  Every op is applied
+// to both tiles; an op with parameter records also adds, per record j,
+// Re <lambda| dG_j |G^dag phi> over the tile into a per-workgroup slot.
+// Ops on disjoint qubits commute with each other's derivatives, so a pass
+// may take an op ahead of skipped ops it does not touch.
+// ---------------------------------------------------------------------------
+enum AKind : int32_t {
+    AK_DENSE = 0,   // 2^k x 2^k (k <= 3) G^dag on local bits t[0] (MSB) .. t[k-1]
+    AK_X = 1,       // X on local bit t[0] (CNOT, Toffoli; self-inverse)
+    AK_SWAP = 2,    // exchange local bits t[0], t[1] (SWAP, CSWAP; self-inverse)
+};
+
+struct AOp {
+    uint64_t gctl;     // controls on physical bits outside the tile
+    int32_t kind;
+    int32_t k;         // targets
+    int32_t nins;      // local bits inserted to enumerate the op's elements
+    int32_t nrec;      // parameter records (<= 3)
+    uint32_t lctl;     // local control bits (set after insertion)
+    int32_t mat;       // AK_DENSE: complex offset of G^dag in the pass matrices
+    int8_t t[4];
+    int8_t ins[8];     // ascending local positions of targets and local controls
+    int32_t dmat[3];   // complex offset of each record's dG (2^k x 2^k)
+    int16_t slot[3];   // pass-local slot of each record
+    uint8_t flags;     // AOP_DIAG: G^dag and every dG are diagonal
+    uint8_t run;       // AK_DENSE: ops of the chain this op heads (0: inside a chain)
+};
+static_assert(sizeof(AOp) == 64, "AOp layout");
+constexpr uint8_t AOP_DIAG = 1;
+// A chain is a run of AK_DENSE ops on the same targets (same order) and the
+// same controls, adjacent in the pass: the kernel applies the whole chain to
+// each element group while it sits in registers.
+constexpr int ACHAIN_1Q = 4;        // longest chain of 1-qubit ops
+constexpr int ACHAIN_2Q = 2;        // ... of 2-qubit ops
+
+constexpr int ADJ_SLOTS = 128;      // parameter records per pass
+constexpr int ADJ_DENSE_MAX = 3;    // widest dense op inside a pass
+constexpr int ADJ_TILE_F64 = 11;    // tile qubits: phi and lambda tiles take 2 x 32 KiB of LDS
+constexpr int ADJ_TILE_F32 = 12;
+
+struct AdjStep {
+    bool big = false;        // one wide inverse op through the general engine (bops[bop])
+    int k = 0;
+    int8_t tq[16] = {0};     // physical bit of tile bit i, ascending
+    int op_begin = 0, op_count = 0;       // AOps
+    int slot_begin = 0, slot_count = 0;   // slot_rec
+    int bop = -1;
+};
+
+struct AdjPlan {
+    int k = 0;
+    std::vector<AdjStep> steps;
+    std::vector<AOp> aops;
+    std::vector<double> mats;        // interleaved complex: G^dag and dG matrices
+    std::vector<int32_t> slot_rec;   // global parameter record of each pass slot
+    std::vector<ksv_op> bops;        // wide inverse ops (KSV_UNITARY with conj-transposed matrix)
+    std::vector<double> bmats;
+};
+
+// adjplan.cpp: the inverse of `ops` split into tile passes of K qubits.
+// params[r].op names the tape op of record r (already validated).
+int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err);
+
 // kernels (k_pass.hip / k_reduce.hip)
 hipError_t launch_pass(const Program& p, const Step& s, void* psi, hipStream_t st);
 hipError_t launch_big(const Program& p, const Step& s, void* psi, int prune, hipStream_t st);

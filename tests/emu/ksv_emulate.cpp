@@ -546,3 +546,249 @@ int emu_run(int n, int prune, const ksv_op* ops, size_t nops, const double* mats
 }
 
 }  // extern "C"
+
+// ---- adjoint backward sweep -------------------------------------------------
+
+static void emu_apply_dense(int n, const std::vector<int>& tg, const cd* m, cd* v) {
+    const int k = (int)tg.size(), dim = 1 << k;
+    uint64_t tmask = 0;
+    for (int q : tg) tmask |= 1ull << q;
+    std::vector<cd> x(dim);
+    std::vector<uint64_t> id(dim);
+    for (uint64_t b = 0; b < (1ull << n); b++) {
+        if (b & tmask) continue;
+        for (int r = 0; r < dim; r++) {
+            uint64_t g = b;
+            for (int i = 0; i < k; i++)
+                if ((r >> (k - 1 - i)) & 1) g |= 1ull << tg[i];
+            id[r] = g;
+            x[r] = v[g];
+        }
+        for (int r = 0; r < dim; r++) {
+            cd y = 0;
+            for (int c = 0; c < dim; c++) y += m[(size_t)r * dim + c] * x[c];
+            v[id[r]] = y;
+        }
+    }
+}
+
+// The backward sweep of ksv_adjoint as k_adjpass runs it: the same plan
+// (plan_adjoint), every pass tile by tile and every op element group by
+// element group on host copies of phi and lambda, wide steps applied
+// directly.  Every index is checked; every amplitude is loaded and stored
+// once per pass; every record is owned by exactly one pass.  sums[r] gets
+// Re <lambda| dG_r |G^dag phi> of record r.  stats: passes, wide steps,
+// ops in the largest pass, ops inside chains (not heads), diagonal ops.
+extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                           const ksv_param* params, size_t nrec, double* phi_io, double* lam_io, double* sums,
+                           int* stats) {
+    AdjPlan plan;
+    std::string err;
+    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err);
+    if (rc) {
+        g_msg = err;
+        return rc;
+    }
+    cd* phi = reinterpret_cast<cd*>(phi_io);
+    cd* lam = reinterpret_cast<cd*>(lam_io);
+    const uint64_t N = 1ull << n;
+    std::vector<int> owner(nrec, 0);
+    for (int32_t r : plan.slot_rec) {
+        CHECK(r >= 0 && (size_t)r < nrec, "slot names a missing record");
+        owner[r]++;
+    }
+    for (size_t r = 0; r < nrec; r++) CHECK(owner[r] == 1, "record not owned by exactly one pass");
+    for (size_t r = 0; r < nrec; r++) sums[r] = 0;
+    int passes = 0, bigs = 0, maxops = 0;
+    std::vector<char> seen(N);
+    for (const AdjStep& s : plan.steps) {
+        if (s.big) {
+            bigs++;
+            const ksv_op& w = plan.bops[s.bop];
+            CHECK(w.kind == KSV_UNITARY && w.nq > ADJ_DENSE_MAX && w.nq <= KSV_MAX_OPERANDS, "bad wide step");
+            CHECK(w.mat >= 0 && (size_t)w.mat + (2ull << (2 * w.nq)) <= plan.bmats.size(), "wide matrix out of range");
+            std::vector<int> tg(w.q, w.q + w.nq);
+            const cd* m = reinterpret_cast<const cd*>(plan.bmats.data() + w.mat);
+            emu_apply_dense(n, tg, m, phi);
+            emu_apply_dense(n, tg, m, lam);
+            continue;
+        }
+        passes++;
+        maxops = std::max(maxops, s.op_count);
+        const int k = s.k;
+        CHECK(k == plan.k && k >= 1 && k <= n && k <= 12, "bad tile width");
+        CHECK(s.slot_count <= ADJ_SLOTS && s.slot_begin + s.slot_count <= (int)plan.slot_rec.size(), "bad slots");
+        CHECK(s.op_begin + s.op_count <= (int)plan.aops.size(), "bad op range");
+        const uint32_t ts = 1u << k;
+        uint64_t tmask = 0;
+        for (int i = 0; i < k; i++) {
+            CHECK(s.tq[i] >= 0 && s.tq[i] < n, "tile bit out of range");
+            CHECK(i == 0 || s.tq[i] > s.tq[i - 1], "tile bits not ascending");
+            tmask |= 1ull << s.tq[i];
+        }
+        std::fill(seen.begin(), seen.end(), 0);
+        std::vector<cd> P(ts), L(ts);
+        std::vector<double> red(s.slot_count, 0.0);
+        for (uint64_t tile = 0; tile < (N >> k); tile++) {
+            uint64_t base = tile;
+            for (int i = 0; i < k; i++) base = ins0(base, s.tq[i]);
+            CHECK((base & tmask) == 0, "base overlaps the tile");
+            for (uint32_t l = 0; l < ts; l++) {
+                uint64_t off = base;
+                for (int i = 0; i < k; i++)
+                    if ((l >> i) & 1) off |= 1ull << s.tq[i];
+                CHECK(off < N && !seen[off], "tile load out of range or repeated");
+                seen[off] = 1;
+                P[l] = phi[off];
+                L[l] = lam[off];
+            }
+            for (int o = 0; o < s.op_count; o++) {
+                const AOp& op = plan.aops[s.op_begin + o];
+                CHECK((op.gctl & tmask) == 0, "global control inside the tile");
+                int run = 1;
+                if (op.kind == AK_DENSE) {
+                    run = op.run;
+                    const int cap = op.k == 1 ? ACHAIN_1Q : (op.k == 2 ? ACHAIN_2Q : 1);
+                    CHECK(run >= 1 && run <= cap && o + run <= s.op_count, "bad chain length");
+                    for (int c = 1; c < run; c++) {
+                        const AOp& m = plan.aops[s.op_begin + o + c];
+                        CHECK(m.kind == AK_DENSE && m.run == 0 && m.k == op.k && m.nins == op.nins &&
+                                  m.lctl == op.lctl && m.gctl == op.gctl &&
+                                  std::memcmp(m.t, op.t, sizeof op.t) == 0 && std::memcmp(m.ins, op.ins, sizeof op.ins) == 0,
+                              "chain member with another support");
+                    }
+                } else {
+                    CHECK(op.run == 0 || op.run == 1, "chain on a permutation");
+                }
+                if ((base & op.gctl) != op.gctl) {
+                    o += run - 1;
+                    continue;
+                }
+                CHECK(op.nins >= op.k && op.nins <= 5, "bad insertion count");
+                for (int i = 0; i < op.nins; i++) CHECK(op.ins[i] >= 0 && op.ins[i] < k && (i == 0 || op.ins[i] > op.ins[i - 1]), "bad insertion bits");
+                for (int b = 0; b < op.k; b++) CHECK(op.t[b] >= 0 && op.t[b] < k, "target outside the tile");
+                const uint32_t ne = ts >> op.nins;
+                for (uint32_t e = 0; e < ne; e++) {
+                    uint32_t l = (uint32_t)e;
+                    for (int i = 0; i < op.nins; i++) l = (uint32_t)ins0(l, op.ins[i]);
+                    l |= op.lctl;
+                    CHECK(l < ts, "element outside the tile");
+                    if (op.kind == AK_X) {
+                        const uint32_t l1 = l | (1u << op.t[0]);
+                        CHECK(l1 != l, "X target not inserted");
+                        std::swap(P[l], P[l1]);
+                        std::swap(L[l], L[l1]);
+                        continue;
+                    }
+                    if (op.kind == AK_SWAP) {
+                        const uint32_t la = l | (1u << op.t[0]), lb = l | (1u << op.t[1]);
+                        CHECK(la != l && lb != l, "swap targets not inserted");
+                        std::swap(P[la], P[lb]);
+                        std::swap(L[la], L[lb]);
+                        continue;
+                    }
+                    CHECK(op.kind == AK_DENSE && op.k <= ADJ_DENSE_MAX, "bad op kind");
+                    const int D = 1 << op.k;
+                    uint32_t id[8];
+                    cd x[8], y[8], a[8], b[8];
+                    for (int r = 0; r < D; r++) {
+                        uint32_t g = l;
+                        for (int bb = 0; bb < op.k; bb++)
+                            if ((r >> (op.k - 1 - bb)) & 1) g |= 1u << op.t[bb];
+                        id[r] = g;
+                        x[r] = P[g];
+                        y[r] = L[g];
+                    }
+                    // the chain, in registers
+                    for (int c = 0; c < run; c++) {
+                        const AOp& m = plan.aops[s.op_begin + o + c];
+                        const bool diag = (m.flags & AOP_DIAG) != 0;
+                        CHECK(m.mat >= 0 && 2 * ((size_t)m.mat + (size_t)D * D) <= plan.mats.size(), "matrix out of range");
+                        const cd* u = reinterpret_cast<const cd*>(plan.mats.data()) + m.mat;
+                        for (int r = 0; r < D; r++) {
+                            a[r] = 0;
+                            b[r] = 0;
+                            for (int cc = 0; cc < D; cc++) {
+                                if (diag && cc != r) {
+                                    CHECK(u[r * D + cc] == cd(0, 0), "diagonal flag on a dense matrix");
+                                    continue;
+                                }
+                                a[r] += u[r * D + cc] * x[cc];
+                                b[r] += u[r * D + cc] * y[cc];
+                            }
+                        }
+                        CHECK(m.nrec == 0 || m.k <= 2, "records on a wide op");
+                        for (int j = 0; j < m.nrec; j++) {
+                            CHECK(m.slot[j] >= 0 && m.slot[j] < s.slot_count, "bad slot");
+                            CHECK(m.dmat[j] >= 0 && 2 * ((size_t)m.dmat[j] + (size_t)D * D) <= plan.mats.size(), "derivative out of range");
+                            const cd* d = reinterpret_cast<const cd*>(plan.mats.data()) + m.dmat[j];
+                            double acc = 0;
+                            for (int r = 0; r < D; r++) {
+                                cd t = 0;
+                                for (int cc = 0; cc < D; cc++) {
+                                    if (diag && cc != r) {
+                                        CHECK(d[r * D + cc] == cd(0, 0), "diagonal flag on a dense derivative");
+                                        continue;
+                                    }
+                                    t += d[r * D + cc] * a[cc];
+                                }
+                                acc += (std::conj(y[r]) * t).real();
+                            }
+                            red[m.slot[j]] += acc;
+                        }
+                        for (int r = 0; r < D; r++) {
+                            x[r] = a[r];
+                            y[r] = b[r];
+                        }
+                    }
+                    for (int r = 0; r < D; r++) {
+                        P[id[r]] = x[r];
+                        L[id[r]] = y[r];
+                    }
+                }
+                o += run - 1;
+            }
+            for (uint32_t l = 0; l < ts; l++) {
+                uint64_t off = base;
+                for (int i = 0; i < k; i++)
+                    if ((l >> i) & 1) off |= 1ull << s.tq[i];
+                phi[off] = P[l];
+                lam[off] = L[l];
+            }
+        }
+        for (uint64_t i = 0; i < N; i++) CHECK(seen[i], "amplitude not covered by the pass");
+        for (int i = 0; i < s.slot_count; i++) sums[plan.slot_rec[s.slot_begin + i]] += red[i];
+    }
+    if (stats) {
+        stats[0] = passes;
+        stats[1] = bigs;
+        stats[2] = maxops;
+        stats[3] = stats[4] = 0;
+        for (const AOp& op : plan.aops) {
+            if (op.kind == AK_DENSE && op.run == 0) stats[3]++;
+            if (op.flags & AOP_DIAG) stats[4]++;
+        }
+    }
+    return 0;
+}
+
+// Shape of the adjoint plan only: passes, wide steps, ops in the largest pass.
+extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                                const ksv_param* params, size_t nrec, int* stats) {
+    AdjPlan plan;
+    std::string err;
+    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err);
+    if (rc) {
+        g_msg = err;
+        return rc;
+    }
+    stats[0] = stats[1] = stats[2] = 0;
+    for (const AdjStep& s : plan.steps) {
+        if (s.big) stats[1]++;
+        else {
+            stats[0]++;
+            stats[2] = std::max(stats[2], s.op_count);
+        }
+    }
+    return 0;
+}

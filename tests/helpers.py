@@ -84,3 +84,57 @@ def shot_calls(n, count, seed, measures=2, resets=1):
         at = int(rng.integers(1, len(calls)))
         calls.insert(at, ("reset", (int(rng.integers(0, n)),)))
     return calls
+
+
+def random_ansatz(n, count, nparam, seed, wide=True):
+    """A seeded ansatz p -> Circuit over ket's parametrised gates (angles affine
+    in the parameters, some shared between gates) mixed with fixed gates of
+    every kind, including 3-qubit and (with `wide`) 4-qubit unitaries."""
+    rnd = random.Random(seed)
+    plan = []
+    for _ in range(count):
+        r = rnd.random()
+        qs = rnd.sample(range(n), 4)
+        if r < 0.45:
+            name = rnd.choice(["rx", "ry", "rz", "p", "u1", "u3", "crx", "cry", "crz", "cu1", "cu3",
+                               "xx", "yy", "zz", "xy", "ms"])
+            k = {"u3": 3, "cu3": 3, "ms": 2}.get(name, 1)
+            width = 1 if name in ("rx", "ry", "rz", "p", "u1", "u3") else 2
+            angles = [(rnd.randrange(nparam), rnd.choice([1.0, -0.5, 2.0]), rnd.uniform(-1, 1)) for _ in range(k)]
+            plan.append((name, angles, qs[:width]))
+        elif r < 0.6:
+            plan.append((rnd.choice(SINGLE0), None, qs[:1]))
+        elif r < 0.75:
+            plan.append((rnd.choice(TWO0), None, qs[:2]))
+        elif r < 0.9:
+            plan.append((rnd.choice(THREE), None, qs[:3]))
+        else:
+            k = rnd.choice([2, 3, 4] if wide else [2, 3])
+            plan.append(("unitary", random_unitary(1 << k, rnd), qs[:k]))
+
+    def ansatz(p):
+        c = ket_amd.Circuit(n)
+        for name, extra, qs in plan:
+            if name == "unitary":
+                c = c.unitary(extra, *qs)
+            elif extra is None:
+                c = getattr(c, name)(*qs)
+            else:
+                c = getattr(c, name)(*[coef * p[i] + off for i, coef, off in extra], *qs)
+        return c
+    return ansatz
+
+
+def pauli_apply(n, psi, terms):
+    """sum_t c_t P_t psi from PauliTerm dicts (ops[0] acts on qubit n-1), by
+    applying the single-qubit Pauli matrices factor by factor."""
+    mats = {"X": np.array([[0, 1], [1, 0]], complex), "Y": np.array([[0, -1j], [1j, 0]]),
+            "Z": np.array([[1, 0], [0, -1]], complex)}
+    out = np.zeros_like(psi)
+    for t in terms:
+        v = psi.reshape([2] * n)          # axis a = qubit n-1-a
+        for a, ch in enumerate(t["ops"].upper()):
+            if ch != "I":
+                v = np.moveaxis(np.tensordot(mats[ch], v, axes=([1], [a])), 0, a)
+        out += t["coeff"] * v.reshape(-1)
+    return out

@@ -640,3 +640,27 @@ def test_qaoa_batch_state_and_finite_differences():
         fd = (batch.run([g], [bp])[0] - batch.run([g], [bm])[0]) / (2 * h)
         assert abs(fd - db[0][l]) < 1e-6
     batch.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,n,seed,tol", [("f64", 14, 11, 1e-7), ("f64", 13, 12, 1e-7), ("f32", 15, 13, 3e-3)])
+def test_adjoint_tiled_random_ansatz(precision, n, seed, tol):
+    """Device adjoint sweep (tile passes over phi and lambda, global controls, wide
+    steps) on a random ansatz over every parametrised gate kind: against central
+    finite differences of the oracle's energy (h = 1e-5), and bit-for-bit-close to
+    the host emulation of the same plan in FP64."""
+    from ket_amd.algorithms import energy_and_gradient
+    from test_adjoint_emu import _fd, _energy, emu_gradient
+    from helpers import random_ansatz
+    nparam = 7
+    ans = random_ansatz(n, 110, nparam, seed)
+    params = list(np.random.default_rng(seed).uniform(-2, 2, nparam))
+    H = workloads.tfim_hamiltonian(n)
+    ket_amd.configure(precision=precision, prune=False)
+    e, g = energy_and_gradient(ans, H, params)
+    fd = _fd(ans, params, H, range(nparam))
+    assert np.max(np.abs(np.array(g) - fd)) < tol, (g, fd)
+    assert abs(e - _energy(ans, params, H)) < (1e-10 if precision == "f64" else 1e-4)
+    if precision == "f64":
+        ge, _, _ = emu_gradient(ans, params, H, K=11)
+        assert np.max(np.abs(np.array(g) - ge)) < 1e-11
