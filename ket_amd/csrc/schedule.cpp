@@ -814,6 +814,25 @@ static void absorb(FBlock& f, const LOp& o) {
     }
 }
 
+// The dense matrix of a fused block from its ops: qubits join the block in
+// the order the ops first touch them (ascending within an op), row bit i =
+// qubit q[i].  The packings below record only which ops each block takes;
+// matrices are built for the chosen packing alone.
+static void block_matrix(int n, FBlock& f, const std::vector<LOp>& lops) {
+    f.q.clear();
+    f.u.assign(1, cd(1, 0));
+    uint64_t have = 0;
+    for (int i : f.ops) {
+        const LOp& o = lops[i];
+        std::vector<int> add;
+        for (int q = 0; q < n; q++)
+            if ((o.touch >> q & 1) && !(have >> q & 1)) add.push_back(q);
+        grow_block(f, add);
+        have = f.touch;
+        absorb(f, o);
+    }
+}
+
 static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector<FBlock>& blocks) {
     std::vector<int> last(n, -1);
     const int window = 96;       // candidate blocks searched per op
@@ -850,11 +869,7 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
             best = (int)blocks.size() - 1;
         }
         FBlock& f = blocks[best];
-        std::vector<int> add;
-        for (int q : S)
-            if (!(f.touch >> q & 1)) add.push_back(q);
-        grow_block(f, add);
-        absorb(f, o);
+        f.touch |= o.touch;
         f.ops.push_back((int)i);
         f.diag = f.diag && o.cls == L_DIAG;
         for (int q : S) last[q] = best;
@@ -936,16 +951,10 @@ static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::
         std::vector<int> got;
         scan(Q, -1, &got);
         FBlock f;
-        f.u.assign(1, cd(1, 0));
         for (int i : got) {
-            const LOp& o = lops[i];
-            std::vector<int> add;
-            for (int q = 0; q < n; q++)
-                if ((o.touch >> q & 1) && !(f.touch >> q & 1)) add.push_back(q);
-            grow_block(f, add);
-            absorb(f, o);
+            f.touch |= lops[i].touch;
             f.ops.push_back(i);
-            f.diag = f.diag && o.cls == L_DIAG;
+            f.diag = f.diag && lops[i].cls == L_DIAG;
             done[i] = 1;
         }
         blocks.push_back(f);
@@ -1088,6 +1097,11 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
         fuse_blocks_front(n, bq, lops, front, look);
         if (front.size() < blocks.size()) blocks.swap(front);
     }
+    for (FBlock& f : blocks)
+        if (f.lop < 0) {
+            f.touch = 0;
+            block_matrix(n, f, lops);
+        }
     // pack blocks into passes (a standalone op is a step of its own)
     std::vector<LOp> proxies(blocks.size());
     for (size_t i = 0; i < blocks.size(); i++) {
