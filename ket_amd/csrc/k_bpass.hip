@@ -4,16 +4,17 @@
 // non-tile bits equal its base index) in LDS.  For every block of the pass
 // the tile is a complex matrix X with 2^BQ rows (the block's local index,
 // bit i = block qubit i) and 2^(K-BQ) columns (the other tile bits), and the
-// block computes Y = U X as one real GEMM on the real embedding
+// block computes Y = U X with Gauss's three real products
 //
-//     [Yr]   [Ur  -Ui] [Xr]
-//     [Yi] = [Ui   Ur] [Xi]
+//     T1 = Ur Xr,  T2 = Ui Xi,  T3 = (Ur + Ui)(Xr + Xi)
+//     Yr = T1 - T2,  Yi = T3 - T1 - T2
 //
-// with v_mfma_{f64,f32}_16x16x4: each wave takes 16-column slices, reads its
-// B fragments (lane l: column l%16, rows 4t + l/16) from LDS, accumulates
-// 2^(BQ+1)/16 output tiles over 2^(BQ+1)/4 k-steps, and writes the result
-// back in place (lane l, element v: row 4v + l/16 for f64, 4(l/16) + v
-// for f32).  A fragments come pre-arranged
+// (three quarters of the MFMA work of the real embedding
+// [[Ur, -Ui], [Ui, Ur]]) on v_mfma_{f64,f32}_16x16x4: each wave takes
+// 16-column slices, reads its B values (lane l: column l%16, rows 4t + l/16)
+// from LDS, accumulates 2^BQ/16 output tiles of each product over 2^BQ/4
+// k-steps, and writes the result back in place (lane l, element v: row
+// 4v + l/16 for f64, 4(l/16) + v for f32).  A fragments come pre-arranged
 // per lane from the host.  A block whose ops are all diagonal is instead one
 // phase per row: its fragments hold the diagonal and each lane multiplies
 // the amplitudes it reads in place.  Columns differ between waves, so blocks need a
@@ -70,8 +71,9 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
     constexpr int NW = NT / 64;         // waves
     constexpr int LNT = NT == 512 ? 9 : 8;
     constexpr int R = 1 << BQ;          // complex rows
-    constexpr int MB = 2 * R / 16;      // output 16-row tiles (real embedding)
-    constexpr int KS = 2 * R / 4;       // k-steps
+    constexpr int MB = R / 16;          // output 16-row tiles of each product
+    constexpr int KS = R / 4;           // k-steps
+    constexpr int NF = 3 * MB * KS;     // A fragments: Ur, Ui, Ur + Ui
     constexpr int NC = 1 << (K - BQ);   // columns
     constexpr int NBW = NC / 16 / NW;   // 16-column slices per wave
     constexpr int PER = (1 << K) / NT;  // amplitudes per thread at load / store
@@ -123,9 +125,9 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
         }
         const int64_t foff = (int64_t)w[6];
         const bool diag = (w[7] & BREC_DIAG) != 0;     // a phase per row (fragments hold the diagonal)
-        T af[MB * KS];
+        T af[NF];
 #pragma unroll
-        for (int f = 0; f < MB * KS; f++) af[f] = frags[foff + f * 64 + lane];
+        for (int f = 0; f < NF; f++) af[f] = frags[foff + f * 64 + lane];
 
         // read rows r = 4t + rl: bits 0,1 from rl, the rest from t
         const uint32_t rd_l = ((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u);
@@ -163,25 +165,29 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
                 }
                 continue;
             }
-            acc_t d[MB];
+            acc_t t1[MB], t2[MB], t3[MB];
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++) d[mb] = acc_t{0, 0, 0, 0};
+            for (int mb = 0; mb < MB; mb++) t1[mb] = t2[mb] = t3[mb] = acc_t{0, 0, 0, 0};
 #pragma unroll
             for (int kk = 0; kk < KS; kk++) {
-                const T bv = kk < R / 4 ? x[kk].x : x[kk - R / 4].y;
+                const T xr = x[kk].x, xi = x[kk].y, xs = xr + xi;
 #pragma unroll
-                for (int mb = 0; mb < MB; mb++) d[mb] = M::step(af[mb * KS + kk], bv, d[mb]);
+                for (int mb = 0; mb < MB; mb++) {
+                    t1[mb] = M::step(af[mb * KS + kk], xr, t1[mb]);
+                    t2[mb] = M::step(af[(MB + mb) * KS + kk], xi, t2[mb]);
+                    t3[mb] = M::step(af[(2 * MB + mb) * KS + kk], xs, t3[mb]);
+                }
             }
 #pragma unroll
-            for (int h = 0; h < R / 16; h++) {
+            for (int h = 0; h < MB; h++) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     uint32_t wa = wr_l ^ (LOW ? (((v & 1) ? ar[2] : 0u) ^ ((v & 2) ? ar[3] : 0u))
                                               : (((v & 1) ? ar[0] : 0u) ^ ((v & 2) ? ar[1] : 0u)));
                     if (h) wa ^= ar[4];
                     V y;
-                    y.x = d[h][v];
-                    y.y = d[h + R / 16][v];
+                    y.x = t1[h][v] - t2[h][v];
+                    y.y = t3[h][v] - t1[h][v] - t2[h][v];
                     lds[wa ^ col] = y;
                 }
             }

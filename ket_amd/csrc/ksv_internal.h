@@ -96,7 +96,8 @@ static_assert(sizeof(ROp) == 128, "ROp layout");
 // ---------------------------------------------------------------------------
 // Fused-block pass: the circuit is fused on the host into dense blocks of
 // BQ (4 or 5) qubits; a pass stages a K-qubit tile in LDS and applies its
-// blocks as real GEMMs on MFMA:  [Yr; Yi] = [[Ur, -Ui], [Ui, Ur]] [Xr; Xi],
+// blocks as complex GEMMs on MFMA with Gauss's three real products:
+// Yr = Ur Xr - Ui Xi, Yi = (Ur + Ui)(Xr + Xi) - Ur Xr - Ui Xi,
 // rows = the block's local index (bit i = qubit q[i]), columns = the other
 // K - BQ tile bits.
 // ---------------------------------------------------------------------------
@@ -231,8 +232,10 @@ int lower_tape(int n, const ksv_op* ops, size_t nops, const double* mats, size_t
 void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog);
 void schedule_rounds(int n, int rtile, bool fuse, std::vector<LOp>& lops, Program& prog);
 void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& prog);
-// A fragments of one block for MFMA 16x16x4: [mb][kk][lane] real values of
-// the real embedding of the 2^bq x 2^bq matrix u (row-major).
+// A fragments of one block for MFMA 16x16x4 (2^(2 bq + 3) / 16 values, the
+// size of the real embedding's fragments): Ur, Ui and Ur + Ui of the
+// 2^bq x 2^bq matrix u (row-major), each [mb][kk][lane]; for a diagonal
+// block, the diagonal (fragments 2t / 2t + 1).
 void block_fragments(const cd* u, int bq, bool diag, double* out);
 // Lower and schedule a tape into `p` (host side only).  kernel: KSV_OPT_PASS_KERNEL
 // value; tile / rtile / btile: LDS, register-round and fused-block tile widths;

@@ -975,15 +975,19 @@ void block_fragments(const cd* u, int bq, bool diag, double* out) {
             }
         return;
     }
-    auto ue = [&](int i, int k) -> double {     // real embedding [[Ur, -Ui], [Ui, Ur]]
-        const int r = i % R, c = k % R;
-        const cd v = u[(size_t)r * R + c];
-        if (i < R) return k < R ? v.real() : -v.imag();
-        return k < R ? v.imag() : v.real();
-    };
-    for (int mb = 0; mb < MB; mb++)
-        for (int kk = 0; kk < KS; kk++)
-            for (int l = 0; l < 64; l++) out[((size_t)mb * KS + kk) * 64 + l] = ue(16 * mb + l % 16, 4 * kk + l / 16);
+    // Gauss's three products: fragments of Ur, Ui and Ur + Ui (which = 0, 1, 2), each
+    // [mb][kk][lane] with lane l holding M[16 mb + l % 16][4 kk + l / 16]; the
+    // kernel forms Yr = Ur Xr - Ui Xi, Yi = (Ur + Ui)(Xr + Xi) - Ur Xr - Ui Xi
+    const int MB3 = R / 16, KS3 = R / 4;
+    std::fill(out, out + (size_t)MB * KS * 64, 0.0);
+    for (int which = 0; which < 3; which++)
+        for (int mb = 0; mb < MB3; mb++)
+            for (int kk = 0; kk < KS3; kk++)
+                for (int l = 0; l < 64; l++) {
+                    const cd v = u[(size_t)(16 * mb + l % 16) * R + 4 * kk + l / 16];
+                    out[(((size_t)which * MB3 + mb) * KS3 + kk) * 64 + l] =
+                        which == 0 ? v.real() : which == 1 ? v.imag() : v.real() + v.imag();
+                }
 }
 
 // Pass swizzle: tile bit p >= 4 XORs v[p] into the low four address bits.

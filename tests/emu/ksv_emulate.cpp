@@ -403,21 +403,25 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
                             x[l][t] = lds[a];
                         }
                     }
-                    // MFMA accumulate
-                    std::vector<std::vector<double>> D(MB, std::vector<double>(256, 0.0));   // D[mb][i*16+j]
-                    for (int kk = 0; kk < KS; kk++)
-                        for (int mb = 0; mb < MB; mb++)
-                            for (int i = 0; i < 16; i++)
-                                for (int j = 0; j < 16; j++) {
-                                    double acc = 0;
-                                    for (int k = 0; k < 4; k++) {
-                                        const double av = af[((size_t)mb * KS + kk) * 64 + (i + 16 * k)];
-                                        const int lb = j + 16 * k;
-                                        const double bv = kk < R / 4 ? x[lb][kk].real() : x[lb][kk - R / 4].imag();
-                                        acc += av * bv;
+                    // MFMA accumulate: Gauss's three products T1 = Ur Xr, T2 = Ui Xi,
+                    // T3 = (Ur + Ui)(Xr + Xi), fragments [which][mb][kk][lane]
+                    const int MB3 = R / 16, KS3 = R / 4;
+                    std::vector<std::vector<std::vector<double>>> T(
+                        3, std::vector<std::vector<double>>(MB3, std::vector<double>(256, 0.0)));   // T[w][mb][i*16+j]
+                    for (int w = 0; w < 3; w++)
+                        for (int kk = 0; kk < KS3; kk++)
+                            for (int mb = 0; mb < MB3; mb++)
+                                for (int i = 0; i < 16; i++)
+                                    for (int j = 0; j < 16; j++) {
+                                        double acc = 0;
+                                        for (int k = 0; k < 4; k++) {
+                                            const double av = af[(((size_t)w * MB3 + mb) * KS3 + kk) * 64 + (i + 16 * k)];
+                                            const cd xv = x[j + 16 * k][kk];
+                                            const double bv = w == 0 ? xv.real() : w == 1 ? xv.imag() : xv.real() + xv.imag();
+                                            acc += av * bv;
+                                        }
+                                        T[w][mb][i * 16 + j] += acc;
                                     }
-                                    D[mb][i * 16 + j] += acc;
-                                }
                     for (int l = 0; l < 64; l++) {
                         const uint32_t rl = l >> 4;
                         for (int h = 0; h < R / 16; h++)
@@ -430,7 +434,8 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
                                 CHECK(a < (1u << K), "block write address out of range");
                                 wr[a]++;
                                 const int i = 4 * v + (int)rl, j = l & 15;
-                                out[a] = cd(D[h][i * 16 + j], D[h + R / 16][i * 16 + j]);
+                                const double t1 = T[0][h][i * 16 + j], t2 = T[1][h][i * 16 + j], t3 = T[2][h][i * 16 + j];
+                                out[a] = cd(t1 - t2, t3 - t1 - t2);
                             }
                     }
                 }
