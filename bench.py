@@ -256,6 +256,15 @@ def adjoint_section(n=24, reps=24, psr_sample=24, fd_sample=8):
             "realAmplitudes_adjoint_s": round(t_ra, 4), "realAmplitudes_params": n * (reps + 1)}
 
 
+def workload_adjoint(args):
+    """The 24-qubit adjoint section as its own line (gradient evaluations per second)."""
+    sec = adjoint_section()
+    return {"metric": "full adjoint gradients/s (energy + 1200-parameter gradient)", "value": sec["adjoint_grads_per_s"],
+            "unit": "gradients/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": round(sec["adjoint_s"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": sec["workload"]}, "adjoint": sec}
+
+
 def workload_noisy(args):
     """run() with device noise: the 12-qubit, 20-layer random circuit (830 gates) plus
     a measurement of every qubit into a register, 'harmony' error rates, 10 000
@@ -389,11 +398,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with one 16 GiB device-to-device state copy (HBM counter calibration)")
     ap.add_argument("--shots", type=int, default=0, help="noisy workload: shots (default 10000)")
-    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy"],
+    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy", "adjoint"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
                          "with accuracy against FP64; qft: qft(n) on a seeded basis state; qaoa: 512 batched "
                          "Max-Cut QAOA instances with adjoint gradients; noisy: run() with device noise, "
-                         "batched trajectories")
+                         "batched trajectories; adjoint: the 24-qubit adjoint-gradient section alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -402,7 +411,7 @@ def main():
 
     if args.workload != "random":
         line = {"f32": workload_f32, "qft": workload_qft, "qaoa": workload_qaoa,
-                "noisy": workload_noisy}[args.workload](args)
+                "noisy": workload_noisy, "adjoint": workload_adjoint}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
 
