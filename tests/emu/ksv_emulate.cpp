@@ -777,7 +777,8 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
     return 0;
 }
 
-// Shape of the adjoint plan only: passes, wide steps, ops in the largest pass.
+// Shape of the adjoint plan only: passes, wide steps, ops in the largest pass,
+// kernel steps over all passes.
 extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                                 const ksv_param* params, size_t nrec, int* stats) {
     AdjPlan plan;
@@ -787,12 +788,18 @@ extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, co
         g_msg = err;
         return rc;
     }
-    stats[0] = stats[1] = stats[2] = 0;
+    stats[0] = stats[1] = stats[2] = stats[3] = 0;
     for (const AdjStep& s : plan.steps) {
         if (s.big) stats[1]++;
         else {
             stats[0]++;
             stats[2] = std::max(stats[2], s.op_count);
+            // kernel steps (one LDS round trip and barrier each): chain heads, permutations
+            for (int o = 0; o < s.op_count; o++) {
+                const AOp& op = plan.aops[s.op_begin + o];
+                stats[3]++;
+                o += op.kind == AK_DENSE ? op.run - 1 : 0;
+            }
         }
     }
     return 0;
