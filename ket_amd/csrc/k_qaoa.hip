@@ -47,6 +47,8 @@ struct QArgs {
     const double* gamma;
     const double* beta;
     double* part;              // partial sums of this launch (2 per work item)
+    const uint8_t* ll;         // per instance: cut of the low 12 qubits for each of the 4096 tile indices
+    const double* phs;         // per instance and layer: exp(i gamma S / 2) for each cut value (re, im)
     int layer;
     int p;
     int mode;                  // QM_*
@@ -55,6 +57,7 @@ struct QArgs {
 enum QMode { QM_INIT = 0, QM_FWD_A = 1, QM_FWD_B = 2, QM_ENERGY = 3, QM_BWD_B = 4, QM_BWD_A = 5 };
 
 constexpr int QNT = 256, QR = 16;
+constexpr int QPMAX = 232;     // phase-table entries per (instance, layer): cut values 0..231 (n <= 22)
 
 template <typename T> struct QV2;
 template <> struct QV2<double> { using type = double2; };
@@ -173,12 +176,8 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     if (tab) {
         const uint32_t th = (uint32_t)(t << 12);      // the tile's high bits
         const uint32_t hm = (uint32_t)(((1ull << q.n) - 1) & ~0xFFFull);
-        for (uint32_t e = tid; e < 4096; e += QNT) {
-            int c = 0;
-#pragma unroll
-            for (int u = 0; u < 12; u++) c += __popc(adj[u] & 0xFFFu & (((e >> u) & 1) ? ~e : e));
-            ll[e] = (uint8_t)(c >> 1);
-        }
+        // the instance's low-low cut table (k_qll), 16 bytes per thread
+        reinterpret_cast<uint4*>(ll)[tid] = reinterpret_cast<const uint4*>(a.ll + (size_t)ii * 4096)[tid];
         int hh = 0;
         for (int u = 12; u < q.n; u++) hh += __popc(adj[u] & hm & (((th >> u) & 1) ? ~th : th));
         cut_hh = hh >> 1;
@@ -188,13 +187,13 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
             a01[u] = a0 | (a1 << 16);
         }
         if (a.mode != QM_ENERGY) {
-            const double g = a.gamma[q.gbase + a.layer] * (a.mode == QM_BWD_A ? -0.5 : 0.5);
+            // this layer's phases (k_qphs); the backward pass takes their conjugates
+            const double* ph = a.phs + ((size_t)q.gbase + a.layer) * QPMAX * 2;
+            const double sg = a.mode == QM_BWD_A ? -1.0 : 1.0;
             for (int c = (int)tid; c <= q.nedges; c += QNT) {
-                double sn, cs;
-                sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);
                 V v;
-                v.x = (T)cs;
-                v.y = (T)sn;
+                v.x = (T)ph[2 * c];
+                v.y = (T)(sg * ph[2 * c + 1]);
                 phs[c] = v;
             }
         }
@@ -339,6 +338,35 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     }
 }
 
+// Cut of the low 12 qubits (the edges among them) for each of the 4096
+// tile indices of instance blockIdx.x.
+__global__ __launch_bounds__(256) void k_qll(const QInst* inst, uint8_t* ll) {
+    const QInst& q = inst[blockIdx.x];
+    uint32_t adj[12];
+#pragma unroll
+    for (int u = 0; u < 12; u++) adj[u] = q.adj[u] & 0xFFFu;
+    for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
+        int c = 0;
+#pragma unroll
+        for (int u = 0; u < 12; u++) c += __popc(adj[u] & (((e >> u) & 1) ? ~e : e));
+        ll[(size_t)blockIdx.x * 4096 + e] = (uint8_t)(c >> 1);
+    }
+}
+
+// exp(i gamma_l S / 2), S = |E| - 2 cut, for every cut value of instance
+// blockIdx.x and layer blockIdx.y (S / 2 the phase of cnot . rz(-gamma) . cnot per edge).
+__global__ __launch_bounds__(256) void k_qphs(const QInst* inst, const double* gamma, double* phs) {
+    const QInst& q = inst[blockIdx.x];
+    const double g = gamma[q.gbase + blockIdx.y] * 0.5;
+    double* out = phs + ((size_t)q.gbase + blockIdx.y) * QPMAX * 2;
+    for (int c = (int)threadIdx.x; c <= q.nedges; c += 256) {
+        double sn, cs;
+        sincos(g * (double)(q.nedges - 2 * c), &sn, &cs);
+        out[2 * c] = cs;
+        out[2 * c + 1] = sn;
+    }
+}
+
 // out[i * stride + k], k < 2: per-instance sums of the two partials of a launch
 __global__ __launch_bounds__(256) void k_qseg(const double* part, const uint64_t* wstart, int ninst, double* out,
                                               int stride) {
@@ -382,6 +410,8 @@ struct ksv_batch {
     uint64_t* d_wstart = nullptr;
     double *d_gamma = nullptr, *d_beta = nullptr;
     double* d_part = nullptr;     // (2p + 1) launches x 2 x nwork
+    uint8_t* d_ll = nullptr;      // count x 4096 low-qubit cut table
+    double* d_phs = nullptr;      // count x p x QPMAX phases (re, im)
     double* d_out = nullptr;      // count x (2p + 1) x 2
     std::vector<QInst> inst;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -414,6 +444,8 @@ static int qlaunch(ksv_batch* b, int mode, int layer, double* part) {
     a.gamma = b->d_gamma;
     a.beta = b->d_beta;
     a.part = part;
+    a.ll = b->d_ll;
+    a.phs = b->d_phs;
     a.layer = layer;
     a.p = b->p;
     a.mode = mode;
@@ -426,6 +458,8 @@ static int qlaunch(ksv_batch* b, int mode, int layer, double* part) {
 // the whole forward (+ backward) schedule; launch slot k writes d_part + 2 nwork k
 static int qschedule(ksv_batch* b, bool grad) {
     const int P = b->p;
+    hipLaunchKernelGGL(k_qphs, dim3(b->count, P), dim3(256), 0, b->st, b->d_inst, b->d_gamma, b->d_phs);
+    QHIP(hipGetLastError());
     int rc = qlaunch(b, QM_INIT, 0, b->d_part);
     for (int l = 0; l < P && !rc; l++) {
         rc = qlaunch(b, QM_FWD_A, l, b->d_part);
@@ -522,6 +556,11 @@ int ksv_qaoa_batch_create(int count, const int32_t* nq, const int64_t* edge_off,
     QHIP(hipMemcpy(b->d_inst, b->inst.data(), sizeof(QInst) * count, hipMemcpyHostToDevice));
     QHIP(hipMemcpy(b->d_winst, winst.data(), 2 * winst.size(), hipMemcpyHostToDevice));
     QHIP(hipMemcpy(b->d_wstart, wstart.data(), 8 * (count + 1), hipMemcpyHostToDevice));
+    QHIP(hipMalloc(&b->d_ll, 4096 * (size_t)count));
+    QHIP(hipMalloc(&b->d_phs, 16 * (size_t)count * p * QPMAX));
+    hipLaunchKernelGGL(k_qll, dim3(count), dim3(256), 0, b->st, b->d_inst, b->d_ll);
+    QHIP(hipGetLastError());
+    QHIP(hipStreamSynchronize(b->st));
     *out = b.release();
     return KSV_OK;
 }
@@ -607,6 +646,8 @@ int ksv_qaoa_batch_destroy(ksv_batch* b) {
     hipFree(b->d_beta);
     hipFree(b->d_part);
     hipFree(b->d_out);
+    hipFree(b->d_ll);
+    hipFree(b->d_phs);
     hipStreamDestroy(b->st);
     delete b;
     return KSV_OK;
