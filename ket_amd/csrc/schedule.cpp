@@ -10,6 +10,7 @@
 // non-diagonal targets are tile qubits; controls and diagonal qubits may lie
 // anywhere (outside the tile they are constant per workgroup).
 #include <algorithm>
+#include <array>
 #include <cstring>
 
 #include "ksv_internal.h"
@@ -474,16 +475,17 @@ static uint32_t fill_slots(int K, uint32_t S) {
     return S;
 }
 
-static int gf2_rank(std::vector<uint32_t> v) {
+// Rank over GF(2) of the first m (<= 4) words of v (v is a copy).
+static int gf2_rank(std::array<uint32_t, 4> v, int m) {
     int r = 0;
     for (int bit = 31; bit >= 0; bit--) {
         int piv = -1;
-        for (size_t i = r; i < v.size(); i++)
-            if (v[i] >> bit & 1) { piv = (int)i; break; }
+        for (int i = r; i < m; i++)
+            if (v[i] >> bit & 1) { piv = i; break; }
         if (piv < 0) continue;
         std::swap(v[r], v[piv]);
-        for (size_t i = 0; i < v.size(); i++)
-            if ((int)i != r && (v[i] >> bit & 1)) v[i] ^= v[r];
+        for (int i = 0; i < m; i++)
+            if (i != r && (v[i] >> bit & 1)) v[i] ^= v[r];
         r++;
     }
     return r;
@@ -498,21 +500,31 @@ static void emit_exch(int K, int TB, const RLayout& W, const RLayout& R, Program
     uint32_t v[16] = {0};
     auto addr = [&](int p) -> uint32_t { return (1u << p) ^ (p >= 4 ? v[p] : 0u); };
     auto good = [&](const RLayout& L) {
-        std::vector<uint32_t> a3, a4;
+        std::array<uint32_t, 4> a3{}, a4{};
+        int m3 = 0, m4 = 0;
         for (int i = 0; i < 4 && i < TB; i++) {
             uint32_t a = addr(L.thr[i]);
-            if (i < 3) a3.push_back(a & 7);
-            a4.push_back(a & 15);
+            if (i < 3) a3[m3++] = a & 7;
+            a4[m4++] = a & 15;
         }
-        return gf2_rank(a3) == (int)a3.size() && gf2_rank(a4) == (int)a4.size();
+        return gf2_rank(a3, m3) == m3 && gf2_rank(a4, m4) == m4;
     };
     std::vector<int> free_p;
     for (int i = 0; i < 4 && i < TB; i++) {
         if (W.thr[i] >= 4 && std::find(free_p.begin(), free_p.end(), W.thr[i]) == free_p.end()) free_p.push_back(W.thr[i]);
         if (R.thr[i] >= 4 && std::find(free_p.begin(), free_p.end(), R.thr[i]) == free_p.end()) free_p.push_back(R.thr[i]);
     }
+    // The search cannot succeed when a layout has tile bit 3 among its first three
+    // thread bits (its address has no low three bits, whatever the swizzle), or
+    // when no thread bit can be swizzled; it then ends with v = 0, as below.
+    auto possible = [&](const RLayout& L) {
+        for (int i = 0; i < 3 && i < TB; i++)
+            if (L.thr[i] == 3) return false;
+        return true;
+    };
+    const bool search = !free_p.empty() && possible(W) && possible(R);
     uint32_t rng = 0x9E3779B9u;
-    for (int attempt = 0; attempt < 4096 && !(good(W) && good(R)); attempt++) {
+    for (int attempt = 0; search && attempt < 4096 && !(good(W) && good(R)); attempt++) {
         for (int p : free_p) {
             rng ^= rng << 13;
             rng ^= rng >> 17;
@@ -998,12 +1010,13 @@ static uint64_t choose_swizzle(int K, const std::vector<std::vector<int>>& colbi
     uint32_t v[16] = {0};
     auto good = [&]() {
         for (const auto& cb : colbits) {
-            std::vector<uint32_t> a;
+            std::array<uint32_t, 4> a{};
+            int m = 0;
             for (size_t i = 0; i < cb.size() && i < 4; i++) {
                 const int p = cb[i];
-                a.push_back(p < 4 ? (1u << p) : v[p]);
+                a[m++] = p < 4 ? (1u << p) : v[p];
             }
-            if (gf2_rank(a) != (int)a.size()) return false;
+            if (gf2_rank(a, m) != m) return false;
         }
         return true;
     };
@@ -1013,7 +1026,7 @@ static uint64_t choose_swizzle(int K, const std::vector<std::vector<int>>& colbi
             if (cb[i] >= 4 && std::find(free_p.begin(), free_p.end(), cb[i]) == free_p.end()) free_p.push_back(cb[i]);
     uint32_t rng = 0x2545F491u;
     bool ok = good();
-    for (int attempt = 0; attempt < 4096 && !ok; attempt++) {
+    for (int attempt = 0; attempt < 4096 && !ok && !free_p.empty(); attempt++) {
         for (int p : free_p) {
             rng ^= rng << 13;
             rng ^= rng >> 17;
