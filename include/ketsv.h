@@ -278,9 +278,15 @@ int ksv_xorshift32(uint32_t* state, size_t count, double* out);
 
 /* A whole batch in one call: the gate tape with events between its ops.
  * Event i happens after ops [0, at): a Pauli list (copy[off..off+count),
- * pauli[...]) on qubit q, or a measurement / reset of qubit q with draws
- * r[off..off+B) and outcomes written to outcome[off..off+B).  Positions
- * ascend; everything is validated before the state changes.              */
+ * pauli[...], each copy at most once) on qubit q, or a measurement / reset
+ * of qubit q with draws r[off..off+B) and outcomes written to
+ * outcome[off..off+B).  Positions ascend; everything is validated before
+ * the state changes.  When a copy fits one LDS tile (n <= 12 FP64, 13 FP32)
+ * and the pass kernel is automatic or KSV_OPT_PASS_KERNEL 1, the gates
+ * between measurements / resets run as one LDS pass per segment (each
+ * workgroup one copy) with the Pauli lists as ops on their copies' tiles;
+ * a single state of such a size takes the same LDS kernel with
+ * KSV_OPT_PASS_KERNEL 1, which gives it bit-identical amplitudes.          */
 #define KSV_SHOT_PAULI   0
 #define KSV_SHOT_MEASURE 1
 #define KSV_SHOT_RESET   2

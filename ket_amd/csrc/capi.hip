@@ -220,8 +220,8 @@ static int plan_into(Program& p, int n, int prec, const ksv_op* ops, size_t nops
 }
 
 // Serialise a planned program's device arrays (ROps, DOps, BRecs, block
-// fragments, matrices in the state's precision) into `blob` at 256-byte
-// aligned offsets; off[] receives the five offsets (SIZE_MAX = absent).
+// fragments, matrices in the state's precision, copy bitmaps) into `blob` at
+// 256-byte aligned offsets; off[] receives the six offsets (SIZE_MAX = absent).
 static void program_blob(const ksv_state* s, const Program& p, std::vector<char>& blob, std::vector<size_t>& off) {
     auto put = [&](const void* src, size_t bytes) -> size_t {
         if (!bytes) return SIZE_MAX;
@@ -230,7 +230,7 @@ static void program_blob(const ksv_state* s, const Program& p, std::vector<char>
         std::memcpy(blob.data() + at, src, bytes);
         return at;
     };
-    off.assign(5, SIZE_MAX);
+    off.assign(6, SIZE_MAX);
     off[0] = put(p.rops.data(), sizeof(ROp) * p.rops.size());
     off[1] = put(p.dops.data(), sizeof(DOp) * p.dops.size());
     off[2] = put(p.brecs.data(), sizeof(BRec) * p.brecs.size());
@@ -268,6 +268,7 @@ static void program_blob(const ksv_state* s, const Program& p, std::vector<char>
             off[4] = put(f.data(), 8 * p.mats.size());
         }
     }
+    off[5] = put(p.bits.data(), sizeof(uint32_t) * p.bits.size());
 }
 
 static void bind_blob(Program& p, char* base, const std::vector<size_t>& off) {
@@ -277,6 +278,7 @@ static void bind_blob(Program& p, char* base, const std::vector<size_t>& off) {
     p.d_brecs = (BRec*)at(off[2]);
     p.d_frags = at(off[3]);
     p.d_mats = at(off[4]);
+    p.d_bits = (uint32_t*)at(off[5]);
 }
 
 static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
@@ -946,34 +948,75 @@ int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const dou
             return fail(KSV_ERR_TYPE, "unknown shot event kind");
         }
     }
-    // plan every gate segment between events on the host, then one upload
+    // Per-copy mode: a copy fits one LDS tile, so each gate segment is one LDS
+    // pass in which every workgroup runs one copy, the Pauli events inside it as
+    // ops on their copies' tiles; only measurements and resets split segments.
+    // Otherwise every event ends a segment and Paulis are launches of their own.
+    const bool copy_mode = (s->kernel == 0 || s->kernel == 1) && n <= (s->prec == KSV_F64 ? 12 : 13);
+    const size_t nwords = (copies + 31) / 32;
+    // plan every segment on the host, then one upload
+    struct Action { int seg; int ev; };             // run program `seg`, or event `ev`
+    std::vector<Action> acts;
     std::vector<std::unique_ptr<ksv_program>> seg;
     std::vector<std::vector<size_t>> segoff;
     std::vector<char> blob;
+    std::vector<CopyPauli> pend;
     size_t cur = 0;
-    for (size_t i = 0; i <= nev; i++) {
-        const size_t upto = i < nev ? (size_t)ev[i].at : nops;
-        if (upto > cur) {
-            std::unique_ptr<ksv_program> p(new ksv_program());
-            rc = plan_into(*p, s->n, s->prec, ops + cur, upto - cur, mats, nmats, s->kernel, default_tile(s), s->rtile,
-                           s->fuse != 0, s->prune, s->btile, s->bq);
-            if (rc) return rc;
-            p->dev = s->dev;
-            p->bthreads = s->bthreads;
-            p->borrowed = true;
-            std::vector<size_t> off;
-            std::vector<char> one;
-            program_blob(s, *p, one, off);
-            const size_t base = (blob.size() + 255) & ~(size_t)255;
-            blob.resize(base + one.size());
-            std::memcpy(blob.data() + base, one.data(), one.size());
-            for (size_t& o : off)
-                if (o != SIZE_MAX) o += base;
-            seg.push_back(std::move(p));
-            segoff.push_back(off);
-            cur = upto;
+    auto close = [&](size_t upto) -> int {
+        if (upto <= cur && pend.empty()) return KSV_OK;
+        std::unique_ptr<ksv_program> p(new ksv_program());
+        int prc;
+        if (copy_mode) {
+            std::string err;
+            prc = plan_copies(*p, s->n, n, s->prec, ops + cur, upto - cur, mats, nmats, pend, s->prune, err);
+            if (prc) return fail(prc, err);
+        } else {
+            prc = plan_into(*p, s->n, s->prec, ops + cur, upto - cur, mats, nmats, s->kernel, default_tile(s),
+                            s->rtile, s->fuse != 0, s->prune, s->btile, s->bq);
+            if (prc) return prc;
         }
+        p->dev = s->dev;
+        p->bthreads = s->bthreads;
+        p->borrowed = true;
+        std::vector<size_t> off;
+        std::vector<char> one;
+        program_blob(s, *p, one, off);
+        const size_t base = (blob.size() + 255) & ~(size_t)255;
+        blob.resize(base + one.size());
+        std::memcpy(blob.data() + base, one.data(), one.size());
+        for (size_t& o : off)
+            if (o != SIZE_MAX) o += base;
+        acts.push_back({(int)seg.size(), -1});
+        seg.push_back(std::move(p));
+        segoff.push_back(off);
+        cur = upto;
+        pend.clear();
+        return KSV_OK;
+    };
+    for (size_t i = 0; i < nev; i++) {
+        const ksv_shot_event& e = ev[i];
+        if (copy_mode && e.kind == KSV_SHOT_PAULI) {
+            // one conditional op per Pauli kind present, over a bitmap of its copies
+            for (int pk = 1; pk <= 3; pk++) {
+                CopyPauli c;
+                c.at = (size_t)e.at - cur;
+                c.q = e.q;
+                c.pauli = pk;
+                c.words.assign(nwords, 0u);
+                bool any = false;
+                for (uint64_t j = e.off; j < e.off + e.count; j++)
+                    if (pauli[j] == pk) {
+                        c.words[copy[j] >> 5] |= 1u << (copy[j] & 31);
+                        any = true;
+                    }
+                if (any) pend.push_back(std::move(c));
+            }
+            continue;
+        }
+        if ((rc = close((size_t)e.at))) return rc;
+        acts.push_back({-1, (int)i});
     }
+    if ((rc = close(nops))) return rc;
     DevBufs b;
     char* d_blob = nullptr;
     uint32_t* d_copy = nullptr;
@@ -983,24 +1026,19 @@ int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const dou
         HIPCHK(hipMemcpy(d_blob, blob.data(), blob.size(), hipMemcpyHostToDevice));
     }
     for (size_t i = 0; i < seg.size(); i++) bind_blob(*seg[i], d_blob, segoff[i]);
-    if (npauli) {
+    if (npauli && !copy_mode) {
         HIPCHK(b.get(&d_copy, npauli));
         HIPCHK(b.get(&d_pauli, npauli));
         HIPCHK(hipMemcpy(d_copy, copy, 4 * npauli, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_pauli, pauli, npauli, hipMemcpyHostToDevice));
     }
-    size_t si = 0;
-    cur = 0;
-    for (size_t i = 0; i <= nev; i++) {
-        const size_t upto = i < nev ? (size_t)ev[i].at : nops;
-        if (upto > cur) {
-            rc = run_program_on(s, seg[si].get(), s->psi, false);
+    for (const Action& act : acts) {
+        if (act.seg >= 0) {
+            rc = run_program_on(s, seg[act.seg].get(), s->psi, false);
             if (rc) return rc;
-            si++;
-            cur = upto;
+            continue;
         }
-        if (i == nev) break;
-        const ksv_shot_event& e = ev[i];
+        const ksv_shot_event& e = ev[act.ev];
         if (e.kind == KSV_SHOT_PAULI) {
             if (e.count)
                 HIPCHK(shots_pauli(n, s->prec, s->psi, e.q, d_copy + e.off, d_pauli + e.off, (uint32_t)e.count, s->st));

@@ -32,6 +32,10 @@ __device__ __forceinline__ uint64_t scatter(uint32_t l, const PassArgs& a, int l
     return off;
 }
 
+template <typename V> struct PScal;
+template <> struct PScal<double2> { using type = double; };
+template <> struct PScal<float2> { using type = float; };
+
 template <typename V, int KT>
 __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     __shared__ V T[1 << KT];
@@ -42,7 +46,18 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     const int k = a.k;
     const uint32_t tsize = 1u << k;
     V* psi = reinterpret_cast<V*>(a.psi);
-    const V* mats = reinterpret_cast<const V*>(a.mats);
+    // op records, matrices and copy bitmaps through the constant address
+    // space: uniform reads become scalar loads
+    using S = typename PScal<V>::type;
+    const KSV_CONST S* cm = to_const(reinterpret_cast<const S*>(a.mats));
+    auto mat = [&](int i) {
+        V r;
+        r.x = cm[2 * i];
+        r.y = cm[2 * i + 1];
+        return r;
+    };
+    const KSV_CONST uint64_t* cops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
+    const KSV_CONST uint32_t* cbits = to_const(a.bits);
 
     uint64_t base = blockIdx.x;
 #pragma unroll
@@ -58,13 +73,23 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
     __syncthreads();
 
     for (int o = 0; o < a.nops; o++) {
-        const DOp op = a.ops[o];
+        DOp op;
+        {
+            uint64_t w[sizeof(DOp) / 8];
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(DOp) / 8); i++) w[i] = cops[(sizeof(DOp) / 8) * o + i];
+            __builtin_memcpy(&op, w, sizeof op);
+        }
         if ((base & op.gctl) != op.gctl) continue;
+        if (op.cond) {      // only on the tiles of the copies in the op's bitmap
+            const uint64_t copy = base >> a.nsys;
+            if (!((cbits[op.cond - 1 + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
+        }
         const uint32_t ne = tsize >> op.nins;
         switch (op.kind) {
         case DK_M1: {
-            const V m0 = mats[op.mat], m1 = mats[op.mat + 1];
-            const V m2 = mats[op.mat + 2], m3 = mats[op.mat + 3];
+            const V m0 = mat(op.mat), m1 = mat(op.mat + 1);
+            const V m2 = mat(op.mat + 2), m3 = mat(op.mat + 3);
             const uint32_t tb = 1u << op.t[0];
             for (uint32_t e = tid; e < ne; e += nt) {
                 uint32_t l0 = op_elem(e, op), l1 = l0 | tb;
@@ -78,7 +103,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
             const uint32_t ba = 1u << op.t[0], bb = 1u << op.t[1];
             V m[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) m[i] = mats[op.mat + i];
+            for (int i = 0; i < 16; i++) m[i] = mat(op.mat + i);
             for (uint32_t e = tid; e < ne; e += nt) {
                 uint32_t l = op_elem(e, op);
                 const uint32_t id[4] = {l, l | bb, l | ba, l | ba | bb};
@@ -122,7 +147,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
 #pragma unroll
                         for (int c = 0; c < (1 << MK_MAX); c++) {
                             if (c < dim) {
-                                V mm = mats[op.mat + r * dim + c];
+                                V mm = mat(op.mat + r * dim + c);
                                 acc.x += mm.x * x[c].x - mm.y * x[c].y;
                                 acc.y += mm.x * x[c].y + mm.y * x[c].x;
                             }
@@ -145,7 +170,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
 #pragma unroll
                 for (int b = 0; b < DIAG_MAX; b++)
                     if (b < kk && op.t[b] >= 0 && ((l >> op.t[b]) & 1)) r |= 1u << (kk - 1 - b);
-                T[l] = prune_amp(cmul(mats[op.mat + (int)r], T[l]), a.prune);
+                T[l] = prune_amp(cmul(mat(op.mat + (int)r), T[l]), a.prune);
             }
             break;
         }
@@ -229,7 +254,8 @@ static hipError_t launch_pass_t(const Program& p, const Step& s, void* psi, hipS
     a.nops = s.op_count;
     a.k = s.k;
     a.prune = p.prune;
-    a.pad = 0;
+    a.nsys = p.nsys;
+    a.bits = p.d_bits;
     for (int i = 0; i < 16; i++) a.tq[i] = s.tq[i];
     const uint64_t blocks = 1ull << (p.n - s.k);
     int nt = 256;

@@ -34,7 +34,8 @@ struct DOp {
     int8_t  t[8];     // targets
     int8_t  ins[8];   // ascending local positions of the inserted bits
     uint32_t lctl;    // local control bits (set after insertion)
-    uint32_t pad;
+    uint32_t cond;    // 0, or 1 + word offset of a bitmap over copies: the op applies to a
+                      // tile only if the bit of its copy (base >> nsys) is set
     uint64_t gctl;    // global control mask: the op applies to a tile only if all set
 };
 static_assert(sizeof(DOp) == 48, "DOp layout");
@@ -144,8 +145,9 @@ struct PassArgs {
     int nops;
     int k;          // tile qubits of this pass
     int prune;
-    int pad;
+    int nsys;       // copy of a tile = base >> nsys (conditional ops)
     int8_t tq[16];  // physical bit of tile-local bit i (ascending)
+    const uint32_t* bits;   // copy bitmaps of conditional ops
 };
 
 // ---------------------------------------------------------------------------
@@ -162,6 +164,7 @@ struct LOp {
     uint64_t need = 0;       // qubits that must be tile-local
     int src = -1;            // tape index
     bool big = false;        // applied by the standalone unitary kernel
+    int cond = 0;            // 0, or 1 + word offset of the copy bitmap it is applied under (DOp::cond)
 };
 
 struct Step {
@@ -202,6 +205,9 @@ struct Program {
     BRec* d_brecs = nullptr;
     void* d_frags = nullptr;
     int bthreads = 256;              // fused-block passes: workgroup size (256 or 512)
+    int nsys = 0;                    // batched copies: qubits of one copy (conditional ops)
+    std::vector<uint32_t> bits;      // copy bitmaps of conditional ops
+    uint32_t* d_bits = nullptr;
     std::vector<cd> mats;
     // re-parameterisation (ksv_program_set_matrix): the tape, its lowered
     // ops and where each lowered op landed
@@ -309,6 +315,20 @@ struct AdjPlan {
 // params[r].op names the tape op of record r (already validated).
 int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
                  const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err);
+
+// Batched copies (ksv_shots_run): a copy of nsys qubits fits one LDS tile, so one
+// LDS pass over the ntot-qubit state runs the whole gate segment on every copy,
+// each workgroup one copy.  A Pauli event (copies given by a bitmap of
+// ceil(2^(ntot - nsys) / 32) words) is an op applied only to the tiles of its
+// copies; it goes before segment op `at`.
+struct CopyPauli {
+    size_t at;
+    int q;
+    int pauli;                      // 1 X, 2 Y, 3 Z
+    std::vector<uint32_t> words;
+};
+int plan_copies(Program& p, int ntot, int nsys, int prec, const ksv_op* ops, size_t nops, const double* mats,
+                size_t nmats, const std::vector<CopyPauli>& paulis, int prune, std::string& err);
 
 // kernels (k_pass.hip / k_reduce.hip)
 hipError_t launch_pass(const Program& p, const Step& s, void* psi, hipStream_t st);

@@ -336,6 +336,7 @@ static void emit_dop(const LOp& o, const int* pos, Program& prog) {
     for (int i = 0; i < nins; i++) d.ins[i] = (int8_t)ins[i];
     d.lctl = lctl;
     d.gctl = gctl;
+    d.cond = (uint32_t)o.cond;
     d.mat = (int32_t)prog.mats.size();
     prog.mats.insert(prog.mats.end(), o.m.begin(), o.m.end());
     prog.dops.push_back(d);
@@ -428,6 +429,57 @@ static void schedule_lds(int n, int tile, std::vector<LOp>& lops, Program& prog)
         emit_lds_pass(n, K, T, lops, chosen, prog);
         rem.swap(rest);
     }
+}
+
+int plan_copies(Program& p, int ntot, int nsys, int prec, const ksv_op* ops, size_t nops, const double* mats,
+                size_t nmats, const std::vector<CopyPauli>& paulis, int prune, std::string& err) {
+    if (nsys < 1 || nsys > KMAX_TILE || nsys > ntot) {
+        err = "a copy must fit one LDS tile";
+        return KSV_ERR_RANGE;
+    }
+    std::vector<LOp> gl;
+    int rc = lower_tape(nsys, ops, nops, mats, nmats, gl, err);
+    if (rc) return rc;
+    // the Pauli events in order, each before the first lowered op of tape op `at`
+    std::vector<LOp> lops;
+    lops.reserve(gl.size() + paulis.size());
+    size_t e = 0;
+    auto flush = [&](size_t upto) {
+        for (; e < paulis.size() && paulis[e].at <= upto; e++) {
+            const CopyPauli& c = paulis[e];
+            LOp o;
+            o.tg = {c.q};
+            if (c.pauli == 1) {
+                o.cls = L_PERMX;
+            } else if (c.pauli == 3) {
+                o.cls = L_DIAG;
+                o.m = {cd(1, 0), cd(-1, 0)};
+            } else {
+                o.cls = L_MAT;
+                o.m = {cd(0, 0), cd(0, -1), cd(0, 1), cd(0, 0)};
+            }
+            finish(o);
+            o.cond = (int)p.bits.size() + 1;
+            p.bits.insert(p.bits.end(), c.words.begin(), c.words.end());
+            lops.push_back(o);
+        }
+    };
+    for (LOp& o : gl) {
+        flush((size_t)o.src);
+        lops.push_back(std::move(o));
+        lops.back().src = -1;
+    }
+    flush(nops);
+    p.n = ntot;
+    p.nsys = nsys;
+    p.precision = prec;
+    p.prune = prune;
+    p.tile = nsys;
+    p.tape_ops = nops;
+    // every op lies inside the tile of one copy: in order, one LDS pass (plus
+    // a step of its own for each unitary too wide for a pass)
+    schedule_lds(ntot, nsys, lops, p);
+    return KSV_OK;
 }
 
 void schedule(int n, int tile, bool fuse, std::vector<LOp>& lops, Program& prog) {

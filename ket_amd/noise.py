@@ -128,7 +128,11 @@ class _Shot:
 def run_per_shot(circ, shots, rng, noise, init, creg_counts):
     """Per-shot simulation; returns (counts {index: n}, creg_counts)."""
     from .circuit import OPTIONS
-    st = State(circ.qubits, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+    # a state that fits one LDS tile runs on LDS passes, the kernel the batched
+    # copies use (ksv_shots_run), so both drivers give bit-identical states
+    small = circ.qubits <= (12 if OPTIONS["precision"] == "f64" else 13)
+    st = State(circ.qubits, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"],
+               kernel="lds" if small else "auto")
     counts = {}
     pm = (noise or {}).get("pMeas", 0) or 0
     ops = circ._ops()

@@ -235,6 +235,11 @@ static int emu_lds(const Program& p, const Step& s, cd* psi, int n) {
         for (int o = 0; o < s.op_count; o++) {
             const DOp& op = p.dops[s.op_begin + o];
             if ((base & op.gctl) != op.gctl) continue;
+            if (op.cond) {
+                const uint64_t copy = base >> p.nsys;
+                CHECK(op.cond - 1 + (copy >> 5) < p.bits.size(), "copy bitmap out of range");
+                if (!((p.bits[op.cond - 1 + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
+            }
             const cd* m = p.mats.data() + op.mat;
             const uint32_t ne = tsize >> op.nins;
             for (uint32_t e = 0; e < ne; e++) {
@@ -802,5 +807,61 @@ extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, co
             }
         }
     }
+    return 0;
+}
+
+// Batched copies as ksv_shots_run plans them in per-copy mode: plan_copies
+// for one segment (Pauli events given as (at, q, pauli, copy) entries, one
+// copy per entry, grouped into bitmaps here exactly as the library does),
+// then every step emulated on psi (2^ntot amplitudes).  stats: passes, LDS
+// tile qubits of the first pass.
+extern "C" int emu_copies(int ntot, int nsys, int prune, const ksv_op* ops, size_t nops, const double* mats,
+                          size_t nmats, const uint64_t* ev_at, const int32_t* ev_q, const int32_t* ev_pauli,
+                          const uint32_t* ev_copy, size_t nent, double* psi_io, int* stats) {
+    const uint64_t copies = 1ull << (ntot - nsys);
+    const size_t nwords = (size_t)((copies + 31) / 32);
+    std::vector<CopyPauli> pend;
+    // entries with equal (at, q) form one event; one bitmap per Pauli kind, X, Y, Z
+    for (size_t i = 0; i < nent;) {
+        size_t j = i;
+        while (j < nent && ev_at[j] == ev_at[i] && ev_q[j] == ev_q[i]) j++;
+        for (int pk = 1; pk <= 3; pk++) {
+            CopyPauli c;
+            c.at = ev_at[i];
+            c.q = ev_q[i];
+            c.pauli = pk;
+            c.words.assign(nwords, 0u);
+            bool any = false;
+            for (size_t k = i; k < j; k++)
+                if (ev_pauli[k] == pk) {
+                    CHECK(ev_copy[k] < copies, "copy out of range");
+                    c.words[ev_copy[k] >> 5] |= 1u << (ev_copy[k] & 31);
+                    any = true;
+                }
+            if (any) pend.push_back(std::move(c));
+        }
+        i = j;
+    }
+    Program p;
+    std::string err;
+    int rc = plan_copies(p, ntot, nsys, KSV_F64, ops, nops, mats, nmats, pend, prune, err);
+    if (rc) {
+        g_msg = err;
+        return rc;
+    }
+    cd* psi = reinterpret_cast<cd*>(psi_io);
+    int passes = 0;
+    for (const Step& s : p.steps) {
+        int r;
+        if (s.big) r = emu_big(p, s, psi, ntot);
+        else {
+            CHECK(!s.reg && !s.blk, "per-copy plan with a non-LDS pass");
+            if (passes == 0 && stats) stats[1] = s.k;
+            passes++;
+            r = emu_lds(p, s, psi, ntot);
+        }
+        if (r) return r;
+    }
+    if (stats) stats[0] = passes;
     return 0;
 }
