@@ -91,7 +91,25 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
             const V m0 = mat(op.mat), m1 = mat(op.mat + 1);
             const V m2 = mat(op.mat + 2), m3 = mat(op.mat + 3);
             const uint32_t tb = 1u << op.t[0];
-            for (uint32_t e = tid; e < ne; e += nt) {
+            uint32_t e = tid;
+            // four disjoint pairs per iteration: all their LDS reads before any write
+            for (; e + 3 * nt < ne; e += 4 * nt) {
+                uint32_t l0[4], l1[4];
+                V x0[4], x1[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    l0[u] = op_elem(e + u * nt, op);
+                    l1[u] = l0[u] | tb;
+                    x0[u] = T[l0[u]];
+                    x1[u] = T[l1[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    T[l0[u]] = prune_amp(cmadd2(m0, x0[u], m1, x1[u]), a.prune);
+                    T[l1[u]] = prune_amp(cmadd2(m2, x0[u], m3, x1[u]), a.prune);
+                }
+            }
+            for (; e < ne; e += nt) {
                 uint32_t l0 = op_elem(e, op), l1 = l0 | tb;
                 V x0 = T[l0], x1 = T[l1];
                 T[l0] = prune_amp(cmadd2(m0, x0, m1, x1), a.prune);
@@ -164,6 +182,23 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
 #pragma unroll
             for (int b = 0; b < DIAG_MAX; b++)
                 if (b < kk && op.t[b] < 0 && ((base >> (-op.t[b] - 1)) & 1)) rg |= 1u << (kk - 1 - b);
+            if (kk <= 2) {
+                // at most four entries: uniform loads, selected per element
+                const V d0 = mat(op.mat);
+                const V d1 = kk >= 1 ? mat(op.mat + 1) : d0;
+                const V d2 = kk == 2 ? mat(op.mat + 2) : d0;
+                const V d3 = kk == 2 ? mat(op.mat + 3) : d0;
+                for (uint32_t e = tid; e < ne; e += nt) {
+                    uint32_t l = op_elem(e, op);
+                    uint32_t r = rg;
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+                        if (b < kk && op.t[b] >= 0 && ((l >> op.t[b]) & 1)) r |= 1u << (kk - 1 - b);
+                    const V dv = r == 0 ? d0 : r == 1 ? d1 : r == 2 ? d2 : d3;
+                    T[l] = prune_amp(cmul(dv, T[l]), a.prune);
+                }
+                break;
+            }
             for (uint32_t e = tid; e < ne; e += nt) {
                 uint32_t l = op_elem(e, op);
                 uint32_t r = rg;
