@@ -302,6 +302,29 @@ def test_batched_shots_equal_per_shot_path():
     assert a.cregs == b.cregs
 
 
+@pytest.mark.parametrize("n,precision,wide", [(13, "f64", False), (13, "f32", False), (10, "f32", False),
+                                               (9, "f64", True)])
+def test_batched_shots_equal_per_shot_path_variants(n, precision, wide):
+    """Batched and shot-by-shot drivers agree exactly: copies that fit one LDS tile
+    (per-copy passes; FP32 up to 13 qubits), 13 FP64 qubits (segmented passes), and a
+    5-qubit unitary that splits a per-copy segment."""
+    import random
+    from helpers import random_unitary
+    calls = shot_calls(n, 50, 11 + n, measures=2, resets=1)
+    if wide:
+        calls.insert(20, ("unitary", (random_unitary(32, random.Random(5)), 0, 2, 4, 6, 8)))
+    ket_amd.configure(precision=precision)
+    c = build(n, calls, cregs={"c": 2})
+    a = c.run(shots=150, seed=31, noise="harmony")
+    old = ket_amd.configure(batch_shots=False)
+    try:
+        b = c.run(shots=150, seed=31, noise="harmony")
+    finally:
+        ket_amd.configure(**old)
+    assert list(a.probs.items()) == list(b.probs.items())
+    assert a.cregs == b.cregs
+
+
 def test_batched_shots_in_several_batches(monkeypatch):
     """More shots than copies per batch: the draw stream continues across batches."""
     import ket_amd.noise as N
