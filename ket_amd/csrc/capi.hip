@@ -948,11 +948,12 @@ int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const dou
             return fail(KSV_ERR_TYPE, "unknown shot event kind");
         }
     }
-    // Per-copy mode: a copy fits one LDS tile, so each gate segment is one LDS
-    // pass in which every workgroup runs one copy, the Pauli events inside it as
-    // ops on their copies' tiles; only measurements and resets split segments.
+    // Per-copy mode: a copy of at most 13 qubits fits one tile, so each gate
+    // segment is one pass (register rounds for 11..13 qubits, LDS below) in which
+    // every workgroup runs one copy, the Pauli events inside it as ops on their
+    // copies' tiles; only measurements and resets split segments.
     // Otherwise every event ends a segment and Paulis are launches of their own.
-    const bool copy_mode = (s->kernel == 0 || s->kernel == 1) && n <= (s->prec == KSV_F64 ? 12 : 13);
+    const bool copy_mode = s->kernel <= 2 && n <= 13;
     const size_t nwords = (copies + 31) / 32;
     // plan every segment on the host, then one upload
     struct Action { int seg; int ev; };             // run program `seg`, or event `ev`

@@ -301,6 +301,7 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     const T* wide_mats = reinterpret_cast<const T*>(a.mats);
     const KSV_CONST T* cmats = to_const(reinterpret_cast<const T*>(a.mats));
     const KSV_CONST uint64_t* ops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
+    const KSV_CONST uint32_t* cbits = to_const(a.bits);
 
     uint64_t tile = blockIdx.x;
     if (tile >= a.ntiles) return;
@@ -341,6 +342,10 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
                 continue;
             }
             if ((base & gctl) != gctl) continue;
+            if ((h1 >> 16) & ROP_COND) {    // only on the tiles of the copies in the op's bitmap
+                const uint64_t copy = base >> a.nsys;
+                if (!((cbits[mat + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
+            }
             const bool lp = (tid & tctl) == tctl;
             const bool sel = tctl != 0;
             switch (kind) {
@@ -416,6 +421,8 @@ hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t 
     a.st_t_hi = tb > 8 ? pack(s.st_t + 8, tb - 8) : 0;
     a.ld_r = (uint32_t)pack(s.ld_r, RSLOTS);
     a.st_r = (uint32_t)pack(s.st_r, RSLOTS);
+    a.bits = p.d_bits;
+    a.nsys = p.nsys;
     if (p.precision == KSV_F64) {
         if (p.prune) launch_tb<double, true>(a, tb, st);
         else launch_tb<double, false>(a, tb, st);

@@ -84,7 +84,8 @@ struct ROp {
     uint8_t k;
     uint8_t loc[6];    // R_DIAG, MSB first: 0..15 thread bit, 16+s slot s, 32+g physical bit g outside the tile;
                        // R_DENSE: the target slots, MSB first
-    uint16_t pad;
+    uint16_t flags;    // ROP_COND: applied only to tiles whose copy (base >> nsys) has its bit
+                       // set in the bitmap at word `mat` (R_M1, R_PERMX, R_DIAG with k <= 2)
     uint32_t tctl;     // thread-bit controls (mask over the thread index)
     uint32_t rctl;     // slot controls (mask over the register index)
     int32_t mat;       // complex offset into the program matrices (R_DENSE, R_DIAG with k > 2)
@@ -93,6 +94,7 @@ struct ROp {
                        // R_EXCH: XRec
 };
 static_assert(sizeof(ROp) == 128, "ROp layout");
+constexpr uint16_t ROP_COND = 1;
 
 // ---------------------------------------------------------------------------
 // Fused-block pass: the circuit is fused on the host into dense blocks of
@@ -136,6 +138,8 @@ struct RPassArgs {
     uint64_t ld_t_lo, ld_t_hi;    // physical bit of thread bit i at load
     uint64_t st_t_lo, st_t_hi;    // ... at store
     uint32_t ld_r, st_r;          // physical bit of slot s at load / store
+    const uint32_t* bits;         // copy bitmaps of conditional ops
+    int nsys;                     // copy of a tile = base >> nsys
 };
 
 struct PassArgs {

@@ -476,9 +476,19 @@ int plan_copies(Program& p, int ntot, int nsys, int prec, const ksv_op* ops, siz
     p.prune = prune;
     p.tile = nsys;
     p.tape_ops = nops;
-    // every op lies inside the tile of one copy: in order, one LDS pass (plus
-    // a step of its own for each unitary too wide for a pass)
-    schedule_lds(ntot, nsys, lops, p);
+    // every op lies inside the tile of one copy: in order, one pass (plus a
+    // step of its own for each unitary too wide for a pass) of register rounds
+    // for copies of 11..13 qubits, an LDS pass below
+    if (nsys >= 11 && nsys <= 13) {
+        for (const LOp& o : lops)
+            if (o.cond && !(o.cls == L_PERMX || o.cls == L_MAT || (o.cls == L_DIAG && o.tg.size() <= 2))) {
+                err = "copy-conditional op of an unexpected kind";
+                return KSV_ERR_TYPE;
+            }
+        schedule_rounds(ntot, nsys, false, lops, p);
+    } else {
+        schedule_lds(ntot, nsys, lops, p);
+    }
     return KSV_OK;
 }
 
@@ -666,6 +676,10 @@ static void emit_rop(const LOp& o, const RLayout& L, const int* pos, Program& pr
         r.b = (uint8_t)std::min(sa, sb);
         break;
     }
+    }
+    if (o.cond) {   // copy-conditional (Pauli events): kinds whose matrix is inline
+        r.flags = ROP_COND;
+        r.mat = o.cond - 1;
     }
     prog.rops.push_back(r);
 }

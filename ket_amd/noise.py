@@ -128,11 +128,12 @@ class _Shot:
 def run_per_shot(circ, shots, rng, noise, init, creg_counts):
     """Per-shot simulation; returns (counts {index: n}, creg_counts)."""
     from .circuit import OPTIONS
-    # a state that fits one LDS tile runs on LDS passes, the kernel the batched
-    # copies use (ksv_shots_run), so both drivers give bit-identical states
-    small = circ.qubits <= (12 if OPTIONS["precision"] == "f64" else 13)
-    st = State(circ.qubits, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"],
-               kernel="lds" if small else "auto")
+    # a state of at most 13 qubits runs on the kernel and tile the batched copies
+    # use (ksv_shots_run: register rounds on an n-qubit tile for 11..13 qubits, LDS
+    # passes below), so both drivers give bit-identical states
+    n = circ.qubits
+    kernel, rtile = ("lds", 0) if n <= 10 else ("reg", n) if n <= 13 else ("auto", 0)
+    st = State(n, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"], kernel=kernel, rtile=rtile)
     counts = {}
     pm = (noise or {}).get("pMeas", 0) or 0
     ops = circ._ops()

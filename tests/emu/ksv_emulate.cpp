@@ -101,8 +101,14 @@ static int emu_reg(const Program& p, const Step& s, cd* psi, int n) {
                 continue;
             }
             if ((base & op.gctl) != op.gctl) continue;
-            CHECK(op.mat >= 0 && (size_t)op.mat <= p.mats.size(), "matrix offset out of range");
-            const cd* wide = p.mats.data() + op.mat;
+            if (op.flags & ROP_COND) {      // `mat` is the copy bitmap's word offset
+                CHECK(op.kind == R_M1 || op.kind == R_PERMX || (op.kind == R_DIAG && op.k <= 2), "conditional op of a kind with a matrix offset");
+                const uint64_t copy = base >> p.nsys;
+                CHECK(op.mat >= 0 && (size_t)op.mat + (copy >> 5) < p.bits.size(), "copy bitmap out of range");
+                if (!((p.bits[(size_t)op.mat + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
+            }
+            CHECK((op.flags & ROP_COND) || (op.mat >= 0 && (size_t)op.mat <= p.mats.size()), "matrix offset out of range");
+            const cd* wide = p.mats.data() + ((op.flags & ROP_COND) ? 0 : op.mat);
             cd inl[4];
             {
                 const double* d = reinterpret_cast<const double*>(op.pay);
@@ -855,10 +861,10 @@ extern "C" int emu_copies(int ntot, int nsys, int prune, const ksv_op* ops, size
         int r;
         if (s.big) r = emu_big(p, s, psi, ntot);
         else {
-            CHECK(!s.reg && !s.blk, "per-copy plan with a non-LDS pass");
+            CHECK(!s.blk, "per-copy plan with a fused-block pass");
             if (passes == 0 && stats) stats[1] = s.k;
             passes++;
-            r = emu_lds(p, s, psi, ntot);
+            r = s.reg ? emu_reg(p, s, psi, ntot) : emu_lds(p, s, psi, ntot);
         }
         if (r) return r;
     }

@@ -1,5 +1,6 @@
-"""Batched copies in per-copy mode (ksv_shots_run with a copy that fits one LDS
-tile): the plan of one gate segment with Pauli events on chosen copies,
+"""Batched copies in per-copy mode (ksv_shots_run with a copy that fits one
+tile: register rounds for 11..13 qubits, an LDS pass below): the plan of one
+gate segment with Pauli events on chosen copies,
 executed by the host emulator (tests/emu), against the oracle run copy by
 copy with the Paulis inserted where they fall.  Runs without a GPU."""
 import ctypes
@@ -24,7 +25,8 @@ def _lib():
     return L
 
 
-@pytest.mark.parametrize("nsys,b,seed,prune", [(6, 3, 1, True), (8, 2, 2, True), (7, 4, 3, False), (5, 5, 4, True)])
+@pytest.mark.parametrize("nsys,b,seed,prune", [(6, 3, 1, True), (8, 2, 2, True), (7, 4, 3, False), (5, 5, 4, True),
+                                               (11, 2, 5, True), (12, 2, 6, True), (13, 1, 7, False)])
 def test_copies_with_pauli_events(nsys, b, seed, prune):
     rng = np.random.default_rng(seed)
     calls = random_calls(nsys, 60, seed=seed)
@@ -57,7 +59,7 @@ def test_copies_with_pauli_events(nsys, b, seed, prune):
         buf = ctypes.create_string_buffer(512)
         L.emu_last_error(buf, 512)
         raise AssertionError(f"emulator: rc={rc}: {buf.value.decode()}")
-    assert stats[0] == 1 and stats[1] == nsys        # one LDS pass, a copy per tile
+    assert stats[0] == 1 and stats[1] == nsys        # one pass, a copy per tile
     for c in range(copies):
         seq = []
         mine = [e for e in ents if e[3] == c]
