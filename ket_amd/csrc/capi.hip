@@ -102,27 +102,32 @@ int ksv_create(int n, int precision, ksv_state** out) {
     HIPCHK(hipGetDevice(&s->dev));
     HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     hipError_t e = hipMalloc(&s->psi, state_bytes(s.get()));
-    if (e != hipSuccess) {
-        hipStreamDestroy(s->st);
-        return fail(KSV_ERR_DEVICE, "out of device memory allocating a " + std::to_string(n) + "-qubit state");
+    const bool oom = e != hipSuccess;
+    if (e == hipSuccess) e = hipMalloc(&s->d_part, sizeof(double) * NBLK * 64);
+    if (e == hipSuccess) e = hipMalloc(&s->d_small, sizeof(double) * 64);
+    if (e == hipSuccess) e = k_init_basis(n, precision, s->psi, 0, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) {      // release what was allocated
+        (void)hipFree(s->psi);
+        (void)hipFree(s->d_part);
+        (void)hipFree(s->d_small);
+        (void)hipStreamDestroy(s->st);
+        if (oom) return fail(KSV_ERR_DEVICE, "out of device memory allocating a " + std::to_string(n) + "-qubit state");
+        return fail(KSV_ERR_DEVICE, std::string("state set-up: ") + hipGetErrorString(e));
     }
-    HIPCHK(hipMalloc(&s->d_part, sizeof(double) * NBLK * 64));
-    HIPCHK(hipMalloc(&s->d_small, sizeof(double) * 64));
-    HIPCHK(k_init_basis(n, precision, s->psi, 0, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
     *out = s.release();
     return KSV_OK;
 }
 
 int ksv_destroy(ksv_state* s) {
     if (!s) return fail(KSV_ERR_HANDLE, "null state");
-    hipStreamSynchronize(s->st);
-    hipFree(s->psi);
-    if (s->psi2) hipFree(s->psi2);
-    hipFree(s->d_part);
-    hipFree(s->d_small);
-    for (auto e : s->ev) hipEventDestroy(e);
-    hipStreamDestroy(s->st);
+    (void)hipStreamSynchronize(s->st);
+    (void)hipFree(s->psi);
+    if (s->psi2) (void)hipFree(s->psi2);
+    (void)hipFree(s->d_part);
+    (void)hipFree(s->d_small);
+    for (auto e : s->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(s->st);
     delete s;
     return KSV_OK;
 }

@@ -491,13 +491,13 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
             cre[t] = re[ny];
             cim[t] = im[ny];
         }
-        uint64_t *d_x, *d_z;
-        double *d_cr, *d_ci;
-        size_t nt = std::max<size_t>(nterms, 1);
-        AHIP(hipMalloc(&d_x, 8 * nt));
-        AHIP(hipMalloc(&d_z, 8 * nt));
-        AHIP(hipMalloc(&d_cr, 8 * nt));
-        AHIP(hipMalloc(&d_ci, 8 * nt));
+        const size_t nt = std::max<size_t>(nterms, 1);
+        DevBuf tb;      // the term arrays, freed on every return
+        AHIP(tb.alloc(4 * 8 * nt));
+        uint64_t* d_x = static_cast<uint64_t*>(tb.p);
+        uint64_t* d_z = d_x + nt;
+        double* d_cr = reinterpret_cast<double*>(d_z + nt);
+        double* d_ci = d_cr + nt;
         if (nterms) {
             AHIP(hipMemcpyAsync(d_x, xmask, 8 * nterms, hipMemcpyHostToDevice, st));
             AHIP(hipMemcpyAsync(d_z, zmask, 8 * nterms, hipMemcpyHostToDevice, st));
@@ -506,7 +506,7 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         }
         int g = (int)std::min<uint64_t>((N + 255) / 256, 8192);
         if (nterms == 0) {
-            AHIP(hipMemsetAsync(psi2, 0, (prec == KSV_F64 ? 16 : 8) << n, st));
+            AHIP(hipMemsetAsync(psi2, 0, (size_t)(prec == KSV_F64 ? 16 : 8) << n, st));
         }
         for (size_t t0 = 0; t0 < nterms; t0 += 8) {
             int m = (int)std::min<size_t>(8, nterms - t0);
@@ -530,10 +530,6 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         double e = 0;
         for (int i = 0; i < gb; i++) e += hp[i];
         if (energy) *energy = e;
-        hipFree(d_x);
-        hipFree(d_z);
-        hipFree(d_cr);
-        hipFree(d_ci);
     }
     // backward sweep: the inverse tape in tile passes over phi (psi) and
     // lambda (psi2); every workgroup writes one partial per record, summed

@@ -632,23 +632,23 @@ int ksv_qaoa_batch_info(const ksv_batch* b, uint64_t* total_amplitudes, uint64_t
 
 int ksv_qaoa_batch_destroy(ksv_batch* b) {
     if (!b) return qfail(KSV_ERR_HANDLE, "null batch");
-    hipStreamSynchronize(b->st);
+    (void)hipStreamSynchronize(b->st);
     for (int g = 0; g < 2; g++) {
-        if (b->exec[g]) hipGraphExecDestroy(b->exec[g]);
-        if (b->graph[g]) hipGraphDestroy(b->graph[g]);
+        if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
+        if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
     }
-    hipFree(b->psi);
-    hipFree(b->lam);
-    hipFree(b->d_inst);
-    hipFree(b->d_winst);
-    hipFree(b->d_wstart);
-    hipFree(b->d_gamma);
-    hipFree(b->d_beta);
-    hipFree(b->d_part);
-    hipFree(b->d_out);
-    hipFree(b->d_ll);
-    hipFree(b->d_phs);
-    hipStreamDestroy(b->st);
+    (void)hipFree(b->psi);
+    (void)hipFree(b->lam);
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->d_winst);
+    (void)hipFree(b->d_wstart);
+    (void)hipFree(b->d_gamma);
+    (void)hipFree(b->d_beta);
+    (void)hipFree(b->d_part);
+    (void)hipFree(b->d_out);
+    (void)hipFree(b->d_ll);
+    (void)hipFree(b->d_phs);
+    (void)hipStreamDestroy(b->st);
     delete b;
     return KSV_OK;
 }

@@ -558,7 +558,7 @@ static int gf2_rank(std::array<uint32_t, 4> v, int m) {
 // layout reach four distinct 16-byte columns of every 256-byte LDS row
 // (and lanes 0..2 distinct ones of every 128-byte half), i.e. conflict-free
 // 16-byte accesses on both sides.
-static void emit_exch(int K, int TB, const RLayout& W, const RLayout& R, Program& prog) {
+static void emit_exch(int TB, const RLayout& W, const RLayout& R, Program& prog) {
     uint32_t v[16] = {0};
     auto addr = [&](int p) -> uint32_t { return (1u << p) ^ (p >= 4 ? v[p] : 0u); };
     auto good = [&](const RLayout& L) {
@@ -742,7 +742,7 @@ static void round_pass(int n, int K, uint64_t T, const std::vector<LOp>& lops, c
     for (size_t r = 0; r < sets.size(); r++) {
         const RLayout L = make_layout(K, sets[r]);
         if (r > 0) {
-            emit_exch(K, TB, make_layout(K, sets[r - 1]), L, prog);
+            emit_exch(TB, make_layout(K, sets[r - 1]), L, prog);
             st.exchanges++;
         }
         for (int id : takes[r]) {

@@ -687,3 +687,15 @@ def test_adjoint_tiled_random_ansatz(precision, n, seed, tol):
     if precision == "f64":
         ge, _, _ = emu_gradient(ans, params, H, K=11)
         assert np.max(np.abs(np.array(g) - ge)) < 1e-11
+
+
+def test_adjoint_without_terms_large_state():
+    """An empty Hamiltonian at 27 qubits: lambda is cleared (2 GiB, a byte count past
+    2^31) and the energy and every gradient entry are exactly zero."""
+    from ket_amd.algorithms import efficientSU2, energy_and_gradient
+    n = 27
+    ans = efficientSU2(n, 1)
+    params = workloads.ansatz_params(4 * n, seed=3)
+    ket_amd.configure(prune=False)
+    e, g = energy_and_gradient(ans, [], params)
+    assert e == 0.0 and all(v == 0.0 for v in g)
