@@ -3,6 +3,8 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <thread>
+#include <vector>
 
 #include "ksv_internal.h"
 
@@ -820,18 +822,83 @@ static int shots_check(ksv_state* s, int n, uint32_t* copies) {
     return KSV_OK;
 }
 
+// xorshift32 (src/circuit.ts:813-819) is linear over GF(2)^32: one step is a
+// 32x32 bit matrix M, stored as the images of the 32 unit vectors, and `steps`
+// steps are M^steps, built by repeated squaring.
+namespace {
+
+uint32_t xs_step(uint32_t v) {
+    v ^= v << 13;
+    v ^= v >> 17;
+    v ^= v << 5;
+    return v;
+}
+
+struct Gf2Map {
+    uint32_t col[32];
+};
+
+uint32_t gf2_apply(const Gf2Map& m, uint32_t v) {
+    uint32_t r = 0;
+    for (int i = 0; v; i++, v >>= 1)
+        if (v & 1) r ^= m.col[i];
+    return r;
+}
+
+uint32_t xs_jump(uint32_t v, uint64_t steps) {
+    Gf2Map p;
+    for (int i = 0; i < 32; i++) p.col[i] = xs_step(1u << i);
+    while (steps) {
+        if (steps & 1) v = gf2_apply(p, v);
+        Gf2Map sq;
+        for (int i = 0; i < 32; i++) sq.col[i] = gf2_apply(p, p.col[i]);
+        p = sq;
+        steps >>= 1;
+    }
+    return v;
+}
+
+// out[i] = the (i+1)-th state after v, divided by 2^32; returns the last state
+uint32_t xs_fill(uint32_t v, double* out, size_t count) {
+    for (size_t i = 0; i < count; i++) {
+        v = xs_step(v);
+        out[i] = v / 4294967296.0;
+    }
+    return v;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ksv_xorshift32(uint32_t* state, size_t count, double* out) {
     if (!state || (count && !out)) return fail(KSV_ERR_TYPE, "null buffer");
-    uint32_t v = *state ? *state : 1u;
-    for (size_t i = 0; i < count; i++) {
-        v ^= v << 13;
-        v ^= v >> 17;
-        v ^= v << 5;
-        out[i] = v / 4294967296.0;
+    const uint32_t v0 = *state ? *state : 1u;
+    // Long streams are cut into chunks, one per thread; each chunk starts from
+    // the state xs_jump gives for its first index, so the output is the serial
+    // stream's.
+    const size_t chunk_min = size_t(1) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({(size_t)std::min(hw, 16u), (count + chunk_min - 1) / chunk_min});
+    if (nt <= 1) {
+        *state = xs_fill(v0, out, count);
+        return KSV_OK;
     }
-    *state = v;
+    const size_t per = (count + nt - 1) / nt;
+    std::vector<std::thread> pool;
+    size_t covered = per;           // [0, covered): chunk 0 here, the others on started threads
+    try {
+        for (size_t t = 1; t < nt && covered < count; t++) {
+            const size_t b = t * per, e = std::min(count, b + per);
+            pool.emplace_back([=] { xs_fill(xs_jump(v0, b), out + b, e - b); });
+            covered = e;
+        }
+    } catch (...) {                 // a thread that could not start: its chunks run below
+    }
+    xs_fill(v0, out, per);
+    if (covered < count) xs_fill(xs_jump(v0, covered), out + covered, count - covered);
+    for (auto& th : pool) th.join();
+    *state = xs_jump(v0, count);
     return KSV_OK;
 }
 

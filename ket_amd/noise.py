@@ -1,7 +1,7 @@
 """run()'s per-shot path: noise, mid-circuit measurement, reset and `if`
 (src/circuit.ts:169-207, :209-304, :3455-3481).
 
-Two drivers.  `run_batched` (circuits without `if`) holds up to 4096 shots
+Two drivers.  `run_batched` (circuits without `if`) holds up to 8192 shots
 as copies of one device state and walks the circuit once for all of them;
 `run_per_shot` re-simulates shot by shot and serves circuits with `if`,
 whose draw count depends on measured values.
@@ -161,7 +161,7 @@ def run_per_shot(circ, shots, rng, noise, init, creg_counts):
 # Batched trajectories
 # ---------------------------------------------------------------------------
 MAX_BATCH_AMPS = 1 << 28          # amplitudes per batched state (4 GiB FP64)
-MAX_COPIES = 4096
+MAX_COPIES = 8192
 
 
 def batchable(circ):
@@ -208,18 +208,24 @@ def run_batched(circ, shots, rng, noise, init, creg_counts):
     b = 0
     while (1 << (b + 1)) <= max(1, shots) and (1 << (b + 1)) <= MAX_COPIES and (2 << b) << n <= MAX_BATCH_AMPS:
         b += 1
-    if (1 << b) < shots and (2 << b) << n <= MAX_BATCH_AMPS and (1 << (b + 1)) <= MAX_COPIES:
-        b += 1                                    # round up: one batch instead of two
     B = 1 << b
-    counts = {}
-    st = State(n + b, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+    counts, memo, states = {}, {}, {}
     try:
         for s0 in range(0, shots, B):
             cnt = min(B, shots - s0)
-            dr = np.full((B, D), 0.5)
-            dr[:cnt] = draws[s0:s0 + cnt]
-            regs = {name: np.zeros((B, size), dtype=bool) for name, size in circ._cregs.items()}
-            idx = _run_copies(st, n, flat, dr, p1, p2, pm, init, regs)
+            bb = b                                    # the last batch: the smallest power of two that holds it
+            while bb > 0 and (1 << (bb - 1)) >= cnt:
+                bb -= 1
+            Bc = 1 << bb
+            if bb not in states:
+                states[bb] = State(n + bb, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+            if cnt == Bc:
+                dr = draws[s0:s0 + Bc]                # rows of a C-ordered array: contiguous
+            else:                                     # the last copies of a short batch idle on 0.5s
+                dr = np.full((Bc, D), 0.5)
+                dr[:cnt] = draws[s0:s0 + cnt]
+            regs = {name: np.zeros((Bc, size), dtype=bool) for name, size in circ._cregs.items()}
+            idx = _run_copies(states[bb], n, flat, dr, p1, p2, pm, init, regs, memo)
             for t in range(cnt):
                 i = int(idx[t])
                 counts[i] = counts.get(i, 0) + 1
@@ -228,7 +234,8 @@ def run_batched(circ, shots, rng, noise, init, creg_counts):
                 for j in range(bits.shape[1]):
                     acc[j] += int(bits[:cnt, j].sum())
     finally:
-        st.free()
+        for st in states.values():
+            st.free()
     return counts, creg_counts
 
 
@@ -238,56 +245,122 @@ _PAIR_A = np.array([0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3], dtype=np.uint8
 _PAIR_B = np.array([1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3], dtype=np.uint8)
 
 
-def _run_copies(st, n, flat, dr, p1, p2, pm, init, regs):
-    """One batch: events are collected on the host, then the whole batch runs in
-    one ksv_shots_run call; returns the drawn index of every copy."""
+_TAPE_KINDS = ("single", "cnot", "controlled", "swap", "two", "toffoli", "cswap", "csrswap", "unitary")
+
+
+def _sparse_flatnonzero(mask):
+    """np.flatnonzero for a mask with few set entries: scans 8 bytes at a time."""
+    flat = np.ascontiguousarray(mask).reshape(-1).view(np.uint8)
+    body = len(flat) // 8 * 8
+    words = np.flatnonzero(flat[:body].view(np.uint64))
+    idx = (words[:, None] * 8 + np.arange(8)).reshape(-1)
+    idx = idx[flat[idx] != 0]
+    tail = body + np.flatnonzero(flat[body:])
+    return np.concatenate([idx, tail]).astype(np.int64)
+
+
+def shot_events(flat, dr, p1, p2, pm):
+    """Host side of one batch: the gate tape and the event list of ksv_shots_run.
+
+    `dr` holds one row of draws per copy.  Returns (gates, ev, copies, codes, r,
+    meas, k): Pauli events are grouped per (gate, qubit) with their copies in
+    ascending order, measure / reset events follow the noise of the gate before
+    them, `r` holds the draws of every measure / reset event (B per event),
+    `meas` lists (op, offset in r, flip mask or None) per measurement and `k` is
+    the next unused draw column.  dep1 picks X, Y, Z by thirds of rand / p1 and
+    dep2 one of the 15 Pauli pairs (src/circuit.ts:263-304); the columns are
+    evaluated for all copies at once.
+    """
     B = dr.shape[0]
-    st.shots_init(n, init)
-    gates, events, copies, codes_all, draws, meas = [], [], [], [], [], []
+    gates, ones, twos, ms = [], [], [], []
     k = 0
-
-    def paulis(q, t, codes):
-        events.append((len(gates), KSV_SHOT_PAULI, q, len(copies), len(t)))
-        copies.extend(t.tolist())
-        codes_all.extend(codes.tolist())
-
-    for op in flat:
+    for op in flat:                      # draw columns, in ket's order
         kind = op[0]
-        if kind in ("single", "cnot", "controlled", "swap", "two", "toffoli", "cswap", "csrswap", "unitary"):
+        if kind in _TAPE_KINDS:
             gates.append(op)
-        if kind == "single" and p1:                     # dep1: X, Y, Z by thirds of rand / p1
-            rand = dr[:, k]
+        at = len(gates)
+        if kind == "single" and p1:
+            ones.append((at, op[1], k))
             k += 1
-            t = np.nonzero(rand < p1)[0]
-            if len(t):
-                r = rand[t] / p1
-                paulis(op[1], t, (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8))
-        elif kind in ("cnot", "controlled", "swap", "two") and p2:   # dep2: one of 15 Pauli pairs
-            rand = dr[:, k]
+        elif kind in ("cnot", "controlled", "swap", "two") and p2:
+            twos.append((at, op[1], op[2], k))
             k += 1
-            t = np.nonzero(rand < p2)[0]
-            if len(t):
-                pick = np.minimum(np.floor(rand[t] / p2 * 15).astype(np.int64), 14)
-                for q, table in ((op[1], _PAIR_A), (op[2], _PAIR_B)):
-                    codes = table[pick]
-                    m = codes > 0
-                    if m.any():
-                        paulis(q, t[m], codes[m])
         elif kind in ("measure", "reset"):
-            off = len(draws) * B
-            events.append((len(gates), KSV_SHOT_MEASURE if kind == "measure" else KSV_SHOT_RESET, op[1], off, 0))
-            draws.append(dr[:, k])
-            k += 1
-            if kind == "measure":
-                flip = None
-                if pm:
-                    flip = dr[:, k] < pm
-                    k += 1
-                meas.append((op, off, flip))
-    ev = np.array(events, dtype=SHOT_EVENT_DTYPE) if events else np.zeros(0, dtype=SHOT_EVENT_DTYPE)
-    r = np.concatenate(draws) if draws else np.zeros(0)
-    out = st.shots_run(n, pack(gates), ev, np.asarray(copies, dtype=np.uint32),
-                       np.asarray(codes_all, dtype=np.uint8), r)
+            fc = k + 1 if (kind == "measure" and pm) else -1
+            ms.append((at, kind, op, k, fc))
+            k += 1 if fc < 0 else 2
+    # one entry per (Pauli, copy): sort keys (at, sub, copy), then qubit and code.
+    # Every noise column is compared with its rate in one pass over the draws.
+    parts = []
+    if ones or twos:
+        D = dr.shape[1]
+        thr = np.full(D, -1.0)                 # draws are >= 0: never a hit
+        at_of = np.zeros(D, np.int64)
+        qa_of = np.zeros(D, np.int64)
+        qb_of = np.zeros(D, np.int64)
+        if ones:
+            a = np.array(ones, dtype=np.int64)
+            thr[a[:, 2]] = p1
+            at_of[a[:, 2]], qa_of[a[:, 2]] = a[:, 0], a[:, 1]
+        if twos:
+            a = np.array(twos, dtype=np.int64)
+            thr[a[:, 3]] = p2
+            at_of[a[:, 3]], qa_of[a[:, 3]], qb_of[a[:, 3]] = a[:, 0], a[:, 1], a[:, 2]
+        c, col = np.divmod(_sparse_flatnonzero(dr < thr), D)
+        rv = dr[c, col]
+        two = np.zeros(D, bool)
+        if twos:
+            two[np.array(twos, dtype=np.int64)[:, 3]] = True
+        t2 = two[col]
+        s = ~t2
+        if s.any():
+            cs, r = col[s], rv[s] / p1
+            parts.append((at_of[cs], np.zeros(len(cs), np.int64), c[s], qa_of[cs],
+                          (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8)))
+        if t2.any():
+            cs, c2 = col[t2], c[t2]
+            pick = np.minimum(np.floor(rv[t2] / p2 * 15).astype(np.int64), 14)
+            for sub, q_of, table in ((0, qa_of, _PAIR_A), (1, qb_of, _PAIR_B)):
+                code = table[pick]
+                m = code > 0
+                parts.append((at_of[cs[m]], np.full(int(m.sum()), sub, np.int64), c2[m], q_of[cs[m]], code[m]))
+    if parts:
+        at, sub, cp, q, code = (np.concatenate(x) for x in zip(*parts))
+        o = np.lexsort((cp, sub, at))
+        at, sub, cp, q, code = at[o], sub[o], cp[o], q[o], code[o]
+        first = np.flatnonzero(np.r_[True, (at[1:] != at[:-1]) | (sub[1:] != sub[:-1])])
+        cnt = np.diff(np.r_[first, len(at)])
+        p_at, p_sub, p_q = at[first], sub[first], q[first]
+    else:
+        cp, code = np.zeros(0, np.int64), np.zeros(0, np.uint8)
+        first = cnt = p_at = p_sub = p_q = np.zeros(0, np.int64)
+    nm = len(ms)
+    ev = np.zeros(len(first) + nm, dtype=SHOT_EVENT_DTYPE)
+    key_at = np.r_[p_at, np.array([m[0] for m in ms], dtype=np.int64)]
+    key_sub = np.r_[p_sub, 2 + np.arange(nm, dtype=np.int64)]   # noise first, then measures in order
+    ev["at"] = key_at
+    ev["kind"] = np.r_[np.full(len(first), KSV_SHOT_PAULI),
+                       [KSV_SHOT_MEASURE if m[1] == "measure" else KSV_SHOT_RESET for m in ms]]
+    ev["q"] = np.r_[p_q, np.array([m[2][1] for m in ms], dtype=np.int64)]
+    ev["off"] = np.r_[first, np.arange(nm, dtype=np.int64) * B]
+    ev["count"] = np.r_[cnt, np.zeros(nm, np.int64)]
+    ev = ev[np.lexsort((key_sub, key_at))]
+    r = np.ascontiguousarray(dr[:, [m[3] for m in ms]].T).ravel() if nm else np.zeros(0)
+    meas = [(m[2], i * B, (dr[:, m[4]] < pm) if m[4] >= 0 else None)
+            for i, m in enumerate(ms) if m[1] == "measure"]
+    return gates, ev, cp.astype(np.uint32), code.astype(np.uint8), r, meas, k
+
+
+def _run_copies(st, n, flat, dr, p1, p2, pm, init, regs, memo):
+    """One batch: events are collected on the host, then the whole batch runs in
+    one ksv_shots_run call; returns the drawn index of every copy.  `memo` keeps
+    the packed gate tape, which is the same for every batch of a run."""
+    st.shots_init(n, init)
+    gates, ev, copies, codes, r, meas, k = shot_events(flat, dr, p1, p2, pm)
+    if "tape" not in memo:
+        memo["tape"] = pack(gates)
+    out = st.shots_run(n, memo["tape"], ev, copies, codes, r)
+    B = dr.shape[0]
     for op, off, flip in meas:
         o = out[off:off + B]
         if flip is not None:

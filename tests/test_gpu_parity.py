@@ -326,16 +326,18 @@ def test_batched_shots_equal_per_shot_path_variants(n, precision, wide):
 
 
 def test_batched_shots_in_several_batches(monkeypatch):
-    """More shots than copies per batch: the draw stream continues across batches."""
+    """More shots than copies per batch: the draw stream continues across batches,
+    and the last batch (1 or 4 shots) runs on a state of 1 or 4 copies."""
     import ket_amd.noise as N
     monkeypatch.setattr(N, "MAX_COPIES", 16)
     n = 4
     calls = shot_calls(n, 25, 5)
-    d = build(n, calls, cregs={"c": 2}).run(shots=100, seed=5, noise={"p1": 0.05, "p2": 0.1, "pMeas": 0.02})
-    counts, cr = O.run_shots(n, calls, 100, 5, {"p1": 0.05, "p2": 0.1, "pMeas": 0.02}, cregs={"c": 2})
-    e = _expected_dist(n, 100, counts, cr)
-    assert list(d.probs.items()) == list(e.probs.items())
-    assert d.cregs == e.cregs
+    for shots in (100, 97):
+        d = build(n, calls, cregs={"c": 2}).run(shots=shots, seed=5, noise={"p1": 0.05, "p2": 0.1, "pMeas": 0.02})
+        counts, cr = O.run_shots(n, calls, shots, 5, {"p1": 0.05, "p2": 0.1, "pMeas": 0.02}, cregs={"c": 2})
+        e = _expected_dist(n, shots, counts, cr)
+        assert list(d.probs.items()) == list(e.probs.items())
+        assert d.cregs == e.cregs
 
 
 def test_batched_shots_entry_points():

@@ -178,6 +178,96 @@ def test_batched_shot_draw_layout():
     assert not batchable(c.if_("c", 1, lambda k: k.x(0)))
 
 
+def _events_loop(flat, dr, p1, p2, pm):
+    """Event list of one batch built op by op and column by column (ket's per-shot
+    loop, src/circuit.ts:263-304, run for every copy of a column at once)."""
+    from ket_amd.noise import _PAIR_A, _PAIR_B, _TAPE_KINDS
+    B = dr.shape[0]
+    gates, events, copies, codes_all, draws, meas = [], [], [], [], [], []
+    k = 0
+
+    def paulis(q, t, codes):
+        events.append((len(gates), 0, q, len(copies), len(t)))
+        copies.extend(t.tolist())
+        codes_all.extend(codes.tolist())
+
+    for op in flat:
+        kind = op[0]
+        if kind in _TAPE_KINDS:
+            gates.append(op)
+        if kind == "single" and p1:
+            rand = dr[:, k]
+            k += 1
+            t = np.nonzero(rand < p1)[0]
+            if len(t):
+                r = rand[t] / p1
+                paulis(op[1], t, (1 + (r >= 1 / 3) + (r >= 2 / 3)).astype(np.uint8))
+        elif kind in ("cnot", "controlled", "swap", "two") and p2:
+            rand = dr[:, k]
+            k += 1
+            t = np.nonzero(rand < p2)[0]
+            if len(t):
+                pick = np.minimum(np.floor(rand[t] / p2 * 15).astype(np.int64), 14)
+                for q, table in ((op[1], _PAIR_A), (op[2], _PAIR_B)):
+                    codes = table[pick]
+                    m = codes > 0
+                    if m.any():
+                        paulis(q, t[m], codes[m])
+        elif kind in ("measure", "reset"):
+            off = len(draws) * B
+            events.append((len(gates), 1 if kind == "measure" else 2, op[1], off, 0))
+            draws.append(dr[:, k])
+            k += 1
+            if kind == "measure":
+                flip = None
+                if pm:
+                    flip = dr[:, k] < pm
+                    k += 1
+                meas.append((op, off, flip))
+    return gates, events, copies, codes_all, draws, meas, k
+
+
+@pytest.mark.parametrize("p1,p2,pm", [(0.2, 0.3, 0.1), (0.0, 0.5, 0.0), (0.4, 0.0, 0.2), (0.0, 0.0, 0.0)])
+def test_batched_event_lists_match_the_column_loop(p1, p2, pm):
+    """shot_events (all draw columns at once) gives the same tape, events, copies,
+    codes, measurement draws and flips as the column-by-column loop."""
+    from ket_amd.noise import draws_per_shot, shot_events
+    from ket_amd.circuit import _flatten
+    rs = np.random.default_rng(5)
+    c = Circuit(4).creg("c", 3)
+    for step in range(30):
+        kind = rs.integers(6)
+        a, b = (int(x) for x in rs.choice(4, 2, replace=False))
+        if kind == 0:
+            c = c.h(a)
+        elif kind == 1:
+            c = c.cnot(a, b)
+        elif kind == 2:
+            c = c.measure(a, "c", int(rs.integers(3)))
+        elif kind == 3:
+            c = c.reset(a)
+        elif kind == 4:
+            c = c.ccx(a, b, 3 - a if 3 - a not in (a, b) else ({0, 1, 2, 3} - {a, b}).pop())
+        else:
+            c = c.rz(0.1 * step, a).swap(a, b)
+    c = c.measure(0, "c", 0).measure(1, "c", 1)
+    flat = _flatten(c._ops())
+    B = 257
+    D = draws_per_shot(flat, 4, p1, p2, pm)
+    dr = rs.random((B, D))
+    g0, ev0, cp0, co0, dr0, me0, k0 = _events_loop(flat, dr, p1, p2, pm)
+    g1, ev1, cp1, co1, r1, me1, k1 = shot_events(flat, dr, p1, p2, pm)
+    assert g1 == g0 and k1 == k0
+    assert [tuple(int(x) for x in e) for e in ev1] == ev0
+    assert cp1.tolist() == cp0 and co1.tolist() == co0
+    assert r1.tolist() == (np.concatenate(dr0).tolist() if dr0 else [])
+    assert len(me1) == len(me0)
+    for (o1, f1, x1), (o0, f0, x0) in zip(me1, me0):
+        assert o1 == o0 and f1 == f0
+        assert (x1 is None and x0 is None) or x1.tolist() == x0.tolist()
+    assert any(e[1] == 0 for e in ev0) == bool(p1 or p2)
+
+
 def test_prng_fill_continues_the_stream():
     """rng.fill(k) (ksv_xorshift32) gives the next k draws of the same stream."""
     a = make_prng(12345)
@@ -188,6 +278,27 @@ def test_prng_fill_continues_the_stream():
     assert filled.tolist() == [a() for _ in range(1000)]
     assert a() == b()
     assert make_prng(0).fill(3).tolist() == make_prng(1).fill(3).tolist()     # seed 0 -> 1
+
+
+def test_prng_long_fill_is_the_serial_stream():
+    """A fill long enough to be cut into per-thread chunks (each started by the
+    GF(2) jump-ahead) is the serial stream: every draw is one xorshift32 step of
+    the one before it, the first follows the seed, and the stream continues."""
+    seed = 987654321
+    a, b = make_prng(seed), make_prng(seed)
+    count = (3 << 20) + 12345
+    out = b.fill(count)
+    s = np.round(out * 4294967296.0).astype(np.uint64).astype(np.uint32)
+    assert np.array_equal(s.astype(np.float64) / 4294967296.0, out)
+    v = s[:-1].copy()
+    v ^= v << np.uint32(13)
+    v ^= v >> np.uint32(17)
+    v ^= v << np.uint32(5)
+    assert np.array_equal(v, s[1:])
+    ref = O.xorshift(seed)
+    assert [next(ref) for _ in range(5)] == out[:5].tolist() == [a() for _ in range(5)]
+    c = make_prng(int(s[-1]))       # state after the fill is the last draw's state
+    assert b() == c()
 
 
 @pytest.mark.parametrize("noise,seed", [(None, 1), ({"p1": 0.05, "p2": 0.1}, 2), ({"p1": 0.02, "p2": 0.03}, 3)])
@@ -278,7 +389,8 @@ def test_batched_shot_driver_on_the_oracle(monkeypatch, noise, seed):
     c = build(n, calls, cregs={"c": 2})
     params = N.resolve_noise(noise)
     cregs = {"c": [0, 0]}
-    counts, cr = N.run_batched(c, 40, make_prng(seed), params, 0, cregs)
-    ref, rcr = O.run_shots(n, calls, 40, seed, params, cregs={"c": 2})
+    shots = {1: 41, 2: 43, 3: 40}[seed]          # last batch of 1, 3 (in 4 copies) and 8 shots
+    counts, cr = N.run_batched(c, shots, make_prng(seed), params, 0, cregs)
+    ref, rcr = O.run_shots(n, calls, shots, seed, params, cregs={"c": 2})
     assert list(counts.items()) == list(ref.items())
     assert cr == rcr
