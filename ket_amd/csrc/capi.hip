@@ -237,7 +237,7 @@ static void program_blob(const ksv_state* s, const Program& p, std::vector<char>
         std::memcpy(blob.data() + at, src, bytes);
         return at;
     };
-    off.assign(6, SIZE_MAX);
+    off.assign(7, SIZE_MAX);
     off[0] = put(p.rops.data(), sizeof(ROp) * p.rops.size());
     off[1] = put(p.dops.data(), sizeof(DOp) * p.dops.size());
     off[2] = put(p.brecs.data(), sizeof(BRec) * p.brecs.size());
@@ -276,6 +276,7 @@ static void program_blob(const ksv_state* s, const Program& p, std::vector<char>
         }
     }
     off[5] = put(p.bits.data(), sizeof(uint32_t) * p.bits.size());
+    off[6] = put(p.rbits.data(), sizeof(uint32_t) * p.rbits.size());
 }
 
 static void bind_blob(Program& p, char* base, const std::vector<size_t>& off) {
@@ -286,6 +287,7 @@ static void bind_blob(Program& p, char* base, const std::vector<size_t>& off) {
     p.d_frags = at(off[3]);
     p.d_mats = at(off[4]);
     p.d_bits = (uint32_t*)at(off[5]);
+    p.d_rbits = (uint32_t*)at(off[6]);
 }
 
 static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,

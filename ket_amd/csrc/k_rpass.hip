@@ -21,8 +21,14 @@
 // amplitude per pass, coalesced: thread bits 0..2 are physical qubits 0..2
 // at load and store, so each 8 lanes move a full 128-byte run.
 //
-// Op records (128 bytes, matrix inline) are read through the constant
-// address space: uniform scalar loads into SGPRs.
+// Op records (128 bytes, matrix inline) are staged through LDS in chunks of
+// 2^TB / 8 records (16 bytes per thread, coalesced), double-buffered so that
+// the next chunk's global loads are in flight while the current one runs; each
+// op then reads its record from LDS (uniform address) into scalar registers.
+// Read straight through the scalar cache instead, every op waited on its
+// record from L2 (a copy program's op stream far exceeds that cache).  The
+// copy-bitmap words of conditional ops come with the chunk, one per lane of
+// every wave, and a ballot turns them into a 64-bit per-chunk mask.
 #include <algorithm>
 #include <cstring>
 
@@ -47,6 +53,27 @@ __device__ __forceinline__ CM<T> ldm(const KSV_CONST T* m, int i) {
     return r;
 }
 
+// A value every lane holds alike (read from LDS at a uniform address) moved to
+// scalar registers, so that branches on it are uniform and FMAs take it as
+// their scalar operand.
+__device__ __forceinline__ uint32_t sreg(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t sreg(uint64_t v) {
+    return (uint64_t)sreg((uint32_t)v) | ((uint64_t)sreg((uint32_t)(v >> 32)) << 32);
+}
+__device__ __forceinline__ double sreg(double v) {
+    return __builtin_bit_cast(double, sreg(__builtin_bit_cast(uint64_t, v)));
+}
+__device__ __forceinline__ float sreg(float v) { return __builtin_bit_cast(float, sreg(__builtin_bit_cast(uint32_t, v))); }
+
+// matrix entry i of an op record staged in LDS
+template <typename T>
+__device__ __forceinline__ CM<T> ldm(const T* m, int i) {
+    CM<T> r;
+    r.x = sreg(m[2 * i]);
+    r.y = sreg(m[2 * i + 1]);
+    return r;
+}
+
 template <typename T, bool PR>
 __device__ __forceinline__ void prune2(T& r, T& i) {
     if (PR) {
@@ -65,8 +92,8 @@ __device__ __forceinline__ void mac2(T& ore, T& oim, CM<T> a, T x0r, T x0i, CM<T
 
 // 2x2 on slot S.  SEL: a thread-bit control exists (lp: this thread is
 // selected).  RC: slot controls exist (rctl).
-template <typename T, int S, bool PR, bool SEL, bool RC>
-__device__ __forceinline__ void r_m1(RREGS_ARGS, const KSV_CONST T* m, uint32_t rctl, bool lp) {
+template <typename T, int S, bool PR, bool SEL, bool RC, typename P>
+__device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp) {
     const CM<T> m0 = ldm(m, 0), m1 = ldm(m, 1), m2 = ldm(m, 2), m3 = ldm(m, 3);
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
@@ -122,8 +149,8 @@ __device__ __forceinline__ void r_swap(RREGS_ARGS, uint32_t rctl, bool lp) {
 // thread bit, a slot or a physical bit outside the tile (loc[i]).  For
 // k <= 2 the entries are inline (scalar registers) and selected per
 // amplitude; wider diagonals are looked up per amplitude.
-template <typename T, bool PR, bool SEL>
-__device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, uint32_t rctl, const KSV_CONST T* pay,
+template <typename T, bool PR, bool SEL, typename P>
+__device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, uint32_t rctl, P pay,
                                        const T* wide, uint32_t tid, uint64_t base, bool lp) {
     uint32_t rl = 0, s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
@@ -216,19 +243,24 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
 }
 
 // LDS transpose from the current slot/thread assignment to the next one.
+// `xw`: the record's 32 16-bit entries (write address of thread bit 0..11,
+// of slot 0..3, read address likewise) as eight words.
 template <typename T, int TB>
-__device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, const KSV_CONST uint16_t* x,
-                                       uint32_t tid) {
+__device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, const uint64_t* xw, uint32_t tid) {
     using V = typename Vec2<T>::type;
+    uint64_t xs[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) xs[i] = sreg(xw[i]);
+    auto x = [&](int i) -> uint32_t { return (uint32_t)(xs[i >> 2] >> (16 * (i & 3))) & 0xFFFF; };
     uint32_t wb = 0, rb = 0;
 #pragma unroll
     for (int i = 0; i < TB; i++) {
         const uint32_t bit = (tid >> i) & 1;
-        wb ^= bit ? (uint32_t)x[i] : 0u;
-        rb ^= bit ? (uint32_t)x[16 + i] : 0u;
+        wb ^= bit ? x(i) : 0u;
+        rb ^= bit ? x(16 + i) : 0u;
     }
-    const uint32_t w0 = x[12], w1 = x[13], w2 = x[14], w3 = x[15];
-    const uint32_t r0 = x[28], r1 = x[29], r2 = x[30], r3 = x[31];
+    const uint32_t w0 = x(12), w1 = x(13), w2 = x(14), w3 = x(15);
+    const uint32_t r0 = x(28), r1 = x(29), r2 = x(30), r3 = x(31);
     __syncthreads();        // the previous LDS use of every thread is finished
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
@@ -295,19 +327,41 @@ template <typename T, int TB, bool PR>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     using V = typename Vec2<T>::type;
     constexpr int K = TB + RSLOTS;
+    constexpr int NT = 1 << TB;
+    constexpr int C = NT / 8;                 // op records per staged chunk: 16 bytes per thread
+    static_assert(C <= 64 && sizeof(ROp) == 128, "one wave's ballot covers a chunk");
     __shared__ V lds[1 << K];
+    __shared__ uint4 stage[2][NT];            // two chunks of op records
     const uint32_t tid = threadIdx.x;
     V* psi = reinterpret_cast<V*>(a.psi);
     const T* wide_mats = reinterpret_cast<const T*>(a.mats);
     const KSV_CONST T* cmats = to_const(reinterpret_cast<const T*>(a.mats));
-    const KSV_CONST uint64_t* ops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
-    const KSV_CONST uint32_t* cbits = to_const(a.bits);
+    const uint4* gops = reinterpret_cast<const uint4*>(a.ops);
 
     uint64_t tile = blockIdx.x;
     if (tile >= a.ntiles) return;
     const uint64_t ld_off = phys_offset(tid, TB, a.ld_t_lo, a.ld_t_hi);
     const uint64_t st_off = phys_offset(tid, TB, a.st_t_lo, a.st_t_hi);
     uint64_t base = tile_base(tile, K, a.tq_lo, a.tq_hi);
+
+    // Chunk c of the op stream: this thread's 16 bytes of its records, and (lanes
+    // 0..C-1 of every wave) the copy-bitmap word of op c*C + lane for this
+    // tile's copy.  Chunk c+1 is loaded while chunk c runs.
+    const uint32_t nops = (uint32_t)a.nops, nch = (nops + C - 1) / C;
+    const uint64_t copy = base >> a.nsys;
+    const uint32_t lane = tid & 63;
+    const uint32_t* crow = a.rbits ? a.rbits + (size_t)(copy >> 5) * a.rstride + a.op_base : nullptr;
+    auto fetch_rec = [&](uint32_t c) -> uint4 {
+        const size_t i = (size_t)c * NT + tid;
+        return i < (size_t)nops * 8 ? gops[i] : make_uint4(0, 0, 0, 0);
+    };
+    auto fetch_cw = [&](uint32_t c) -> uint32_t {
+        const uint32_t o = c * C + lane;
+        return (crow && lane < (uint32_t)C && o < nops) ? crow[o] : 0u;
+    };
+    auto cmask_of = [&](uint32_t cw) -> uint64_t {
+        return __builtin_amdgcn_ballot_w64(((cw >> (copy & 31)) & 1u) != 0);
+    };
 
     T re[RREGS], im[RREGS];
 #pragma unroll
@@ -318,64 +372,79 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     }
 
     {
-        for (int o = 0; o < a.nops; o++) {
-            const KSV_CONST uint64_t* w = ops + 16 * o;
-            const uint64_t h0 = w[0], h1 = w[1], h2 = w[2], h3 = w[3];
-            const uint32_t kind = (uint32_t)h0 & 0xFF;
-            const uint32_t sa = (uint32_t)(h0 >> 8) & 0xFF, sb = (uint32_t)(h0 >> 16) & 0xFF;
-            const uint32_t kq = (uint32_t)(h0 >> 24) & 0xFF;
-            const uint64_t locw = (h0 >> 32) | ((h1 & 0xFFFF) << 32);
-            const uint32_t tctl = (uint32_t)(h1 >> 32);
-            const uint32_t rctl = (uint32_t)h2;
-            const int32_t mat = (int32_t)(h2 >> 32);
-            const uint64_t gctl = h3;
-            const KSV_CONST T* pay = reinterpret_cast<const KSV_CONST T*>(w + 4);
-            if (kind == R_EXCH) {
-                r_exch<T, TB>(re, im, lds, reinterpret_cast<const KSV_CONST uint16_t*>(w + 4), tid);
-                continue;
-            }
-            if (kind == R_DENSE) {
-                // every thread reaches the barrier inside, whatever gctl says
-                const bool on = (base & gctl) == gctl;
-                const bool lp = on && ((tid & tctl) == tctl);
-                r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
-                continue;
-            }
-            if ((base & gctl) != gctl) continue;
-            if ((h1 >> 16) & ROP_COND) {    // only on the tiles of the copies in the op's bitmap
-                const uint64_t copy = base >> a.nsys;
-                if (!((cbits[mat + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
-            }
-            const bool lp = (tid & tctl) == tctl;
-            const bool sel = tctl != 0;
-            switch (kind) {
-            case R_M1:
-                if (rctl == 0) {
-#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
-                    RSLOT_CASES(sa, F)
-#undef F
-                } else {
-#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
-                    RSLOT_CASES(sa, F)
-#undef F
+        stage[0][tid] = fetch_rec(0);
+        uint64_t cmask = cmask_of(fetch_cw(0));
+        uint4 pre = fetch_rec(1);
+        uint32_t pre_cw = fetch_cw(1);
+        __syncthreads();
+        for (uint32_t c = 0; c < nch; c++) {
+            const uint64_t* recs = reinterpret_cast<const uint64_t*>(stage[c & 1]);
+            const uint32_t cn = nops - c * C < (uint32_t)C ? nops - c * C : (uint32_t)C;
+            for (uint32_t oi = 0; oi < cn; oi++) {
+                const uint64_t* w = recs + 16 * oi;
+                const uint64_t h0 = sreg(w[0]), h1 = sreg(w[1]), h2 = sreg(w[2]), h3 = sreg(w[3]);
+                const uint32_t kind = (uint32_t)h0 & 0xFF;
+                const uint32_t sa = (uint32_t)(h0 >> 8) & 0xFF, sb = (uint32_t)(h0 >> 16) & 0xFF;
+                const uint32_t kq = (uint32_t)(h0 >> 24) & 0xFF;
+                const uint64_t locw = (h0 >> 32) | ((h1 & 0xFFFF) << 32);
+                const uint32_t tctl = (uint32_t)(h1 >> 32);
+                const uint32_t rctl = (uint32_t)h2;
+                const int32_t mat = (int32_t)(h2 >> 32);
+                const uint64_t gctl = h3;
+                const T* pay = reinterpret_cast<const T*>(w + 4);
+                if (kind == R_EXCH) {
+                    r_exch<T, TB>(re, im, lds, w + 4, tid);
+                    continue;
                 }
-                break;
-            case R_DIAG:
-                if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
-                else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
-                break;
-            case R_PERMX:
+                if (kind == R_DENSE) {
+                    // every thread reaches the barrier inside, whatever gctl says
+                    const bool on = (base & gctl) == gctl;
+                    const bool lp = on && ((tid & tctl) == tctl);
+                    r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
+                    continue;
+                }
+                if ((base & gctl) != gctl) continue;
+                // only on the tiles of the copies in the op's bitmap
+                if (((h1 >> 16) & ROP_COND) && !((cmask >> oi) & 1)) continue;
+                const bool lp = (tid & tctl) == tctl;
+                const bool sel = tctl != 0;
+                switch (kind) {
+                case R_M1:
+                    if (rctl == 0) {
+#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
+                        RSLOT_CASES(sa, F)
+#undef F
+                    } else {
+#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
+                        RSLOT_CASES(sa, F)
+#undef F
+                    }
+                    break;
+                case R_DIAG:
+                    if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+                    else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+                    break;
+                case R_PERMX:
 #define F(S) (sel ? r_permx<T, S, true>(re, im, rctl, lp) : r_permx<T, S, false>(re, im, rctl, lp))
-                RSLOT_CASES(sa, F)
+                    RSLOT_CASES(sa, F)
 #undef F
-                break;
-            case R_SWAP:
+                    break;
+                case R_SWAP:
 #define F(A, B) (sel ? r_swap<T, A, B, true>(re, im, rctl, lp) : r_swap<T, A, B, false>(re, im, rctl, lp))
-                RPAIR_CASES(sa, sb, F)
+                    RPAIR_CASES(sa, sb, F)
 #undef F
-                break;
-            default:
-                break;
+                    break;
+                default:
+                    break;
+                }
+            }
+            if (c + 1 < nch) {
+                // chunk c-1's buffer: every thread finished it before the last barrier
+                stage[(c + 1) & 1][tid] = pre;
+                cmask = cmask_of(pre_cw);
+                pre = fetch_rec(c + 2);
+                pre_cw = fetch_cw(c + 2);
+                __syncthreads();
             }
         }
 
@@ -421,7 +490,9 @@ hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t 
     a.st_t_hi = tb > 8 ? pack(s.st_t + 8, tb - 8) : 0;
     a.ld_r = (uint32_t)pack(s.ld_r, RSLOTS);
     a.st_r = (uint32_t)pack(s.st_r, RSLOTS);
-    a.bits = p.d_bits;
+    a.rbits = p.d_rbits;
+    a.rstride = (uint32_t)p.rops.size();
+    a.op_base = (uint32_t)s.op_begin;
     a.nsys = p.nsys;
     if (p.precision == KSV_F64) {
         if (p.prune) launch_tb<double, true>(a, tb, st);

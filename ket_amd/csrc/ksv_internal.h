@@ -138,7 +138,9 @@ struct RPassArgs {
     uint64_t ld_t_lo, ld_t_hi;    // physical bit of thread bit i at load
     uint64_t st_t_lo, st_t_hi;    // ... at store
     uint32_t ld_r, st_r;          // physical bit of slot s at load / store
-    const uint32_t* bits;         // copy bitmaps of conditional ops
+    const uint32_t* rbits;        // word w of op g's copy bitmap at rbits[w * rstride + g] (null: no conditional ops)
+    uint32_t rstride;             // ops of the whole program
+    uint32_t op_base;             // program index of ops[0]
     int nsys;                     // copy of a tile = base >> nsys
 };
 
@@ -212,6 +214,11 @@ struct Program {
     int nsys = 0;                    // batched copies: qubits of one copy (conditional ops)
     std::vector<uint32_t> bits;      // copy bitmaps of conditional ops
     uint32_t* d_bits = nullptr;
+    // register-round copy programs: the same bitmaps by op, word w of rops[g]'s
+    // at rbits[w * rops.size() + g] (0 for unconditional ops), so that a
+    // workgroup reads the words of consecutive ops from one row
+    std::vector<uint32_t> rbits;
+    uint32_t* d_rbits = nullptr;
     std::vector<cd> mats;
     // re-parameterisation (ksv_program_set_matrix): the tape, its lowered
     // ops and where each lowered op landed

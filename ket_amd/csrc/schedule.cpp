@@ -486,6 +486,15 @@ int plan_copies(Program& p, int ntot, int nsys, int prec, const ksv_op* ops, siz
                 return KSV_ERR_TYPE;
             }
         schedule_rounds(ntot, nsys, false, lops, p);
+        const size_t nw = ((size_t(1) << (ntot - nsys)) + 31) / 32, nr = p.rops.size();
+        bool any = false;
+        for (const ROp& r : p.rops) any = any || (r.flags & ROP_COND);
+        if (any) {
+            p.rbits.assign(nw * nr, 0u);
+            for (size_t g = 0; g < nr; g++)
+                if (p.rops[g].flags & ROP_COND)
+                    for (size_t w = 0; w < nw; w++) p.rbits[w * nr + g] = p.bits[(size_t)p.rops[g].mat + w];
+        }
     } else {
         schedule_lds(ntot, nsys, lops, p);
     }

@@ -105,7 +105,11 @@ static int emu_reg(const Program& p, const Step& s, cd* psi, int n) {
                 CHECK(op.kind == R_M1 || op.kind == R_PERMX || (op.kind == R_DIAG && op.k <= 2), "conditional op of a kind with a matrix offset");
                 const uint64_t copy = base >> p.nsys;
                 CHECK(op.mat >= 0 && (size_t)op.mat + (copy >> 5) < p.bits.size(), "copy bitmap out of range");
-                if (!((p.bits[(size_t)op.mat + (copy >> 5)] >> (copy & 31)) & 1u)) continue;
+                // the kernel reads the by-op copy of the bitmaps
+                const size_t g = (size_t)s.op_begin + o, at = (size_t)(copy >> 5) * p.rops.size() + g;
+                CHECK(at < p.rbits.size(), "by-op copy bitmap out of range");
+                CHECK(p.rbits[at] == p.bits[(size_t)op.mat + (copy >> 5)], "by-op copy bitmap differs");
+                if (!((p.rbits[at] >> (copy & 31)) & 1u)) continue;
             }
             CHECK((op.flags & ROP_COND) || (op.mat >= 0 && (size_t)op.mat <= p.mats.size()), "matrix offset out of range");
             const cd* wide = p.mats.data() + ((op.flags & ROP_COND) ? 0 : op.mat);
