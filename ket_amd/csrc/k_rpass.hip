@@ -28,7 +28,11 @@
 // Read straight through the scalar cache instead, every op waited on its
 // record from L2 (a copy program's op stream far exceeds that cache).  The
 // copy-bitmap words of conditional ops come with the chunk, one per lane of
-// every wave, and a ballot turns them into a 64-bit per-chunk mask.
+// every wave, and a ballot turns them into a 64-bit per-chunk mask of the ops
+// that run on the tile (unconditional ops have all-ones words); the op loop
+// walks its set bits, so an op skipped by a copy costs nothing.  (Looping over
+// every op instead, each skipped op paid the loop's copy of the 32-register
+// state from one register set to the other.)
 #include <algorithm>
 #include <cstring>
 
@@ -380,7 +384,12 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
         for (uint32_t c = 0; c < nch; c++) {
             const uint64_t* recs = reinterpret_cast<const uint64_t*>(stage[c & 1]);
             const uint32_t cn = nops - c * C < (uint32_t)C ? nops - c * C : (uint32_t)C;
-            for (uint32_t oi = 0; oi < cn; oi++) {
+            // the chunk's ops that run on this tile: the ballot of the copy-bitmap
+            // words (all ones for unconditional ops), else every op
+            uint64_t run = crow ? cmask : (cn >= 64 ? ~0ull : ((1ull << cn) - 1));
+            while (run) {
+                const uint32_t oi = (uint32_t)__builtin_ctzll(run);
+                run &= run - 1;
                 const uint64_t* w = recs + 16 * oi;
                 const uint64_t h0 = sreg(w[0]), h1 = sreg(w[1]), h2 = sreg(w[2]), h3 = sreg(w[3]);
                 const uint32_t kind = (uint32_t)h0 & 0xFF;
@@ -404,8 +413,6 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
                     continue;
                 }
                 if ((base & gctl) != gctl) continue;
-                // only on the tiles of the copies in the op's bitmap
-                if (((h1 >> 16) & ROP_COND) && !((cmask >> oi) & 1)) continue;
                 const bool lp = (tid & tctl) == tctl;
                 const bool sel = tctl != 0;
                 switch (kind) {

@@ -215,7 +215,7 @@ struct Program {
     std::vector<uint32_t> bits;      // copy bitmaps of conditional ops
     uint32_t* d_bits = nullptr;
     // register-round copy programs: the same bitmaps by op, word w of rops[g]'s
-    // at rbits[w * rops.size() + g] (0 for unconditional ops), so that a
+    // at rbits[w * rops.size() + g] (all ones for unconditional ops), so that a
     // workgroup reads the words of consecutive ops from one row
     std::vector<uint32_t> rbits;
     uint32_t* d_rbits = nullptr;

@@ -490,7 +490,7 @@ int plan_copies(Program& p, int ntot, int nsys, int prec, const ksv_op* ops, siz
         bool any = false;
         for (const ROp& r : p.rops) any = any || (r.flags & ROP_COND);
         if (any) {
-            p.rbits.assign(nw * nr, 0u);
+            p.rbits.assign(nw * nr, ~0u);       // unconditional: every copy
             for (size_t g = 0; g < nr; g++)
                 if (p.rops[g].flags & ROP_COND)
                     for (size_t w = 0; w < nw; w++) p.rbits[w * nr + g] = p.bits[(size_t)p.rops[g].mat + w];

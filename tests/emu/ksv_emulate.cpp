@@ -100,6 +100,11 @@ static int emu_reg(const Program& p, const Step& s, cd* psi, int n) {
                 }
                 continue;
             }
+            if (!p.rbits.empty() && !(op.flags & ROP_COND)) {
+                const uint64_t copy = base >> p.nsys;
+                const size_t at = (size_t)(copy >> 5) * p.rops.size() + s.op_begin + o;
+                CHECK(at < p.rbits.size() && p.rbits[at] == ~0u, "unconditional op without an all-ones by-op word");
+            }
             if ((base & op.gctl) != op.gctl) continue;
             if (op.flags & ROP_COND) {      // `mat` is the copy bitmap's word offset
                 CHECK(op.kind == R_M1 || op.kind == R_PERMX || (op.kind == R_DIAG && op.k <= 2), "conditional op of a kind with a matrix offset");
