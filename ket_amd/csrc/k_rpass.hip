@@ -223,9 +223,19 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
         v.y = im[j];
         lds[j * NT + tid] = v;
     }
+    // Amplitudes the op leaves alone come back from the thread's own LDS column
+    // (no barrier: same thread), so the old register values die at the writes
+    // above.  Kept live as fallbacks instead, they overlapped the new values
+    // and the op loop paid a copy of the whole state between two register sets
+    // on every op.
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
-        if (((uint32_t)j & rctl) != rctl) continue;
+        const V old = lds[j * NT + tid];
+        if (((uint32_t)j & rctl) != rctl) {
+            re[j] = old.x;
+            im[j] = old.y;
+            continue;
+        }
         uint32_t row = 0;
         for (uint32_t i = 0; i < k; i++) row |= (((uint32_t)j >> ((locw >> (8 * i)) & 0xFF)) & 1) << (k - 1 - i);
         const uint32_t jb = (uint32_t)j & ~tmask;
@@ -239,10 +249,9 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
             ai = __builtin_fma(e.x, x.y, __builtin_fma(e.y, x.x, ai));
         }
         prune2<T, PR>(ar, ai);
-        if (!sel || lp) {
-            re[j] = ar;
-            im[j] = ai;
-        }
+        const bool u = !sel || lp;
+        re[j] = u ? ar : old.x;
+        im[j] = u ? ai : old.y;
     }
 }
 
@@ -327,7 +336,14 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
     return ro;
 }
 
-template <typename T, int TB, bool PR>
+// DN: the kernel has the dense-op path.  Register allocation is global to the
+// kernel, and whether the op loop copies the 32-register state between two
+// register sets on every op depends on the whole body.  Measured on 12-qubit
+// tiles: with the dense path, ordinary programs run 1.7x faster (5443 against
+// 3219 gate-updates/s on the 30-qubit random circuit); without it, copy
+// programs run faster (42.9 against 62.5 ms for the 12-qubit noisy batch).  So
+// copy programs without dense ops take DN = false, every other step DN = true.
+template <typename T, int TB, bool PR, bool DN>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     using V = typename Vec2<T>::type;
     constexpr int K = TB + RSLOTS;
@@ -405,7 +421,7 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
                     r_exch<T, TB>(re, im, lds, w + 4, tid);
                     continue;
                 }
-                if (kind == R_DENSE) {
+                if (DN && kind == R_DENSE) {
                     // every thread reaches the barrier inside, whatever gctl says
                     const bool on = (base & gctl) == gctl;
                     const bool lp = on && ((tid & tctl) == tctl);
@@ -465,12 +481,18 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     }
 }
 
-template <typename T, bool PR>
+template <typename T, bool PR, bool DN>
 static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
     const dim3 grid((unsigned)a.ntiles);
-    if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR>), grid, dim3(1 << 7), 0, st, a);
-    else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR>), grid, dim3(1 << 8), 0, st, a);
-    else hipLaunchKernelGGL((k_rpass<T, 9, PR>), grid, dim3(1 << 9), 0, st, a);
+    if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR, DN>), grid, dim3(1 << 7), 0, st, a);
+    else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR, DN>), grid, dim3(1 << 8), 0, st, a);
+    else hipLaunchKernelGGL((k_rpass<T, 9, PR, DN>), grid, dim3(1 << 9), 0, st, a);
+}
+
+template <typename T, bool PR>
+static void launch_dn(const RPassArgs& a, int tb, bool dense, hipStream_t st) {
+    if (dense) launch_tb<T, PR, true>(a, tb, st);
+    else launch_tb<T, PR, false>(a, tb, st);
 }
 
 hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st) {
@@ -501,12 +523,14 @@ hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t 
     a.rstride = (uint32_t)p.rops.size();
     a.op_base = (uint32_t)s.op_begin;
     a.nsys = p.nsys;
+    bool dense = p.d_rbits == nullptr;       // see k_rpass: DN by program kind
+    for (int i = 0; i < s.op_count && !dense; i++) dense = p.rops[(size_t)s.op_begin + i].kind == R_DENSE;
     if (p.precision == KSV_F64) {
-        if (p.prune) launch_tb<double, true>(a, tb, st);
-        else launch_tb<double, false>(a, tb, st);
+        if (p.prune) launch_dn<double, true>(a, tb, dense, st);
+        else launch_dn<double, false>(a, tb, dense, st);
     } else {
-        if (p.prune) launch_tb<float, true>(a, tb, st);
-        else launch_tb<float, false>(a, tb, st);
+        if (p.prune) launch_dn<float, true>(a, tb, dense, st);
+        else launch_dn<float, false>(a, tb, dense, st);
     }
     return hipGetLastError();
 }
