@@ -223,19 +223,9 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
         v.y = im[j];
         lds[j * NT + tid] = v;
     }
-    // Amplitudes the op leaves alone come back from the thread's own LDS column
-    // (no barrier: same thread), so the old register values die at the writes
-    // above.  Kept live as fallbacks instead, they overlapped the new values
-    // and the op loop paid a copy of the whole state between two register sets
-    // on every op.
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
-        const V old = lds[j * NT + tid];
-        if (((uint32_t)j & rctl) != rctl) {
-            re[j] = old.x;
-            im[j] = old.y;
-            continue;
-        }
+        if (((uint32_t)j & rctl) != rctl) continue;
         uint32_t row = 0;
         for (uint32_t i = 0; i < k; i++) row |= (((uint32_t)j >> ((locw >> (8 * i)) & 0xFF)) & 1) << (k - 1 - i);
         const uint32_t jb = (uint32_t)j & ~tmask;
@@ -249,9 +239,10 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
             ai = __builtin_fma(e.x, x.y, __builtin_fma(e.y, x.x, ai));
         }
         prune2<T, PR>(ar, ai);
-        const bool u = !sel || lp;
-        re[j] = u ? ar : old.x;
-        im[j] = u ? ai : old.y;
+        if (!sel || lp) {
+            re[j] = ar;
+            im[j] = ai;
+        }
     }
 }
 
@@ -337,12 +328,10 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
 }
 
 // DN: the kernel has the dense-op path.  Register allocation is global to the
-// kernel, and whether the op loop copies the 32-register state between two
-// register sets on every op depends on the whole body.  Measured on 12-qubit
-// tiles: with the dense path, ordinary programs run 1.7x faster (5443 against
-// 3219 gate-updates/s on the 30-qubit random circuit); without it, copy
-// programs run faster (42.9 against 62.5 ms for the 12-qubit noisy batch).  So
-// copy programs without dense ops take DN = false, every other step DN = true.
+// kernel, so leaving that path out changes the code of every other op.  Copy
+// programs without dense ops (the batched noisy trajectories) run the build
+// without it: 42.9 against 44.5 ms for a 12-qubit batch of 10 240 copies.
+// Every other step runs DN = true.
 template <typename T, int TB, bool PR, bool DN>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     using V = typename Vec2<T>::type;
