@@ -69,6 +69,19 @@ __device__ __forceinline__ double sreg(double v) {
 }
 __device__ __forceinline__ float sreg(float v) { return __builtin_bit_cast(float, sreg(__builtin_bit_cast(uint32_t, v))); }
 
+// word i of an op record: staged in LDS (moved to scalar registers) or read
+// through the constant address space (already scalar)
+__device__ __forceinline__ uint64_t rec_word(const uint64_t* w, int i) { return sreg(w[i]); }
+__device__ __forceinline__ uint64_t rec_word(const KSV_CONST uint64_t* w, int i) { return w[i]; }
+__device__ __forceinline__ const double* rec_pay(const uint64_t* w, double*) { return reinterpret_cast<const double*>(w + 4); }
+__device__ __forceinline__ const float* rec_pay(const uint64_t* w, float*) { return reinterpret_cast<const float*>(w + 4); }
+__device__ __forceinline__ const KSV_CONST double* rec_pay(const KSV_CONST uint64_t* w, double*) {
+    return reinterpret_cast<const KSV_CONST double*>(w + 4);
+}
+__device__ __forceinline__ const KSV_CONST float* rec_pay(const KSV_CONST uint64_t* w, float*) {
+    return reinterpret_cast<const KSV_CONST float*>(w + 4);
+}
+
 // matrix entry i of an op record staged in LDS
 template <typename T>
 __device__ __forceinline__ CM<T> ldm(const T* m, int i) {
@@ -249,12 +262,12 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
 // LDS transpose from the current slot/thread assignment to the next one.
 // `xw`: the record's 32 16-bit entries (write address of thread bit 0..11,
 // of slot 0..3, read address likewise) as eight words.
-template <typename T, int TB>
-__device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, const uint64_t* xw, uint32_t tid) {
+template <typename T, int TB, typename P>
+__device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, P xw, uint32_t tid) {
     using V = typename Vec2<T>::type;
     uint64_t xs[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) xs[i] = sreg(xw[i]);
+    for (int i = 0; i < 8; i++) xs[i] = rec_word(xw, i);
     auto x = [&](int i) -> uint32_t { return (uint32_t)(xs[i >> 2] >> (16 * (i & 3))) & 0xFFFF; };
     uint32_t wb = 0, rb = 0;
 #pragma unroll
@@ -331,8 +344,12 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
 // kernel, so leaving that path out changes the code of every other op.  Copy
 // programs without dense ops (the batched noisy trajectories) run the build
 // without it: 42.9 against 44.5 ms for a 12-qubit batch of 10 240 copies.
-// Every other step runs DN = true.
-template <typename T, int TB, bool PR, bool DN>
+// STG: op records staged through LDS with the per-chunk run mask (copy
+// programs: long op streams, copy-conditional ops); without it each op reads
+// its record through the scalar cache, which holds an ordinary pass's short
+// op stream (3147 against 3367 ms for the 30-qubit random circuit in 11-qubit
+// register rounds).
+template <typename T, int TB, bool PR, bool DN, bool STG>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     using V = typename Vec2<T>::type;
     constexpr int K = TB + RSLOTS;
@@ -340,7 +357,7 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     constexpr int C = NT / 8;                 // op records per staged chunk: 16 bytes per thread
     static_assert(C <= 64 && sizeof(ROp) == 128, "one wave's ballot covers a chunk");
     __shared__ V lds[1 << K];
-    __shared__ uint4 stage[2][NT];            // two chunks of op records
+    __shared__ uint4 stage[STG ? 2 : 1][STG ? NT : 1];   // two chunks of op records
     const uint32_t tid = threadIdx.x;
     V* psi = reinterpret_cast<V*>(a.psi);
     const T* wide_mats = reinterpret_cast<const T*>(a.mats);
@@ -380,7 +397,67 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
         im[j] = t.y;
     }
 
-    {
+    // one op; `w`: its record (LDS or constant address space)
+    auto op_body = [&](auto w) {
+        const uint64_t h0 = rec_word(w, 0), h1 = rec_word(w, 1), h2 = rec_word(w, 2), h3 = rec_word(w, 3);
+        const uint32_t kind = (uint32_t)h0 & 0xFF;
+        const uint32_t sa = (uint32_t)(h0 >> 8) & 0xFF, sb = (uint32_t)(h0 >> 16) & 0xFF;
+        const uint32_t kq = (uint32_t)(h0 >> 24) & 0xFF;
+        const uint64_t locw = (h0 >> 32) | ((h1 & 0xFFFF) << 32);
+        const uint32_t tctl = (uint32_t)(h1 >> 32);
+        const uint32_t rctl = (uint32_t)h2;
+        const int32_t mat = (int32_t)(h2 >> 32);
+        const uint64_t gctl = h3;
+        const auto pay = rec_pay(w, (T*)nullptr);
+        if (kind == R_EXCH) {
+            r_exch<T, TB>(re, im, lds, w + 4, tid);
+            return;
+        }
+        if (DN && kind == R_DENSE) {
+            // every thread reaches the barrier inside, whatever gctl says
+            const bool on = (base & gctl) == gctl;
+            const bool lp = on && ((tid & tctl) == tctl);
+            r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
+            return;
+        }
+        if ((base & gctl) != gctl) return;
+        const bool lp = (tid & tctl) == tctl;
+        const bool sel = tctl != 0;
+        switch (kind) {
+        case R_M1:
+            if (rctl == 0) {
+#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
+                RSLOT_CASES(sa, F)
+#undef F
+            } else {
+#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
+                RSLOT_CASES(sa, F)
+#undef F
+            }
+            break;
+        case R_DIAG:
+            if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+            else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
+            break;
+        case R_PERMX:
+#define F(S) (sel ? r_permx<T, S, true>(re, im, rctl, lp) : r_permx<T, S, false>(re, im, rctl, lp))
+            RSLOT_CASES(sa, F)
+#undef F
+            break;
+        case R_SWAP:
+#define F(A, B) (sel ? r_swap<T, A, B, true>(re, im, rctl, lp) : r_swap<T, A, B, false>(re, im, rctl, lp))
+            RPAIR_CASES(sa, sb, F)
+#undef F
+            break;
+        default:
+            break;
+        }
+    };
+
+    if constexpr (!STG) {
+        const KSV_CONST uint64_t* cops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
+        for (int o = 0; o < a.nops; o++) op_body(cops + 16 * o);
+    } else {
         stage[0][tid] = fetch_rec(0);
         uint64_t cmask = cmask_of(fetch_cw(0));
         uint4 pre = fetch_rec(1);
@@ -395,60 +472,7 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
             while (run) {
                 const uint32_t oi = (uint32_t)__builtin_ctzll(run);
                 run &= run - 1;
-                const uint64_t* w = recs + 16 * oi;
-                const uint64_t h0 = sreg(w[0]), h1 = sreg(w[1]), h2 = sreg(w[2]), h3 = sreg(w[3]);
-                const uint32_t kind = (uint32_t)h0 & 0xFF;
-                const uint32_t sa = (uint32_t)(h0 >> 8) & 0xFF, sb = (uint32_t)(h0 >> 16) & 0xFF;
-                const uint32_t kq = (uint32_t)(h0 >> 24) & 0xFF;
-                const uint64_t locw = (h0 >> 32) | ((h1 & 0xFFFF) << 32);
-                const uint32_t tctl = (uint32_t)(h1 >> 32);
-                const uint32_t rctl = (uint32_t)h2;
-                const int32_t mat = (int32_t)(h2 >> 32);
-                const uint64_t gctl = h3;
-                const T* pay = reinterpret_cast<const T*>(w + 4);
-                if (kind == R_EXCH) {
-                    r_exch<T, TB>(re, im, lds, w + 4, tid);
-                    continue;
-                }
-                if (DN && kind == R_DENSE) {
-                    // every thread reaches the barrier inside, whatever gctl says
-                    const bool on = (base & gctl) == gctl;
-                    const bool lp = on && ((tid & tctl) == tctl);
-                    r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
-                    continue;
-                }
-                if ((base & gctl) != gctl) continue;
-                const bool lp = (tid & tctl) == tctl;
-                const bool sel = tctl != 0;
-                switch (kind) {
-                case R_M1:
-                    if (rctl == 0) {
-#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
-                        RSLOT_CASES(sa, F)
-#undef F
-                    } else {
-#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
-                        RSLOT_CASES(sa, F)
-#undef F
-                    }
-                    break;
-                case R_DIAG:
-                    if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
-                    else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
-                    break;
-                case R_PERMX:
-#define F(S) (sel ? r_permx<T, S, true>(re, im, rctl, lp) : r_permx<T, S, false>(re, im, rctl, lp))
-                    RSLOT_CASES(sa, F)
-#undef F
-                    break;
-                case R_SWAP:
-#define F(A, B) (sel ? r_swap<T, A, B, true>(re, im, rctl, lp) : r_swap<T, A, B, false>(re, im, rctl, lp))
-                    RPAIR_CASES(sa, sb, F)
-#undef F
-                    break;
-                default:
-                    break;
-                }
+                op_body(recs + 16 * oi);
             }
             if (c + 1 < nch) {
                 // chunk c-1's buffer: every thread finished it before the last barrier
@@ -459,29 +483,32 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
                 __syncthreads();
             }
         }
+    }
 
 #pragma unroll
-        for (int j = 0; j < RREGS; j++) {
-            V t;
-            t.x = re[j];
-            t.y = im[j];
-            psi[base | st_off | slot_offset<T>(j, a.st_r)] = t;
-        }
+    for (int j = 0; j < RREGS; j++) {
+        V t;
+        t.x = re[j];
+        t.y = im[j];
+        psi[base | st_off | slot_offset<T>(j, a.st_r)] = t;
     }
 }
 
-template <typename T, bool PR, bool DN>
+template <typename T, bool PR, bool DN, bool STG>
 static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
     const dim3 grid((unsigned)a.ntiles);
-    if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR, DN>), grid, dim3(1 << 7), 0, st, a);
-    else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR, DN>), grid, dim3(1 << 8), 0, st, a);
-    else hipLaunchKernelGGL((k_rpass<T, 9, PR, DN>), grid, dim3(1 << 9), 0, st, a);
+    if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR, DN, STG>), grid, dim3(1 << 7), 0, st, a);
+    else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR, DN, STG>), grid, dim3(1 << 8), 0, st, a);
+    else hipLaunchKernelGGL((k_rpass<T, 9, PR, DN, STG>), grid, dim3(1 << 9), 0, st, a);
 }
 
+// copy programs: staged records, with or without the dense path; every other
+// program: records through the scalar cache, with the dense path
 template <typename T, bool PR>
-static void launch_dn(const RPassArgs& a, int tb, bool dense, hipStream_t st) {
-    if (dense) launch_tb<T, PR, true>(a, tb, st);
-    else launch_tb<T, PR, false>(a, tb, st);
+static void launch_dn(const RPassArgs& a, int tb, bool copies, bool dense, hipStream_t st) {
+    if (!copies) launch_tb<T, PR, true, false>(a, tb, st);
+    else if (dense) launch_tb<T, PR, true, true>(a, tb, st);
+    else launch_tb<T, PR, false, true>(a, tb, st);
 }
 
 hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t st) {
@@ -512,14 +539,15 @@ hipError_t launch_rpass(const Program& p, const Step& s, void* psi, hipStream_t 
     a.rstride = (uint32_t)p.rops.size();
     a.op_base = (uint32_t)s.op_begin;
     a.nsys = p.nsys;
-    bool dense = p.d_rbits == nullptr;       // see k_rpass: DN by program kind
+    const bool copies = p.nsys > 0;
+    bool dense = false;
     for (int i = 0; i < s.op_count && !dense; i++) dense = p.rops[(size_t)s.op_begin + i].kind == R_DENSE;
     if (p.precision == KSV_F64) {
-        if (p.prune) launch_dn<double, true>(a, tb, dense, st);
-        else launch_dn<double, false>(a, tb, dense, st);
+        if (p.prune) launch_dn<double, true>(a, tb, copies, dense, st);
+        else launch_dn<double, false>(a, tb, copies, dense, st);
     } else {
-        if (p.prune) launch_dn<float, true>(a, tb, dense, st);
-        else launch_dn<float, false>(a, tb, dense, st);
+        if (p.prune) launch_dn<float, true>(a, tb, copies, dense, st);
+        else launch_dn<float, false>(a, tb, copies, dense, st);
     }
     return hipGetLastError();
 }
