@@ -21,12 +21,15 @@
 // amplitude per pass, coalesced: thread bits 0..2 are physical qubits 0..2
 // at load and store, so each 8 lanes move a full 128-byte run.
 //
-// Op records (128 bytes, matrix inline) are staged through LDS in chunks of
-// 2^TB / 8 records (16 bytes per thread, coalesced), double-buffered so that
-// the next chunk's global loads are in flight while the current one runs; each
-// op then reads its record from LDS (uniform address) into scalar registers.
-// Read straight through the scalar cache instead, every op waited on its
-// record from L2 (a copy program's op stream far exceeds that cache).  The
+// Op records are 128 bytes, matrix inline.  An ordinary pass reads them
+// through the constant address space (uniform scalar loads; its op stream fits
+// the scalar cache).  A copy program (batched trajectories) stages them through
+// LDS in chunks of 2^TB / 8 records (16 bytes per thread, coalesced),
+// double-buffered so that the next chunk's global loads are in flight while
+// the current one runs; each op then reads its record from LDS (uniform
+// address) into scalar registers.  Read straight through the scalar cache
+// instead, every op waited on its record from L2 (a copy program's op stream
+// far exceeds that cache).  The
 // copy-bitmap words of conditional ops come with the chunk, one per lane of
 // every wave, and a ballot turns them into a 64-bit per-chunk mask of the ops
 // that run on the tile (unconditional ops have all-ones words); the op loop
@@ -347,7 +350,7 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
 // STG: op records staged through LDS with the per-chunk run mask (copy
 // programs: long op streams, copy-conditional ops); without it each op reads
 // its record through the scalar cache, which holds an ordinary pass's short
-// op stream (3147 against 3367 ms for the 30-qubit random circuit in 11-qubit
+// op stream (2888 against 3367 ms for the 30-qubit random circuit in 11-qubit
 // register rounds).
 template <typename T, int TB, bool PR, bool DN, bool STG>
 __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
