@@ -248,6 +248,24 @@ _PAIR_B = np.array([1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3], dtype=np.uint8
 _TAPE_KINDS = ("single", "cnot", "controlled", "swap", "two", "toffoli", "cswap", "csrswap", "unitary")
 
 
+def _hits(dr, thr):
+    """Flat indices of dr < thr (row-major order), rows split over a few threads
+    (numpy's comparison and nonzero release the GIL)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    B, D = dr.shape
+    nt = min(8, os.cpu_count() or 1, max(1, B // 1024))
+    if nt <= 1:
+        return _sparse_flatnonzero(dr < thr)
+    cut = [B * i // nt for i in range(nt + 1)]
+
+    def part(i):
+        return cut[i] * D + _sparse_flatnonzero(dr[cut[i]:cut[i + 1]] < thr)
+
+    with ThreadPoolExecutor(nt) as ex:
+        return np.concatenate(list(ex.map(part, range(nt))))
+
+
 def _sparse_flatnonzero(mask):
     """np.flatnonzero for a mask with few set entries: scans 8 bytes at a time."""
     flat = np.ascontiguousarray(mask).reshape(-1).view(np.uint8)
@@ -306,7 +324,7 @@ def shot_events(flat, dr, p1, p2, pm):
             a = np.array(twos, dtype=np.int64)
             thr[a[:, 3]] = p2
             at_of[a[:, 3]], qa_of[a[:, 3]], qb_of[a[:, 3]] = a[:, 0], a[:, 1], a[:, 2]
-        c, col = np.divmod(_sparse_flatnonzero(dr < thr), D)
+        c, col = np.divmod(_hits(dr, thr), D)
         rv = dr[c, col]
         two = np.zeros(D, bool)
         if twos:

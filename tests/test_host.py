@@ -268,6 +268,19 @@ def test_batched_event_lists_match_the_column_loop(p1, p2, pm):
     assert any(e[1] == 0 for e in ev0) == bool(p1 or p2)
 
 
+def test_noise_hits_split_over_threads_match_flatnonzero():
+    """The draw-versus-rate comparison split by rows over threads, and the
+    8-bytes-at-a-time nonzero scan, give np.flatnonzero's indices in order."""
+    from ket_amd.noise import _hits, _sparse_flatnonzero
+    rs = np.random.default_rng(9)
+    for rows, cols in ((4096, 867), (3001, 13), (5, 3)):
+        dr = rs.random((rows, cols))
+        thr = np.where(rs.random(cols) < 0.8, 0.01, -1.0)
+        ref = np.flatnonzero(dr < thr)
+        assert np.array_equal(_hits(dr, thr), ref)
+        assert np.array_equal(_sparse_flatnonzero(dr < thr), ref)
+
+
 def test_prng_fill_continues_the_stream():
     """rng.fill(k) (ksv_xorshift32) gives the next k draws of the same stream."""
     a = make_prng(12345)
