@@ -344,9 +344,10 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
 }
 
 // DN: the kernel has the dense-op path.  Register allocation is global to the
-// kernel, so leaving that path out changes the code of every other op.  Copy
-// programs without dense ops (the batched noisy trajectories) run the build
-// without it: 42.9 against 44.5 ms for a 12-qubit batch of 10 240 copies.
+// kernel, so leaving that path out changes the code of every other op; steps
+// without dense ops run the build without it (42.9 against 44.5 ms for the
+// 12-qubit noisy batch of 10 240 copies; 2880 against 2888 ms for the 30-qubit
+// random circuit in 11-qubit register rounds).
 // STG: op records staged through LDS with the per-chunk run mask (copy
 // programs: long op streams, copy-conditional ops); without it each op reads
 // its record through the scalar cache, which holds an ordinary pass's short
@@ -505,12 +506,14 @@ static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
     else hipLaunchKernelGGL((k_rpass<T, 9, PR, DN, STG>), grid, dim3(1 << 9), 0, st, a);
 }
 
-// copy programs: staged records, with or without the dense path; every other
-// program: records through the scalar cache, with the dense path
+// copy programs: staged records; every other program: records through the
+// scalar cache; each with the dense path only when the step has dense ops
 template <typename T, bool PR>
 static void launch_dn(const RPassArgs& a, int tb, bool copies, bool dense, hipStream_t st) {
-    if (!copies) launch_tb<T, PR, true, false>(a, tb, st);
-    else if (dense) launch_tb<T, PR, true, true>(a, tb, st);
+    if (!copies) {
+        if (dense) launch_tb<T, PR, true, false>(a, tb, st);
+        else launch_tb<T, PR, false, false>(a, tb, st);
+    } else if (dense) launch_tb<T, PR, true, true>(a, tb, st);
     else launch_tb<T, PR, false, true>(a, tb, st);
 }
 
