@@ -273,7 +273,9 @@ int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out); /* on
 
 /* makePrng's stream (src/circuit.ts:813-819: xorshift32 <<13 >>>17 <<5,
  * r = s / 2^32) as `count` draws; *state advances.  Host-only: the batched
- * path draws every shot's numbers up front.                               */
+ * path draws every shot's numbers up front.  Fills of 2^20 draws or more are
+ * cut into chunks on up to 16 host threads, each started by a GF(2)
+ * jump-ahead; the output is the serial stream.                            */
 int ksv_xorshift32(uint32_t* state, size_t count, double* out);
 
 /* A whole batch in one call: the gate tape with events between its ops.
