@@ -165,6 +165,29 @@ __device__ __forceinline__ void r_swap(RREGS_ARGS, uint32_t rctl, bool lp) {
     }
 }
 
+// One-qubit diagonal diag(d0, d1).  S >= 0: the qubit is slot S, so each
+// register's entry is fixed at compile time; S < 0: the qubit is a thread bit
+// or a bit outside the tile, `hi` says which entry this thread takes.  The
+// same arithmetic as r_diag's, with the row selection out of the loop.
+template <typename T, int S, bool PR, bool SEL, typename P>
+__device__ __forceinline__ void r_diag1(RREGS_ARGS, uint32_t rctl, P pay, bool hi, bool lp) {
+    const CM<T> d0 = ldm(pay, 0), d1 = ldm(pay, 1);
+    CM<T> dt;
+    dt.x = hi ? d1.x : d0.x;
+    dt.y = hi ? d1.y : d0.y;
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) {
+        if (((uint32_t)j & rctl) != rctl) continue;
+        const CM<T> d = S >= 0 ? (((j >> (S < 0 ? 0 : S)) & 1) ? d1 : d0) : dt;
+        T yr = __builtin_fma(d.x, re[j], -(d.y * im[j]));
+        T yi = __builtin_fma(d.x, im[j], d.y * re[j]);
+        prune2<T, PR>(yr, yi);
+        if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
+        re[j] = yr;
+        im[j] = yi;
+    }
+}
+
 // Diagonal over k <= 6 qubits.  Row bit (k-1-i) of qubit i comes from a
 // thread bit, a slot or a physical bit outside the tile (loc[i]).  For
 // k <= 2 the entries are inline (scalar registers) and selected per
@@ -440,6 +463,19 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
             }
             break;
         case R_DIAG:
+            if (kq == 1) {
+                const uint32_t L = (uint32_t)locw & 0xFF;
+                if (L >= 16 && L < 32) {
+#define F(S) (sel ? r_diag1<T, S, PR, true>(re, im, rctl, pay, false, lp) : r_diag1<T, S, PR, false>(re, im, rctl, pay, false, lp))
+                    RSLOT_CASES(L - 16, F)
+#undef F
+                } else {
+                    const bool hi = L < 16 ? ((tid >> L) & 1) != 0 : ((base >> (L - 32)) & 1) != 0;
+                    if (sel) r_diag1<T, -1, PR, true>(re, im, rctl, pay, hi, lp);
+                    else r_diag1<T, -1, PR, false>(re, im, rctl, pay, hi, lp);
+                }
+                break;
+            }
             if (sel) r_diag<T, PR, true>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
             else r_diag<T, PR, false>(re, im, kq, locw, rctl, pay, wide_mats + 2 * mat, tid, base, lp);
             break;
