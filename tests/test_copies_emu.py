@@ -26,7 +26,8 @@ def _lib():
 
 
 @pytest.mark.parametrize("nsys,b,seed,prune", [(6, 3, 1, True), (8, 2, 2, True), (7, 4, 3, False), (5, 5, 4, True),
-                                               (11, 2, 5, True), (12, 2, 6, True), (13, 1, 7, False)])
+                                               (11, 2, 5, True), (12, 2, 6, True), (13, 1, 7, False),
+                                               (11, 6, 8, True)])           # 64 copies: two bitmap words per op
 def test_copies_with_pauli_events(nsys, b, seed, prune):
     rng = np.random.default_rng(seed)
     calls = random_calls(nsys, 60, seed=seed)
