@@ -129,11 +129,10 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) x[c] = T[id[c]];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
+                for (int r = 0; r < 4; r++) {       // ((m0 x0 + m1 x1) + m2 x2) + m3 x3, ket's order
                     V p = cmadd2(m[4 * r], x[0], m[4 * r + 1], x[1]);
-                    V q = cmadd2(m[4 * r + 2], x[2], m[4 * r + 3], x[3]);
-                    p.x += q.x;
-                    p.y += q.y;
+                    kacc(p.x, p.y, m[4 * r + 2].x, m[4 * r + 2].y, x[2].x, x[2].y);
+                    kacc(p.x, p.y, m[4 * r + 3].x, m[4 * r + 3].y, x[3].x, x[3].y);
                     T[id[r]] = prune_amp(p, a.prune);
                 }
             }
@@ -166,8 +165,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a) {
                         for (int c = 0; c < (1 << MK_MAX); c++) {
                             if (c < dim) {
                                 V mm = mat(op.mat + r * dim + c);
-                                acc.x += mm.x * x[c].x - mm.y * x[c].y;
-                                acc.y += mm.x * x[c].y + mm.y * x[c].x;
+                                kacc(acc.x, acc.y, mm.x, mm.y, x[c].x, x[c].y);
                             }
                         }
                         T[id[r]] = prune_amp(acc, a.prune);
@@ -273,8 +271,7 @@ __global__ __launch_bounds__(256) void k_big(BigArgs a) {
         acc.y = 0;
         for (int c = 0; c < dim; c++) {
             V mm = m[(size_t)r * dim + c];
-            acc.x += mm.x * x[c].x - mm.y * x[c].y;
-            acc.y += mm.x * x[c].y + mm.y * x[c].x;
+            kacc(acc.x, acc.y, mm.x, mm.y, x[c].x, x[c].y);
         }
         psi[gidx(r)] = prune_amp(acc, a.prune);
     }

@@ -416,6 +416,19 @@ struct ksv_batch {
     std::vector<QInst> inst;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+
+    // releases whatever was allocated, so every error return of create / run frees it
+    ~ksv_batch() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (int g = 0; g < 2; g++) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
+        void* bufs[] = {psi, lam, d_inst, d_winst, d_wstart, d_gamma, d_beta, d_part, d_out, d_ll, d_phs};
+        for (void* ptr : bufs)
+            if (ptr) (void)hipFree(ptr);
+        if (st) (void)hipStreamDestroy(st);
+    }
 };
 
 static thread_local std::string q_err;
@@ -526,10 +539,14 @@ int ksv_qaoa_batch_create(int count, const int32_t* nq, const int64_t* edge_off,
         for (int64_t e = edge_off[i]; e < edge_off[i + 1]; e++) {
             const int u = edges[2 * e], v = edges[2 * e + 1];
             if (u < 0 || v < 0 || u >= n || v >= n || u == v) return qfail(KSV_ERR_RANGE, "edge endpoint out of range");
+            // the cut counts each vertex pair once (adjacency bits), while qaoa() and
+            // maxCutHamiltonian() apply every listed edge: a repeat would be silently dropped
+            if (q.adj[u] & (1u << v)) return qfail(KSV_ERR_RANGE, "repeated edge (u, v) or (v, u) in one instance");
             q.adj[u] |= 1u << v;
             q.adj[v] |= 1u << u;
             q.nedges++;
         }
+        if (q.nedges >= QPMAX) return qfail(KSV_ERR_RANGE, "too many edges for the phase table");
         // pass B tile: qubits 0..L-1 and 12..n-1 (L = 24 - n); the mixer on tile bits L..11
         q.lb = 24 - n;
         q.mixb = 0xFFFu & ~((1u << q.lb) - 1);
@@ -580,8 +597,11 @@ int ksv_qaoa_batch_run(ksv_batch* b, const double* gamma, const double* beta, in
         int rc = qschedule(b, want_grad != 0);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(b->st, &gr);
-        if (rc) return rc;
-        if (e != hipSuccess) return qfail(KSV_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        if (rc || e != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            if (rc) return rc;
+            return qfail(KSV_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        }
         b->graph[g] = gr;
         QHIP(hipGraphInstantiate(&b->exec[g], gr, nullptr, nullptr, 0));
     }
@@ -632,23 +652,6 @@ int ksv_qaoa_batch_info(const ksv_batch* b, uint64_t* total_amplitudes, uint64_t
 
 int ksv_qaoa_batch_destroy(ksv_batch* b) {
     if (!b) return qfail(KSV_ERR_HANDLE, "null batch");
-    (void)hipStreamSynchronize(b->st);
-    for (int g = 0; g < 2; g++) {
-        if (b->exec[g]) (void)hipGraphExecDestroy(b->exec[g]);
-        if (b->graph[g]) (void)hipGraphDestroy(b->graph[g]);
-    }
-    (void)hipFree(b->psi);
-    (void)hipFree(b->lam);
-    (void)hipFree(b->d_inst);
-    (void)hipFree(b->d_winst);
-    (void)hipFree(b->d_wstart);
-    (void)hipFree(b->d_gamma);
-    (void)hipFree(b->d_beta);
-    (void)hipFree(b->d_part);
-    (void)hipFree(b->d_out);
-    (void)hipFree(b->d_ll);
-    (void)hipFree(b->d_phs);
-    (void)hipStreamDestroy(b->st);
     delete b;
     return KSV_OK;
 }

@@ -4,14 +4,13 @@
 // histograms for top-k.  Every sum accumulates in FP64 whatever the state
 // precision.  Reductions use a fixed grid and a fixed combine order, so a
 // result is bit-reproducible run to run.
-#include "ksv_internal.h"
+#include "kcommon.h"
 
 namespace ksv {
 
 template <typename V>
 __device__ __forceinline__ double prob(V a) {
-    double re = (double)a.x, im = (double)a.y;
-    return __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
+    return knorm2((double)a.x, (double)a.y);     // each product rounded, as ket's norm2
 }
 __device__ __forceinline__ uint64_t ins0_64r(uint64_t p, int q) {
     return ((p >> q) << (q + 1)) | (p & ((1ull << q) - 1));

@@ -97,15 +97,38 @@ __device__ __forceinline__ CM<T> ldm(const T* m, int i) {
 template <typename T, bool PR>
 __device__ __forceinline__ void prune2(T& r, T& i) {
     if (PR) {
-        if (((double)r * r + (double)i * i) < 1e-14) { r = 0; i = 0; }
+        if (knorm2((double)r, (double)i) < 1e-14) { r = 0; i = 0; }
     }
 }
 
-// (ore, oim) = a * x0 + b * x1, four fused multiply-adds per component pair
-template <typename T>
+// (ore, oim) = a * x0 + b * x1.  PR (ket's per-gate pruning): ket's
+// add(mul(a, x0), mul(b, x1)), every product and sum rounded on its own
+// (src/statevector.ts:61-62), so results and prune decisions are ket's bit
+// for bit; exact mode: four fused multiply-adds per component.
+template <typename T, bool PR>
 __device__ __forceinline__ void mac2(T& ore, T& oim, CM<T> a, T x0r, T x0i, CM<T> b, T x1r, T x1i) {
-    ore = __builtin_fma(a.x, x0r, __builtin_fma(-a.y, x0i, __builtin_fma(b.x, x1r, -(b.y * x1i))));
-    oim = __builtin_fma(a.x, x0i, __builtin_fma(a.y, x0r, __builtin_fma(b.x, x1i, b.y * x1r)));
+    if (PR) {
+        T pr, pi, qr, qi;
+        kmul(a.x, a.y, x0r, x0i, pr, pi);
+        kmul(b.x, b.y, x1r, x1i, qr, qi);
+        ore = rn_add(pr, qr);
+        oim = rn_add(pi, qi);
+    } else {
+        ore = __builtin_fma(a.x, x0r, __builtin_fma(-a.y, x0i, __builtin_fma(b.x, x1r, -(b.y * x1i))));
+        oim = __builtin_fma(a.x, x0i, __builtin_fma(a.y, x0r, __builtin_fma(b.x, x1i, b.y * x1r)));
+    }
+}
+
+// (yr, yi) = d * x: ket's mul under PR (a diagonal gate's other products are
+// zeros, which leave ket's sum unchanged), fused otherwise
+template <typename T, bool PR>
+__device__ __forceinline__ void dmul(T& yr, T& yi, T dx, T dy, T xr, T xi) {
+    if (PR) {
+        kmul(dx, dy, xr, xi, yr, yi);
+    } else {
+        yr = __builtin_fma(dx, xr, -(dy * xi));
+        yi = __builtin_fma(dx, xi, dy * xr);
+    }
 }
 
 #define RREGS_ARGS T (&re)[RREGS], T (&im)[RREGS]
@@ -122,8 +145,8 @@ __device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp) {
         const int j1 = j | (1 << S);
         const T x0r = re[j], x0i = im[j], x1r = re[j1], x1i = im[j1];
         T y0r, y0i, y1r, y1i;
-        mac2(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
-        mac2(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
+        mac2<T, PR>(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
+        mac2<T, PR>(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
         prune2<T, PR>(y0r, y0i);
         prune2<T, PR>(y1r, y1i);
         if (SEL) {
@@ -179,8 +202,8 @@ __device__ __forceinline__ void r_diag1(RREGS_ARGS, uint32_t rctl, P pay, bool h
     for (int j = 0; j < RREGS; j++) {
         if (((uint32_t)j & rctl) != rctl) continue;
         const CM<T> d = S >= 0 ? (((j >> (S < 0 ? 0 : S)) & 1) ? d1 : d0) : dt;
-        T yr = __builtin_fma(d.x, re[j], -(d.y * im[j]));
-        T yi = __builtin_fma(d.x, im[j], d.y * re[j]);
+        T yr, yi;
+        dmul<T, PR>(yr, yi, d.x, d.y, re[j], im[j]);
         prune2<T, PR>(yr, yi);
         if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
         re[j] = yr;
@@ -221,8 +244,8 @@ __device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, ui
             const uint32_t row = rl | ((j & 1) ? s0 : 0) | ((j & 2) ? s1 : 0) | ((j & 4) ? s2 : 0) | ((j & 8) ? s3 : 0);
             const T dx = (row & 2) ? ((row & 1) ? d3.x : d2.x) : ((row & 1) ? d1.x : d0.x);
             const T dy = (row & 2) ? ((row & 1) ? d3.y : d2.y) : ((row & 1) ? d1.y : d0.y);
-            T yr = __builtin_fma(dx, re[j], -(dy * im[j]));
-            T yi = __builtin_fma(dx, im[j], dy * re[j]);
+            T yr, yi;
+            dmul<T, PR>(yr, yi, dx, dy, re[j], im[j]);
             prune2<T, PR>(yr, yi);
             if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
             re[j] = yr;
@@ -234,8 +257,8 @@ __device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, ui
             if (((uint32_t)j & rctl) != rctl) continue;
             const uint32_t row = rl | ((j & 1) ? s0 : 0) | ((j & 2) ? s1 : 0) | ((j & 4) ? s2 : 0) | ((j & 8) ? s3 : 0);
             const T dx = wide[2 * row], dy = wide[2 * row + 1];
-            T yr = __builtin_fma(dx, re[j], -(dy * im[j]));
-            T yi = __builtin_fma(dx, im[j], dy * re[j]);
+            T yr, yi;
+            dmul<T, PR>(yr, yi, dx, dy, re[j], im[j]);
             prune2<T, PR>(yr, yi);
             if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
             re[j] = yr;
@@ -274,8 +297,12 @@ __device__ __forceinline__ void r_dense(RREGS_ARGS, typename Vec2<T>::type* lds,
             for (uint32_t i = 0; i < k; i++) jc |= ((c >> (k - 1 - i)) & 1) << ((locw >> (8 * i)) & 0xFF);
             const V x = lds[jc * NT + tid];
             const CM<T> e = ldm(m, (int)(row * dim + c));
-            ar = __builtin_fma(e.x, x.x, __builtin_fma(-e.y, x.y, ar));
-            ai = __builtin_fma(e.x, x.y, __builtin_fma(e.y, x.x, ai));
+            if (PR) {       // add(out, mul(m, x)) in column order (src/statevector.ts:125-126, :263-264)
+                kacc(ar, ai, e.x, e.y, x.x, x.y);
+            } else {
+                ar = __builtin_fma(e.x, x.x, __builtin_fma(-e.y, x.y, ar));
+                ai = __builtin_fma(e.x, x.y, __builtin_fma(e.y, x.x, ai));
+            }
         }
         prune2<T, PR>(ar, ai);
         if (!sel || lp) {

@@ -8,14 +8,13 @@
 // reduction repeats, copy by copy, the exact addition order of its
 // single-state counterpart in k_reduce.hip, so a copy's outcome and draw
 // equal those of the same shot simulated alone.
-#include "ksv_internal.h"
+#include "kcommon.h"
 
 namespace ksv {
 
 template <typename V>
 __device__ __forceinline__ double sprob(V a) {
-    double re = (double)a.x, im = (double)a.y;
-    return __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
+    return knorm2((double)a.x, (double)a.y);     // each product rounded, as ket's norm2
 }
 __device__ __forceinline__ uint64_t sins0(uint64_t p, int q) { return ((p >> q) << (q + 1)) | (p & ((1ull << q) - 1)); }
 

@@ -5,27 +5,69 @@
 
 namespace ksv {
 
+// ket's arithmetic, rounded step by step.  JavaScript rounds every product
+// and every sum on its own (add / mul / norm2, src/complex.ts:10-21); clang
+// contracts a * b + c into one fused multiply-add by default, and
+// __dmul_rn / __dadd_rn are plain * and + in this toolchain, contracted like
+// any other expression.  An operation written under contract(off) carries no
+// contraction flag, so it stays unfused after inlining into any kernel.
+template <typename T>
+__device__ __forceinline__ T rn_mul(T a, T b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+template <typename T>
+__device__ __forceinline__ T rn_add(T a, T b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+template <typename T>
+__device__ __forceinline__ T rn_sub(T a, T b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+
+// norm2 (src/complex.ts:21) in FP64: re * re + im * im, each step rounded
+__device__ __forceinline__ double knorm2(double re, double im) { return rn_add(rn_mul(re, re), rn_mul(im, im)); }
+
+// mul (src/complex.ts:13-14): (a.re b.re - a.im b.im, a.re b.im + a.im b.re)
+template <typename T>
+__device__ __forceinline__ void kmul(T ar, T ai, T br, T bi, T& re, T& im) {
+    re = rn_sub(rn_mul(ar, br), rn_mul(ai, bi));
+    im = rn_add(rn_mul(ar, bi), rn_mul(ai, br));
+}
+
+// out += m * x, as add(out, mul(m, x)) (applyTwo / applyUnitary's inner loop,
+// src/statevector.ts:125-126, :263-264)
+template <typename T>
+__device__ __forceinline__ void kacc(T& ore, T& oim, T mr, T mi, T xr, T xi) {
+    T pr, pi;
+    kmul(mr, mi, xr, xi, pr, pi);
+    ore = rn_add(ore, pr);
+    oim = rn_add(oim, pi);
+}
+
 template <typename V>
 __device__ __forceinline__ V cmul(V a, V b) {
     V r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
+    kmul(a.x, a.y, b.x, b.y, r.x, r.y);
     return r;
 }
 
-// m0 * x0 + m1 * x1 (complex)
+// add(mul(m0, x0), mul(m1, x1)) (applySingle, src/statevector.ts:61-62)
 template <typename V>
 __device__ __forceinline__ V cmadd2(V m0, V x0, V m1, V x1) {
-    V r;
-    r.x = (m0.x * x0.x - m0.y * x0.y) + (m1.x * x1.x - m1.y * x1.y);
-    r.y = (m0.x * x0.y + m0.y * x0.x) + (m1.x * x1.y + m1.y * x1.x);
+    V a = cmul(m0, x0), b = cmul(m1, x1), r;
+    r.x = rn_add(a.x, b.x);
+    r.y = rn_add(a.y, b.y);
     return r;
 }
 
 // ket's accumulate(): an amplitude with |a|^2 < 1e-14 becomes 0
+// (isNegligible, src/complex.ts:24; src/statevector.ts:19-27)
 template <typename V>
 __device__ __forceinline__ V prune_amp(V v, int prune) {
-    if (prune && ((double)v.x * v.x + (double)v.y * v.y) < 1e-14) { v.x = 0; v.y = 0; }
+    if (prune && knorm2((double)v.x, (double)v.y) < 1e-14) { v.x = 0; v.y = 0; }
     return v;
 }
 

@@ -189,8 +189,9 @@ def draws_per_shot(flat, n, p1, p2, pm):
 def run_batched(circ, shots, rng, noise, init, creg_counts):
     """All shots of a circuit without `if` as copies of one device state.
 
-    Shot s consumes draws [s D, (s + 1) D) of the stream, so they are drawn
-    up front; gates go to every copy at once, Pauli errors, measurements,
+    Shot s consumes draws [s D, (s + 1) D) of the stream, so each batch's
+    draws are the next cnt * D of it, drawn when the batch starts (host
+    memory holds one batch of draws, not all shots); gates go to every copy at once, Pauli errors, measurements,
     resets and the final draw copy by copy (ksv_shots_*).  Gives the same
     counts as `run_per_shot` for the same seed.
     """
@@ -201,10 +202,12 @@ def run_batched(circ, shots, rng, noise, init, creg_counts):
     p2 = (noise or {}).get("p2", 0) or 0
     pm = (noise or {}).get("pMeas", 0) or 0
     D = draws_per_shot(flat, n, p1, p2, pm)
-    if hasattr(rng, "fill"):
-        draws = rng.fill(shots * D).reshape(shots, D)
-    else:
-        draws = np.fromiter((rng() for _ in range(shots * D)), dtype=np.float64, count=shots * D).reshape(shots, D)
+
+    def next_draws(cnt):
+        if hasattr(rng, "fill"):
+            return rng.fill(cnt * D).reshape(cnt, D)
+        return np.fromiter((rng() for _ in range(cnt * D)), dtype=np.float64, count=cnt * D).reshape(cnt, D)
+
     b = 0
     while (1 << (b + 1)) <= max(1, shots) and (1 << (b + 1)) <= MAX_COPIES and (2 << b) << n <= MAX_BATCH_AMPS:
         b += 1
@@ -219,11 +222,12 @@ def run_batched(circ, shots, rng, noise, init, creg_counts):
             Bc = 1 << bb
             if bb not in states:
                 states[bb] = State(n + bb, OPTIONS["precision"], prune=OPTIONS["prune"], tile=OPTIONS["tile"])
+            draws = next_draws(cnt)                   # the stream is sequential: batch by batch
             if cnt == Bc:
-                dr = draws[s0:s0 + Bc]                # rows of a C-ordered array: contiguous
+                dr = draws
             else:                                     # the last copies of a short batch idle on 0.5s
                 dr = np.full((Bc, D), 0.5)
-                dr[:cnt] = draws[s0:s0 + cnt]
+                dr[:cnt] = draws
             regs = {name: np.zeros((Bc, size), dtype=bool) for name, size in circ._cregs.items()}
             idx = _run_copies(states[bb], n, flat, dr, p1, p2, pm, init, regs, memo)
             for t in range(cnt):

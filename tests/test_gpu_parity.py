@@ -1,6 +1,9 @@
 """GPU parity: libketsv (through the C ABI) against the CPU oracle.
 
-Bars: amplitudes and probabilities within 1e-12 (FP64), seeded shot
+Bars: under ket's per-gate pruning (the default mode) amplitudes and
+probabilities equal the oracle's bit for bit: the kernels round every
+product and sum in ket's order, and the oracle is built without FMA
+contraction.  Exact (unpruned) mode: within 1e-12 (FP64).  Seeded shot
 indices identical, gradients within 1e-8 of the parameter-shift rule.
 """
 import math
@@ -41,13 +44,13 @@ def test_config1_probs_and_shots():
     ref = dense_ref(n, calls, prune=True)
     c = build(n, calls)
     got = c.statevectorDense()
-    assert np.max(np.abs(got - ref)) < TOL
+    assert np.array_equal(got, ref)
     probs = c.exactProbs()
     idx, p = O.probabilities(n, ref)
     assert len(probs) == len(idx)
     for i, v in zip(idx.tolist(), p.tolist()):
         bs = format(i, "b").zfill(n)[::-1]
-        assert abs(probs[bs] - v) < TOL
+        assert probs[bs] == v
     dist = c.run(shots=10000, seed=42)
     r = O.prng(42, 10000)
     want = O.sample(n, ref, r)
@@ -63,16 +66,24 @@ def test_random_circuits_every_op(n, prune):
     calls = random_calls(n, 150, seed=1000 + n)
     ref = dense_ref(n, calls, prune=prune)
     got = gpu_state(n, calls, prune=prune)
-    assert np.max(np.abs(got - ref)) < TOL
+    if prune:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < TOL
 
 
 @pytest.mark.parametrize("n,tile", [(14, 8), (15, 10), (16, 12), (17, 11), (18, 13), (18, 12)])
-def test_multi_pass_tiles(n, tile):
-    """LDS-tile passes: global controls, diagonal gates on global qubits."""
-    calls = random_calls(n, 120, seed=7 * n + tile, with_unitary=False)
-    ref = dense_ref(n, calls, prune=False)
-    got = gpu_state(n, calls, prune=False, tile=tile, kernel="lds")
-    assert np.max(np.abs(got - ref)) < TOL
+@pytest.mark.parametrize("prune", [False, True])
+def test_multi_pass_tiles(n, tile, prune):
+    """LDS-tile passes: global controls, diagonal gates on global qubits; under
+    pruning bit-equal to the oracle."""
+    calls = random_calls(n, 120, seed=7 * n + tile, with_unitary=prune)
+    ref = dense_ref(n, calls, prune=prune)
+    got = gpu_state(n, calls, prune=prune, tile=tile, kernel="lds")
+    if prune:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < TOL
 
 
 @pytest.mark.parametrize("n,rtile", [(11, 11), (12, 12), (13, 12), (13, 13), (14, 11), (16, 12), (17, 13), (18, 12)])
@@ -82,7 +93,10 @@ def test_register_round_passes(n, rtile, prune):
     calls = random_calls(n, 160, seed=31 * n + rtile, with_unitary=True)
     ref = dense_ref(n, calls, prune=prune)
     got = gpu_state(n, calls, prune=prune, kernel="reg", rtile=rtile)
-    assert np.max(np.abs(got - ref)) < TOL
+    if prune:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < TOL
 
 
 @pytest.mark.parametrize("n,btile,bq,nt", [(12, 12, 4, 256), (14, 12, 4, 256), (15, 13, 4, 256), (15, 13, 5, 256),
@@ -137,7 +151,8 @@ def test_fused_one_qubit_runs_exact_mode():
     assert np.max(np.abs(got - ref)) < TOL
 
 
-def test_wide_unitary():
+@pytest.mark.parametrize("prune", [False, True])
+def test_wide_unitary(prune):
     import random as _r
     from helpers import random_unitary
     rnd = _r.Random(5)
@@ -146,9 +161,12 @@ def test_wide_unitary():
     calls.append(("unitary", (random_unitary(32, rnd), 0, 4, 2, 7, 8)))
     calls.append(("unitary", (random_unitary(64, rnd), 8, 1, 3, 5, 6, 0)))
     calls.append(("unitary", (random_unitary(16, rnd), 3, 1, 8, 2)))
-    ref = dense_ref(n, calls, prune=False)
-    got = gpu_state(n, calls, prune=False)
-    assert np.max(np.abs(got - ref)) < TOL
+    ref = dense_ref(n, calls, prune=prune)
+    got = gpu_state(n, calls, prune=prune)
+    if prune:
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref)) < TOL
 
 
 def test_qft_closed_form():
@@ -177,6 +195,7 @@ def test_readouts_against_oracle():
     calls = random_calls(n, 120, seed=99)
     ref = dense_ref(n, calls, prune=True)
     c = build(n, calls)
+    assert np.array_equal(c.statevectorDense(), ref)
     m = c.marginals()
     assert np.max(np.abs(np.array(m) - O.marginals(n, ref))) < TOL
     for q in (0, 4, 8):

@@ -407,3 +407,13 @@ def test_batched_shot_driver_on_the_oracle(monkeypatch, noise, seed):
     ref, rcr = O.run_shots(n, calls, shots, seed, params, cregs={"c": 2})
     assert list(counts.items()) == list(ref.items())
     assert cr == rcr
+
+
+@pytest.mark.parametrize("edges", [[(0, 1), (0, 1)], [(2, 5), (5, 2)], [(0, 1), (1, 2), (3, 1), (2, 1)]])
+def test_qaoa_batch_rejects_repeated_edges(edges):
+    """qaoa() and maxCutHamiltonian() apply every listed edge (src/algorithms.ts:231-276)
+    while the batch's cut counts each vertex pair once: a repeat, in either order, is a
+    RangeError from create, before any device call."""
+    from ket_amd._native import QaoaBatch, RangeError
+    with pytest.raises(RangeError, match="repeated edge"):
+        QaoaBatch([(14, [(0, 3)]), (14, edges)], p=2)
