@@ -416,3 +416,33 @@ def test_dm_csrswap_with_definite_control_is_unitary():
     psi = O.zero_state(3)
     O.apply_named(3, psi, calls, prune=False)
     assert np.max(np.abs(O.run_dm(3, calls) - np.outer(psi, psi.conj()))) < 1e-12
+
+
+@pytest.mark.parametrize("n,heavy", [(8, False), (10, True), (12, True)])
+def test_layer0_restatements_used_at_30_qubits(n, heavy):
+    """The two per-index restatements the 30-qubit GPU test relies on equal the
+    oracle on whole states: the product-state closed form in exact mode, and ket's
+    pruned evolution bit for bit (with angles that make most amplitudes prune)."""
+    import random
+    from test_gpu_configs import brick_product_amplitudes, ket_pruned_layer0
+    from ket_amd import workloads
+    if heavy:
+        rnd = random.Random(n)
+        calls = []
+        for q in range(n):
+            s = rnd.choice([1e-4, 3e-3, 1.0, 2e-5])
+            calls += [("rx", (s * rnd.uniform(-1, 1), q)), ("ry", (s * rnd.uniform(-1, 1), q)),
+                      ("rz", (rnd.uniform(0, 6), q))]
+        calls += [("cnot", (a, a + 1)) for a in range(0, n - 1, 2)]
+    else:
+        calls = workloads.random_layer_calls(n, 1, seed=42)
+    idx = np.arange(1 << n, dtype=np.uint64)
+    exact = O.zero_state(n)
+    O.apply_named(n, exact, calls, prune=False)
+    pruned = O.zero_state(n)
+    O.apply_named(n, pruned, calls, prune=True)
+    assert np.max(np.abs(brick_product_amplitudes(n, calls, idx) - exact)) < 1e-14
+    got = ket_pruned_layer0(n, calls, idx)
+    assert np.array_equal(got, pruned)
+    if heavy:
+        assert np.count_nonzero(pruned == 0) > (1 << n) // 2
