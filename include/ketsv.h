@@ -195,7 +195,11 @@ int ksv_reduce_1q(ksv_state* s, int q, double out[4]);
 /* Shot sampling of run()'s fast path (src/circuit.ts:3425-3450): probabilities
  * p > 1e-14 in ascending index order, inclusive CDF with its last entry
  * forced to 1.0, and for each r the first index with cdf >= r.  The host
- * supplies the xorshift32 draws r (src/circuit.ts:813-819) in ket's order. */
+ * supplies the xorshift32 draws r (src/circuit.ts:813-819) in ket's order.
+ * Up to 24 qubits (ket's own range) the CDF is ket's sequential sum, every
+ * addition rounded in ket's order, so each draw in [0, 1) lands on ket's
+ * index however close it is to a CDF boundary; above 24 qubits the CDF comes
+ * from a chunked scan whose sums are associated differently. */
 int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out);
 
 /* Pauli expectations.  Term t is coeff[t] * prod_q P_q with
@@ -269,7 +273,11 @@ int ksv_shots_pauli(ksv_state* s, int n, int q, const uint32_t* copy,   /* pauli
                     const uint8_t* pauli, size_t count);                 /* qubit q of copy[i]          */
 int ksv_shots_measure(ksv_state* s, int n, int q, const double* r,      /* outcome = r < P(q = 1);     */
                       const uint8_t* reset, uint8_t* outcome);           /* project; reset[t]: to |0>   */
-int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out); /* one draw per copy        */
+/* one draw per copy by sampleSV (src/circuit.ts:74-81): cum += p over the copy's
+ * non-zero amplitudes in ascending index, each addition rounded in ket's order;
+ * the first index with r <= cum, else the last, else 0 (with n == s->n: the
+ * single state, the per-shot driver's final draw) */
+int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out);
 
 /* makePrng's stream (src/circuit.ts:813-819: xorshift32 <<13 >>>17 <<5,
  * r = s / 2^32) as `count` draws; *state advances.  Host-only: the batched

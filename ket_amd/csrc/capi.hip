@@ -41,6 +41,8 @@ struct ksv_state {
     hipStream_t st = nullptr;
     void* psi = nullptr;
     void* psi2 = nullptr;           // second state buffer, allocated on first use
+    void* ws = nullptr;             // read-out scratch: grows on demand, freed with the state
+    size_t ws_bytes = 0;
     double* d_part = nullptr;       // NBLK x 64 partials
     double* d_small = nullptr;      // 64 results
     int prune = 1;
@@ -64,6 +66,38 @@ struct ksv_program : public Program {
 };
 
 static size_t amp_bytes(const ksv_state* s) { return s->prec == KSV_F64 ? 16 : 8; }
+
+// Read-out scratch.  A call adds up what it needs (wsz), reserves it in the
+// state's workspace, and carves 256-byte-aligned pieces from it (Carve).  The
+// workspace only grows and stays with the state, so a read-out allocates
+// nothing once it is large enough, and an error return leaves nothing behind.
+static size_t wsz(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255; }
+
+static int ws_reserve(ksv_state* s, size_t bytes) {
+    if (bytes <= s->ws_bytes) return KSV_OK;
+    hipError_t e = hipStreamSynchronize(s->st);       // the old workspace may still be in use
+    if (e != hipSuccess) return fail(KSV_ERR_DEVICE, std::string("read-out scratch: ") + hipGetErrorString(e));
+    if (s->ws) (void)hipFree(s->ws);
+    s->ws = nullptr;
+    s->ws_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, (size_t)1 << 20);
+    if (hipMalloc(&s->ws, want) != hipSuccess) {
+        s->ws = nullptr;
+        return fail(KSV_ERR_DEVICE, "out of device memory for read-out scratch (" + std::to_string(want) + " bytes)");
+    }
+    s->ws_bytes = want;
+    return KSV_OK;
+}
+
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <typename T> T* take(size_t count) {
+        T* p = reinterpret_cast<T*>(base + off);
+        off += wsz(count * sizeof(T));
+        return p;
+    }
+};
 static size_t state_bytes(const ksv_state* s) { return amp_bytes(s) << s->n; }
 
 static int default_tile(const ksv_state* s) {
@@ -126,6 +160,7 @@ int ksv_destroy(ksv_state* s) {
     (void)hipStreamSynchronize(s->st);
     (void)hipFree(s->psi);
     if (s->psi2) (void)hipFree(s->psi2);
+    if (s->ws) (void)hipFree(s->ws);
     (void)hipFree(s->d_part);
     (void)hipFree(s->d_small);
     for (auto e : s->ev) (void)hipEventDestroy(e);
@@ -541,10 +576,9 @@ int ksv_reduce_1q(ksv_state* s, int q, double out[4]) {
 static int compact(ksv_state* s, int mode, double thresh, uint64_t kv, uint64_t* idx, double* p, uint64_t cap,
                    uint64_t* total) {
     const uint64_t nch = ((1ull << s->n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG;
-    uint32_t* d_cnt = nullptr;
-    uint64_t* d_off = nullptr;
-    HIPCHK(hipMalloc(&d_cnt, 4 * nch));
-    HIPCHK(hipMalloc(&d_off, 8 * nch));
+    int rc = ws_reserve(s, wsz(4 * nch));
+    if (rc) return rc;
+    uint32_t* d_cnt = Carve{(char*)s->ws}.take<uint32_t>(nch);
     if (mode == 0) HIPCHK(chunk_counts(s->n, s->prec, s->psi, thresh, d_cnt, s->st));
     else HIPCHK(chunk_counts_pred(s->n, s->prec, s->psi, mode, kv, d_cnt, s->st));
     std::vector<uint32_t> cnt(nch);
@@ -556,21 +590,19 @@ static int compact(ksv_state* s, int mode, double thresh, uint64_t kv, uint64_t*
     *total = acc;
     const uint64_t m = std::min(acc, cap);
     if (m) {
-        uint64_t* d_idx = nullptr;
-        double* d_p = nullptr;
+        rc = ws_reserve(s, wsz(8 * nch) + 2 * wsz(8 * m));
+        if (rc) return rc;
+        Carve c{(char*)s->ws};
+        uint64_t* d_off = c.take<uint64_t>(nch);
+        uint64_t* d_idx = c.take<uint64_t>(m);
+        double* d_p = c.take<double>(m);
         HIPCHK(hipMemcpyAsync(d_off, off.data(), 8 * nch, hipMemcpyHostToDevice, s->st));
-        HIPCHK(hipMalloc(&d_idx, 8 * m));
-        HIPCHK(hipMalloc(&d_p, 8 * m));
         if (mode == 0) HIPCHK(chunk_compact(s->n, s->prec, s->psi, thresh, d_off, d_idx, d_p, m, s->st));
         else HIPCHK(chunk_compact_pred(s->n, s->prec, s->psi, mode, kv, d_off, d_idx, d_p, m, s->st));
         if (idx) HIPCHK(hipMemcpyAsync(idx, d_idx, 8 * m, hipMemcpyDeviceToHost, s->st));
         if (p) HIPCHK(hipMemcpyAsync(p, d_p, 8 * m, hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
-        hipFree(d_idx);
-        hipFree(d_p);
     }
-    hipFree(d_cnt);
-    hipFree(d_off);
     return KSV_OK;
 }
 
@@ -590,8 +622,9 @@ int ksv_topk(ksv_state* s, size_t k, uint64_t* idx, double* p) {
     if (k > N) return fail(KSV_ERR_RANGE, "k exceeds the number of amplitudes");
     if (!idx || !p) return fail(KSV_ERR_TYPE, "null output buffer");
     // radix-select the k-th largest key (bit pattern of p) with 12-bit digits
-    uint32_t* d_hist = nullptr;
-    HIPCHK(hipMalloc(&d_hist, 4 * 4096));
+    int rc = ws_reserve(s, wsz(4 * 4096));
+    if (rc) return rc;
+    uint32_t* d_hist = Carve{(char*)s->ws}.take<uint32_t>(4096);
     std::vector<uint32_t> h(4096);
     uint64_t prefix = 0, above = 0;   // `above` = keys strictly greater than the current prefix range
     uint64_t want = k;                // rank (1-based) of the target within the prefix range
@@ -614,13 +647,12 @@ int ksv_topk(ksv_state* s, size_t k, uint64_t* idx, double* p) {
         want -= cum;
         prefix = (prefix << (hi - lo)) | (uint64_t)d;
     }
-    hipFree(d_hist);
     const uint64_t kth = prefix;      // full 64-bit key of the k-th largest
     // keys > kth: `above` of them, all wanted; keys == kth: the first (k - above) by index
     std::vector<uint64_t> ig(above + 1), ie(k - above + 1);
     std::vector<double> pg(above + 1), pe(k - above + 1);
     uint64_t tg = 0, te = 0;
-    int rc = compact(s, 2, 0, kth, ig.data(), pg.data(), above, &tg);
+    rc = compact(s, 2, 0, kth, ig.data(), pg.data(), above, &tg);
     if (rc) return rc;
     rc = compact(s, 1, 0, kth, ie.data(), pe.data(), k - above, &te);
     if (rc) return rc;
@@ -639,48 +671,71 @@ int ksv_topk(ksv_state* s, size_t k, uint64_t* idx, double* p) {
     return KSV_OK;
 }
 
+// ket's run() draw (src/circuit.ts:3425-3450) on ket's sequential CDF: one
+// wave forms the running sums in ket's addition order (d_cum: 2^n doubles),
+// then one thread per draw searches them.  d_r / d_out hold `shots` doubles /
+// indices; the draws are uploaded and the indices read back on the state stream.
+static int sample_seq(ksv_state* s, const double* r, size_t shots, uint64_t* out, double* d_cum,
+                      unsigned long long* d_ends, double* d_r, uint64_t* d_out) {
+    HIPCHK(seq_cdf(s->n, s->prec, s->psi, 1e-14, d_cum, d_ends, s->st));
+    HIPCHK(hipMemcpyAsync(d_r, r, 8 * shots, hipMemcpyHostToDevice, s->st));
+    HIPCHK(cdf_search(d_cum, d_ends, d_r, shots, d_out, s->st));
+    HIPCHK(hipMemcpyAsync(out, d_out, 8 * shots, hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return KSV_OK;
+}
+
 int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out) {
     if (!s) return fail(KSV_ERR_HANDLE, "null state");
     if (!shots) return KSV_OK;
     if (!r || !out) return fail(KSV_ERR_TYPE, "null buffer");
     const double thresh = 1e-14;
+    if (s->n <= SEQ_CDF_MAX_QUBITS) {
+        const uint64_t N = 1ull << s->n;
+        int rc = ws_reserve(s, wsz(8 * N) + wsz(16) + 2 * wsz(8 * shots));
+        if (rc) return rc;
+        Carve c{(char*)s->ws};
+        double* d_cum = c.take<double>(N);
+        unsigned long long* d_ends = c.take<unsigned long long>(2);
+        double* d_r = c.take<double>(shots);
+        uint64_t* d_out = c.take<uint64_t>(shots);
+        return sample_seq(s, r, shots, out, d_cum, d_ends, d_r, d_out);
+    }
+    // beyond ket's 2^24-entry range: the CDF from a chunked scan (its sums are
+    // associated differently from a sequential loop)
     const int64_t nch = (int64_t)(((1ull << s->n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG);
-    double *d_sum = nullptr, *d_r = nullptr;
-    uint64_t *d_out = nullptr, *d_last = nullptr;
-    HIPCHK(hipMalloc(&d_sum, 8 * nch));
+    int rc = ws_reserve(s, wsz(8 * nch) + wsz(8) + 2 * wsz(8 * shots));
+    if (rc) return rc;
+    Carve c{(char*)s->ws};
+    double* d_sum = c.take<double>(nch);
+    uint64_t* d_last = c.take<uint64_t>(1);
+    double* d_r = c.take<double>(shots);
+    uint64_t* d_out = c.take<uint64_t>(shots);
     HIPCHK(chunk_sums(s->n, s->prec, s->psi, thresh, d_sum, s->st));
     std::vector<double> cum(nch);
     HIPCHK(hipMemcpyAsync(cum.data(), d_sum, 8 * nch, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     int64_t lastc = -1;
     double acc = 0;
-    for (int64_t c = 0; c < nch; c++) {
-        if (cum[c] > 0) lastc = c;
-        acc += cum[c];
-        cum[c] = acc;
+    for (int64_t k = 0; k < nch; k++) {
+        if (cum[k] > 0) lastc = k;
+        acc += cum[k];
+        cum[k] = acc;
     }
     if (lastc < 0) {   // no probability above the filter: ket returns index 0
         for (size_t i = 0; i < shots; i++) out[i] = 0;
-        hipFree(d_sum);
         return KSV_OK;
     }
-    HIPCHK(hipMalloc(&d_last, 8));
     HIPCHK(hipMemsetAsync(d_last, 0, 8, s->st));
     HIPCHK(last_nonzero_chunk(s->n, s->prec, s->psi, thresh, lastc, d_last, s->st));
     uint64_t last_idx = 0;
     HIPCHK(hipMemcpyAsync(&last_idx, d_last, 8, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipMemcpyAsync(d_sum, cum.data(), 8 * nch, hipMemcpyHostToDevice, s->st));
-    HIPCHK(hipMalloc(&d_r, 8 * shots));
-    HIPCHK(hipMalloc(&d_out, 8 * shots));
     HIPCHK(hipMemcpyAsync(d_r, r, 8 * shots, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     HIPCHK(sample_search(s->n, s->prec, s->psi, thresh, d_sum, nch, d_r, shots, last_idx, d_out, s->st));
     HIPCHK(hipMemcpyAsync(out, d_out, 8 * shots, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
-    hipFree(d_sum);
-    hipFree(d_r);
-    hipFree(d_out);
-    hipFree(d_last);
     return KSV_OK;
 }
 
@@ -725,8 +780,11 @@ int ksv_expect_pauli(ksv_state* s, const uint64_t* xmask, const uint64_t* zmask,
     if (mode == 0) {
         uint64_t *d_x = nullptr, *d_z = nullptr;
         if (nterms) {
-            HIPCHK(hipMalloc(&d_x, 8 * nterms));
-            HIPCHK(hipMalloc(&d_z, 8 * nterms));
+            int rc = ws_reserve(s, 2 * wsz(8 * nterms));
+            if (rc) return rc;
+            Carve c{(char*)s->ws};
+            d_x = c.take<uint64_t>(nterms);
+            d_z = c.take<uint64_t>(nterms);
             HIPCHK(hipMemcpyAsync(d_x, xmask, 8 * nterms, hipMemcpyHostToDevice, s->st));
             HIPCHK(hipMemcpyAsync(d_z, zmask, 8 * nterms, hipMemcpyHostToDevice, s->st));
         }
@@ -750,8 +808,6 @@ int ksv_expect_pauli(ksv_state* s, const uint64_t* xmask, const uint64_t* zmask,
                 tot += out[t];
             }
         }
-        if (d_x) hipFree(d_x);
-        if (d_z) hipFree(d_z);
     } else {
         std::vector<ksv_op> ops;
         std::vector<double> mats;
@@ -1131,39 +1187,14 @@ int ksv_shots_sample(ksv_state* s, int n, const double* r, uint64_t* out) {
     int rc = shots_check(s, n, &copies);
     if (rc) return rc;
     if (!r || !out) return fail(KSV_ERR_TYPE, "null buffer");
-    const double thresh = 1e-14;
-    const int64_t nch = (int64_t)(((1ull << n) + (1ull << CHUNK_LOG) - 1) >> CHUNK_LOG);
-    DevBufs b;
-    double *d_sum = nullptr, *d_r = nullptr;
-    int64_t* d_lastc = nullptr;
-    unsigned long long* d_last = nullptr;
-    uint64_t* d_out = nullptr;
-    HIPCHK(b.get(&d_sum, (size_t)nch * copies));
-    HIPCHK(b.get(&d_r, copies));
-    HIPCHK(b.get(&d_lastc, copies));
-    HIPCHK(b.get(&d_last, copies));
-    HIPCHK(b.get(&d_out, copies));
-    int64_t nc = 0;
-    HIPCHK(shots_chunk_sums(n, s->prec, s->psi, copies, thresh, d_sum, &nc, s->st));
-    std::vector<double> cum((size_t)nch * copies);
-    HIPCHK(hipMemcpyAsync(cum.data(), d_sum, 8 * cum.size(), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
-    std::vector<int64_t> lastc(copies, -1);
-    for (uint32_t t = 0; t < copies; t++) {       // ksv_sample's host scan, copy by copy
-        double acc = 0;
-        double* c = cum.data() + (size_t)t * nch;
-        for (int64_t k = 0; k < nch; k++) {
-            if (c[k] > 0) lastc[t] = k;
-            acc += c[k];
-            c[k] = acc;
-        }
-    }
-    HIPCHK(hipMemcpyAsync(d_sum, cum.data(), 8 * cum.size(), hipMemcpyHostToDevice, s->st));
-    HIPCHK(hipMemcpyAsync(d_lastc, lastc.data(), 8 * copies, hipMemcpyHostToDevice, s->st));
+    rc = ws_reserve(s, 2 * wsz(8 * (size_t)copies));
+    if (rc) return rc;
+    Carve c{(char*)s->ws};
+    double* d_r = c.take<double>(copies);
+    uint64_t* d_out = c.take<uint64_t>(copies);
     HIPCHK(hipMemcpyAsync(d_r, r, 8 * copies, hipMemcpyHostToDevice, s->st));
-    HIPCHK(hipMemsetAsync(d_last, 0, 8 * copies, s->st));
-    HIPCHK(shots_lastnz(n, s->prec, s->psi, copies, thresh, d_lastc, d_last, s->st));
-    HIPCHK(shots_sample(n, s->prec, s->psi, copies, thresh, d_sum, nch, d_r, d_lastc, d_last, d_out, s->st));
+    // sampleSV takes every entry of the shot's map: the non-zero amplitudes
+    HIPCHK(shots_seq_sample(n, s->prec, s->psi, copies, 0.0, d_r, d_out, s->st));
     HIPCHK(hipMemcpyAsync(out, d_out, 8 * copies, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     return KSV_OK;

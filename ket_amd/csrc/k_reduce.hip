@@ -361,6 +361,80 @@ __global__ __launch_bounds__(256) void k_sample(const V* psi, uint64_t N, double
     if (threadIdx.x == 0) out[blockIdx.x] = best == ~0ull ? last_idx : best;
 }
 
+// ---------------------------------------------------------------------------
+// ket's CDF in ket's order (run()'s fast path, src/circuit.ts:3427-3436):
+// cum += p over the entries with p > thresh in ascending index, each addition
+// rounded on its own.  The additions are one dependent chain, so one wave
+// walks the state: 64 coalesced loads per step, then that step's 64
+// additions lane by lane on a value every lane holds alike.  cum[i] is the
+// JavaScript loop's running sum after index i (unchanged over zero entries),
+// so a draw's index is ket's whatever its distance to a CDF boundary.
+// ends[0], ends[1]: first and last index with p > thresh (~0 when none).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), k);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <typename V>
+__global__ __launch_bounds__(64) void k_seq_cdf(const V* psi, uint64_t N, double thresh, double* cum,
+                                                unsigned long long* ends) {
+    const int lane = threadIdx.x;
+    double run = 0.0;
+    unsigned long long first = ~0ull, last = ~0ull;
+    for (uint64_t b = 0; b < N; b += 64) {
+        const uint64_t i = b + lane;
+        const double p = pfilt(psi, N, i, thresh);
+        double mine = 0.0;
+#pragma unroll
+        for (int k = 0; k < 64; k++) {
+            run = rn_add(run, lane_value(p, k));
+            mine = lane == k ? run : mine;
+        }
+        const unsigned long long m = __ballot(p > 0.0);
+        if (m) {
+            if (first == ~0ull) first = b + (unsigned long long)(__ffsll((long long)m) - 1);
+            last = b + 63 - (unsigned long long)__clzll((long long)m);
+        }
+        if (i < N) cum[i] = mine;
+    }
+    if (lane == 0) {
+        ends[0] = first;
+        ends[1] = last;
+    }
+}
+
+// One thread per draw: ket's binary search over its CDF (first entry with
+// cumP >= r, the last entry's cumP set to 1.0), on the dense running sums of
+// k_seq_cdf.  The sum only grows at entries, so the first dense index whose
+// sum reaches r > 0 is an entry; the search stops below the last entry, which
+// takes every draw the sums leave (ket's 1.0).  Draws r <= 0 take the first
+// entry, a state without entries gives index 0 (ket's `?? 0n`).
+__global__ void k_cdf_search(const double* cum, const unsigned long long* ends, const double* rr, uint64_t shots,
+                             uint64_t* out) {
+    const uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (s >= shots) return;
+    const unsigned long long first = ends[0], last = ends[1];
+    if (first == ~0ull) {
+        out[s] = 0;
+        return;
+    }
+    const double r = rr[s];
+    if (!(r > 0.0)) {
+        out[s] = first;
+        return;
+    }
+    uint64_t lo = first, hi = last;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (cum[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    out[s] = lo;
+}
+
 // Predicate of the ordered compaction: mode 0 p > thresh, mode 1 key == kv,
 // mode 2 key > kv (key = bit pattern of p, monotone for p >= 0).
 struct Pred {
@@ -488,6 +562,20 @@ hipError_t chunk_compact_pred(int n, int prec, const void* psi, int mode, uint64
 hipError_t chunk_sums(int n, int prec, const void* psi, double thresh, double* d_sum, hipStream_t st) {
     DISPATCH(prec, hipLaunchKernelGGL((k_chunk_sums<V>), dim3((unsigned)nchunks(n)), dim3(256), 0, st,
                                       (const V*)psi, 1ull << n, thresh, d_sum));
+    return hipGetLastError();
+}
+
+hipError_t seq_cdf(int n, int prec, const void* psi, double thresh, double* d_cum, unsigned long long* d_ends,
+                   hipStream_t st) {
+    DISPATCH(prec, hipLaunchKernelGGL((k_seq_cdf<V>), dim3(1), dim3(64), 0, st, (const V*)psi, 1ull << n, thresh,
+                                      d_cum, d_ends));
+    return hipGetLastError();
+}
+
+hipError_t cdf_search(const double* d_cum, const unsigned long long* d_ends, const double* d_r, uint64_t shots,
+                      uint64_t* d_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_cdf_search, dim3((unsigned)((shots + 255) / 256)), dim3(256), 0, st, d_cum, d_ends, d_r,
+                       shots, d_out);
     return hipGetLastError();
 }
 

@@ -4,10 +4,10 @@
 // the kernels here do what differs per copy: the initial basis state, the
 // Pauli errors of ket's depolarising noise (src/circuit.ts:289-304),
 // measurement and reset (collapseQubit, src/circuit.ts:55-70) and the final
-// draw (the CDF search of src/circuit.ts:3425-3450).  Each per-copy
-// reduction repeats, copy by copy, the exact addition order of its
-// single-state counterpart in k_reduce.hip, so a copy's outcome and draw
-// equal those of the same shot simulated alone.
+// draw (sampleSV, src/circuit.ts:74-81).  Each per-copy reduction repeats,
+// copy by copy, the exact addition order of its single-state counterpart in
+// k_reduce.hip, and the final draw is sampleSV's sequential sum itself, so a
+// copy's outcome and draw equal those of the same shot simulated alone.
 #include "kcommon.h"
 
 namespace ksv {
@@ -139,107 +139,47 @@ __global__ void k_shots_project(V* psi, int n, int q, const uint8_t* outcome, co
     }
 }
 
-// ---- per-copy chunked CDF (the arithmetic of k_chunk_sums / k_sample) ----
-constexpr int SCH = 1 << CHUNK_LOG;
-constexpr int SQW = SCH / 4;
-constexpr int SSEGS = SQW / 64;
-
 template <typename V>
 __device__ __forceinline__ double spfilt(const V* c, uint64_t N, uint64_t i, double thresh) {
     if (i >= N) return 0.0;
     double p = sprob(c[i]);
     return p > thresh ? p : 0.0;
 }
-__device__ __forceinline__ double s_wave_scan(double v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        double t = __shfl_up(v, o, 64);
-        if (lane >= o) v = __dadd_rn(v, t);
-    }
-    return v;
-}
-template <typename V>
-__device__ void s_chunk_totals(const V* c, uint64_t N, uint64_t c0, double thresh, double* wt, double* woff) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double run = 0;
-    for (int sg = 0; sg < SSEGS; sg++) {
-        double v = s_wave_scan(spfilt(c, N, c0 + (uint64_t)w * SQW + sg * 64 + lane, thresh));
-        run = __dadd_rn(run, __shfl(v, 63, 64));
-    }
-    if (lane == 0) wt[w] = run;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double o = 0;
-        for (int i = 0; i < 4; i++) { woff[i] = o; o = __dadd_rn(o, wt[i]); }
-    }
-    __syncthreads();
+
+// One wave per copy: ket's sampleSV (src/circuit.ts:74-81) on the copy, i.e.
+// cum += p over its entries (p > thresh) in ascending index, one rounding per
+// addition, and the first entry whose running sum reaches r[t]; the last entry
+// when none does, index 0 for a copy without entries.  64 coalesced loads per
+// step, the step's additions lane by lane on a value every lane holds alike;
+// the walk stops at the step that holds the draw.
+__device__ __forceinline__ double s_lane_value(double v, int k) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), k);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// block (chunk, copy)
 template <typename V>
-__global__ __launch_bounds__(256) void k_shots_chunk_sums(const V* psi, int n, double thresh, double* sums) {
-    __shared__ double wt[4], woff[4];
+__global__ __launch_bounds__(64) void k_shots_seq_sample(const V* psi, int n, double thresh, const double* rr,
+                                                         uint64_t* out) {
     const uint64_t N = 1ull << n;
-    const V* c = psi + ((uint64_t)blockIdx.y << n);
-    s_chunk_totals(c, N, (uint64_t)blockIdx.x * SCH, thresh, wt, woff);
-    if (threadIdx.x == 0) sums[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = __dadd_rn(woff[3], wt[3]);
-}
-
-// block t: last index of copy t with p > thresh inside chunk lastc[t] (lastc < 0: none)
-template <typename V>
-__global__ __launch_bounds__(256) void k_shots_lastnz(const V* psi, int n, double thresh, const int64_t* lastc,
-                                                      unsigned long long* out) {
-    const uint64_t N = 1ull << n;
-    const int64_t lc = lastc[blockIdx.x];
-    if (lc < 0) return;
     const V* c = psi + ((uint64_t)blockIdx.x << n);
-    const uint64_t c0 = (uint64_t)lc * SCH;
-    for (uint64_t i = c0 + threadIdx.x; i < c0 + SCH && i < N; i += blockDim.x)
-        if (sprob(c[i]) > thresh) atomicMax(&out[blockIdx.x], (unsigned long long)i);
-}
-
-// block t: copy t's draw r[t] against its inclusive chunk CDF ccum[t * nch ...]
-template <typename V>
-__global__ __launch_bounds__(256) void k_shots_sample(const V* psi, int n, double thresh, const double* ccum,
-                                                      int64_t nch, const double* rr, const int64_t* lastc,
-                                                      const unsigned long long* last_idx, uint64_t* out) {
-    __shared__ double wt[4], woff[4];
-    __shared__ unsigned long long best;
-    const uint64_t N = 1ull << n;
-    const int t = blockIdx.x;
-    if (lastc[t] < 0) {                        // nothing above the filter: ket returns index 0
-        if (threadIdx.x == 0) out[t] = 0;
-        return;
+    const double r = rr[blockIdx.x];
+    const int lane = threadIdx.x;
+    double run = 0.0;
+    unsigned long long last = ~0ull, hit = ~0ull;
+    for (uint64_t b = 0; b < N && hit == ~0ull; b += 64) {
+        const double p = spfilt(c, N, b + lane, thresh);
+        const unsigned long long m = __ballot(p > 0.0);
+        if (!m) continue;
+        last = b + 63 - (unsigned long long)__clzll((long long)m);
+#pragma unroll
+        for (int k = 0; k < 64; k++) {
+            run = rn_add(run, s_lane_value(p, k));
+            if (hit == ~0ull && ((m >> k) & 1) && r <= run) hit = b + k;
+        }
     }
-    const V* c = psi + ((uint64_t)t << n);
-    const double* cc = ccum + (size_t)t * nch;
-    const double r = rr[t];
-    int64_t lo = 0, hi = nch;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (cc[mid] < r) lo = mid + 1;
-        else hi = mid;
-    }
-    if (lo >= nch) {
-        if (threadIdx.x == 0) out[t] = last_idx[t];
-        return;
-    }
-    const double cbase = lo > 0 ? cc[lo - 1] : 0.0;
-    const uint64_t c0 = (uint64_t)lo * SCH;
-    if (threadIdx.x == 0) best = ~0ull;
-    s_chunk_totals(c, N, c0, thresh, wt, woff);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double run = 0;
-    for (int sg = 0; sg < SSEGS; sg++) {
-        uint64_t i = c0 + (uint64_t)w * SQW + sg * 64 + lane;
-        double v = s_wave_scan(spfilt(c, N, i, thresh));
-        double cum = __dadd_rn(cbase, __dadd_rn(woff[w], __dadd_rn(run, v)));
-        if (cum >= r && i < N) atomicMin(&best, (unsigned long long)i);
-        run = __dadd_rn(run, __shfl(v, 63, 64));
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) out[t] = best == ~0ull ? last_idx[t] : best;
+    if (lane == 0) out[blockIdx.x] = hit != ~0ull ? hit : last != ~0ull ? last : 0;
 }
 
 // Sum the per-block partials of each copy in block order (k_final's order).
@@ -296,27 +236,10 @@ hipError_t shots_project(int ntot, int n, int prec, void* psi, int q, const uint
     return hipGetLastError();
 }
 
-hipError_t shots_chunk_sums(int n, int prec, const void* psi, uint32_t copies, double thresh, double* d_sums,
-                            int64_t* nch, hipStream_t st) {
-    const int64_t c = (int64_t)(((1ull << n) + SCH - 1) >> CHUNK_LOG);
-    *nch = c;
-    SDISPATCH(prec, hipLaunchKernelGGL((k_shots_chunk_sums<V>), dim3((unsigned)c, copies), dim3(256), 0, st,
-                                       (const V*)psi, n, thresh, d_sums));
-    return hipGetLastError();
-}
-
-hipError_t shots_lastnz(int n, int prec, const void* psi, uint32_t copies, double thresh, const int64_t* d_lastc,
-                        unsigned long long* d_last, hipStream_t st) {
-    SDISPATCH(prec, hipLaunchKernelGGL((k_shots_lastnz<V>), dim3(copies), dim3(256), 0, st, (const V*)psi, n, thresh,
-                                       d_lastc, d_last));
-    return hipGetLastError();
-}
-
-hipError_t shots_sample(int n, int prec, const void* psi, uint32_t copies, double thresh, const double* d_ccum,
-                        int64_t nch, const double* d_r, const int64_t* d_lastc, const unsigned long long* d_last,
-                        uint64_t* d_out, hipStream_t st) {
-    SDISPATCH(prec, hipLaunchKernelGGL((k_shots_sample<V>), dim3(copies), dim3(256), 0, st, (const V*)psi, n, thresh,
-                                       d_ccum, nch, d_r, d_lastc, d_last, d_out));
+hipError_t shots_seq_sample(int n, int prec, const void* psi, uint32_t copies, double thresh, const double* d_r,
+                            uint64_t* d_out, hipStream_t st) {
+    SDISPATCH(prec, hipLaunchKernelGGL((k_shots_seq_sample<V>), dim3(copies), dim3(64), 0, st, (const V*)psi, n,
+                                       thresh, d_r, d_out));
     return hipGetLastError();
 }
 

@@ -379,13 +379,9 @@ hipError_t shots_bit1(int n, int prec, const void* psi, int q, uint32_t copies, 
 hipError_t shots_final(const double* d_part, int nblk, uint32_t copies, double* d_out, hipStream_t st);
 hipError_t shots_project(int ntot, int n, int prec, void* psi, int q, const uint8_t* d_out, const double* d_scale,
                          const uint8_t* d_reset, hipStream_t st);
-hipError_t shots_chunk_sums(int n, int prec, const void* psi, uint32_t copies, double thresh, double* d_sums,
-                            int64_t* nch, hipStream_t st);
-hipError_t shots_lastnz(int n, int prec, const void* psi, uint32_t copies, double thresh, const int64_t* d_lastc,
-                        unsigned long long* d_last, hipStream_t st);
-hipError_t shots_sample(int n, int prec, const void* psi, uint32_t copies, double thresh, const double* d_ccum,
-                        int64_t nch, const double* d_r, const int64_t* d_lastc, const unsigned long long* d_last,
-                        uint64_t* d_out, hipStream_t st);
+
+hipError_t shots_seq_sample(int n, int prec, const void* psi, uint32_t copies, double thresh, const double* d_r,
+                            uint64_t* d_out, hipStream_t st);
 
 // probability compaction / sampling / top-k
 constexpr int CHUNK_LOG = 12;     // amplitudes per block chunk in scans
@@ -398,6 +394,12 @@ hipError_t chunk_sums(int n, int prec, const void* psi, double thresh, double* d
 hipError_t sample_search(int n, int prec, const void* psi, double thresh, const double* d_chunk_cum,
                          int64_t nchunks, const double* d_r, uint64_t shots, uint64_t last_idx,
                          uint64_t* d_out, hipStream_t st);
+// ket's sequential CDF (exact running sums, first/last entry) and the draw search on it
+constexpr int SEQ_CDF_MAX_QUBITS = 24;   // one wave walks 2^n amplitudes; ket's own path stops at 2^24 entries
+hipError_t seq_cdf(int n, int prec, const void* psi, double thresh, double* d_cum, unsigned long long* d_ends,
+                   hipStream_t st);
+hipError_t cdf_search(const double* d_cum, const unsigned long long* d_ends, const double* d_r, uint64_t shots,
+                      uint64_t* d_out, hipStream_t st);
 hipError_t key_histogram(int n, int prec, const void* psi, uint64_t prefix, int shift_hi,
                          int shift, uint32_t* d_hist, hipStream_t st);
 hipError_t last_nonzero_chunk(int n, int prec, const void* psi, double thresh, int64_t chunk,

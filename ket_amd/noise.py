@@ -142,7 +142,7 @@ def run_per_shot(circ, shots, rng, noise, init, creg_counts):
         st.init_basis(init)
         sh = _Shot(st, rng, noise)
         sh.run_ops(ops, shot_cregs)
-        idx = int(st.sample(np.array([rng()]))[0])
+        idx = int(st.shots_sample(n, np.array([rng()]))[0])     # sampleSV on the one state
         if pm:
             for q in range(circ.qubits):
                 if rng() < pm:

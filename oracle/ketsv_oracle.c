@@ -461,6 +461,22 @@ int orc_sample(int n, const double* psi, const double* r, size_t shots, uint64_t
     return 0;
 }
 
+/* sampleSV, src/circuit.ts:74-81 (the per-shot path's final draw): cum +=
+ * |a|^2 over the map's entries (the non-zero amplitudes) in ascending index;
+ * the first entry with rand <= cum, else the last entry, else index 0. */
+uint64_t orc_sample_sv(int n, const double* psi, double rand) {
+    double cum = 0;
+    uint64_t last = 0;
+    for (uint64_t i = 0; i < (1ull << n); i++) {
+        double p = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        if (!(p > 0)) continue;
+        cum += p;
+        last = i;
+        if (rand <= cum) return i;
+    }
+    return last;
+}
+
 /* marginals(), src/circuit.ts:1508-1518 */
 void orc_marginals(int n, const double* psi, double* out) {
     for (int q = 0; q < n; q++) out[q] = 0;

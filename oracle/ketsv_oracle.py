@@ -45,6 +45,8 @@ def lib():
         L.orc_prng.argtypes = [c_uint32, c_size_t, c_void_p]
         L.orc_sample.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p]
         L.orc_sample.restype = c_int
+        L.orc_sample_sv.argtypes = [c_int, c_void_p, c_double]
+        L.orc_sample_sv.restype = c_uint64
         L.orc_marginals.argtypes = [c_int, c_void_p, c_void_p]
         L.orc_bloch.argtypes = [c_int, c_void_p, c_int, POINTER(c_double), POINTER(c_double)]
         L.orc_expectation.argtypes = [c_int, c_void_p, c_char_p]
@@ -156,6 +158,11 @@ def sample(n, psi, r):
     return out
 
 
+def sample_sv(n, psi, r):
+    """sampleSV (src/circuit.ts:74-81): the per-shot path's final draw."""
+    return int(lib().orc_sample_sv(n, _p(psi), float(r)))
+
+
 def marginals(n, psi):
     out = np.empty(n, dtype=np.float64)
     lib().orc_marginals(n, _p(psi), _p(out))
@@ -250,7 +257,7 @@ def run_shots(n, calls, shots, seed, noise=None, init=0, cregs=None, prune=True)
                         apply_named(n, psi, [(pa, (qs[0],))], prune)
                     if pb:
                         apply_named(n, psi, [(pb, (qs[1],))], prune)
-        idx = int(sample(n, psi, np.array([next(rng)]))[0])
+        idx = sample_sv(n, psi, next(rng))
         if pm:
             for q in range(n):
                 if next(rng) < pm:

@@ -262,6 +262,36 @@ def test_sampling_matches_oracle(n):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("n", [12, 16])
+def test_sampling_boundary_draws_bit_exact(n):
+    """Draws equal to ket's running sums (cum += p in ascending index, one rounding
+    per addition) and to their neighbouring doubles: run()'s draw (ksv_sample) gives
+    the oracle's index for every one, and the per-shot draw (ksv_shots_sample, one
+    copy and 64 copies) gives sampleSV's."""
+    import itertools
+    calls = workloads.random_layer_calls(n, 4, seed=5 + n)
+    ref = dense_ref(n, calls, prune=True)
+    st = build(n, calls).simulate()
+    assert np.array_equal(st.amplitudes(), ref)
+    p = ref.real * ref.real + ref.imag * ref.imag
+    keep = np.nonzero(p > 1e-14)[0]
+    cum = np.array(list(itertools.accumulate(p[keep].tolist())))
+    sel = np.random.default_rng(n).choice(len(cum) - 1, size=min(2000, len(cum) - 1), replace=False)
+    rs = np.concatenate([cum[sel], np.nextafter(cum[sel], 0), np.nextafter(cum[sel], 2)])
+    assert np.array_equal(st.sample(rs), O.sample(n, ref, rs))
+    for r in rs[::97]:
+        assert int(st.shots_sample(n, np.array([r]))[0]) == O.sample_sv(n, ref, r)
+    st.free()
+    b = 6
+    cs = State(n + b, "f64", prune=True)
+    cs.shots_init(n, 0)
+    cs.apply(build(n, calls).tape())
+    draws = rs[:1 << b]
+    got = cs.shots_sample(n, draws)
+    cs.free()
+    assert got.tolist() == [O.sample_sv(n, ref, r) for r in draws.tolist()]
+
+
 def test_sampling_sparse_state_last_entry():
     """r above the last CDF value lands on the last non-zero index (cdf[-1] := 1)."""
     c = Circuit(14).h(3).cnot(3, 13)

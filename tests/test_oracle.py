@@ -446,3 +446,54 @@ def test_layer0_restatements_used_at_30_qubits(n, heavy):
     assert np.array_equal(got, pruned)
     if heavy:
         assert np.count_nonzero(pruned == 0) > (1 << n) // 2
+
+
+def _ket_cdf(n, psi, thresh):
+    """ket's running sums, written as its loops are: p = re*re + im*im per entry,
+    cum += p in ascending index (Python floats round every operation)."""
+    idx, cum, acc = [], [], 0.0
+    for i in range(1 << n):
+        re, im = float(psi[i].real), float(psi[i].imag)
+        p = re * re + im * im
+        if p > thresh:
+            acc += p
+            idx.append(i)
+            cum.append(acc)
+    return idx, cum
+
+
+def test_sample_boundary_draws_follow_the_sequential_cdf():
+    """Draws equal to a running sum and its neighbouring doubles: the oracle's
+    fast-path search (src/circuit.ts:3425-3450) and sampleSV (:74-81) against
+    linear scans over ket's sequential sums."""
+    n = 7
+    psi = O.zero_state(n)
+    O.apply_named(n, psi, random_calls(n, 60, seed=8), prune=True)
+    idx, cum = _ket_cdf(n, psi, 1e-14)
+    rs = []
+    for c in cum[:-1]:
+        rs += [c, float(np.nextafter(c, 0)), float(np.nextafter(c, 2))]
+    rs = np.array(rs)
+    got = O.sample(n, psi, rs)
+    cdf = cum[:-1] + [1.0]
+    for r, g in zip(rs.tolist(), got.tolist()):
+        want = next(i for i, c in zip(idx, cdf) if c >= r)
+        assert g == want
+    idx0, cum0 = _ket_cdf(n, psi, 0.0)
+    for r in rs.tolist():
+        want = next((i for i, c in zip(idx0, cum0) if r <= c), idx0[-1])
+        assert O.sample_sv(n, psi, r) == want
+
+
+def test_sample_sv_takes_every_entry():
+    """sampleSV walks every map entry, the fast path only p > 1e-14; beyond the
+    total both give the last entry; an empty state gives index 0."""
+    psi = np.zeros(8, dtype=complex)
+    psi[0] = 1e-8                      # p = 1e-16: a map entry, filtered by the fast path
+    psi[3] = math.sqrt(0.5)
+    psi[5] = math.sqrt(0.5)
+    assert O.sample_sv(3, psi, 1e-17) == 0
+    assert O.sample(3, psi, np.array([1e-17]))[0] == 3
+    assert O.sample_sv(3, psi, 0.7) == 5
+    assert O.sample_sv(3, psi, 1.5) == 5
+    assert O.sample_sv(3, np.zeros(8, dtype=complex), 0.3) == 0
