@@ -201,6 +201,14 @@ int ksv_reduce_1q(ksv_state* s, int q, double out[4]);
  * index however close it is to a CDF boundary; above 24 qubits the CDF comes
  * from a chunked scan whose sums are associated differently. */
 int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out);
+/* ksv_sample with no device allocation: no counterpart in ket; for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Its scratch is one fixed device array per GPU (9 MiB,
+ * held until the process ends), so it takes states of at most 20 qubits and at
+ * most 65536 draws per call (KSV_ERR_RANGE beyond).  Same results as ksv_sample.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (any states, any threads) race. */
+int ksv_sample_noalloc(ksv_state* s, const double* r, size_t shots, uint64_t* out);
 
 /* Pauli expectations.  Term t is coeff[t] * prod_q P_q with
  * P_q = X if xmask bit q only, Z if zmask bit q only, Y if both.

@@ -72,6 +72,7 @@ SIGNATURES = [
     ("ksv_marginals", c_int, [_P, _P]),
     ("ksv_reduce_1q", c_int, [_P, c_int, _P]),
     ("ksv_sample", c_int, [_P, _P, c_size_t, _P]),
+    ("ksv_sample_noalloc", c_int, [_P, _P, c_size_t, _P]),
     ("ksv_expect_pauli", c_int, [_P, _P, _P, _P, c_size_t, c_int, _P, _DP]),
     ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
     ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
@@ -278,6 +279,14 @@ class State:
         r = np.ascontiguousarray(r, dtype=np.float64)
         out = np.empty(len(r), dtype=np.uint64)
         check(lib().ksv_sample(self.handle, ptr(r), len(r), ptr(out)))
+        return out
+
+    def sample_noalloc(self, r):
+        """ksv_sample without any device allocation (fixed scratch: <= 20 qubits,
+        <= 65536 draws per call; calls that overlap on one GPU race)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.empty(len(r), dtype=np.uint64)
+        check(lib().ksv_sample_noalloc(self.handle, ptr(r), len(r), ptr(out)))
         return out
 
     def expect_pauli(self, xmask, zmask, coeff, mode=0):

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -685,6 +686,45 @@ static int sample_seq(ksv_state* s, const double* r, size_t shots, uint64_t* out
     return KSV_OK;
 }
 
+// ---- allocation-free sampling ---------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops, harnesses that forbid hipMalloc):
+// ksv_sample's exact path with its scratch in one fixed device array per GPU,
+// held until the process ends: 2^20 running sums (8 MiB), the two end
+// indices, and 65536 draws and indices (1 MiB).  Host code reaches the array
+// only through its device address, looked up once per device.
+constexpr int NOALLOC_MAX_QUBITS = 20;
+constexpr size_t NOALLOC_MAX_SHOTS = 65536;
+constexpr size_t NOALLOC_CUM = (size_t)8 << NOALLOC_MAX_QUBITS;
+constexpr size_t NOALLOC_ENDS = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_CUM + NOALLOC_ENDS + 2 * 8 * NOALLOC_MAX_SHOTS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on device `dev` (looked up with that device
+// current, once per device; each GPU holds its own copy at its own address).
+static int scratch_on(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(KSV_ERR_DEVICE, "device index beyond the scratch table");
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != dev) HIPCHK(hipSetDevice(dev));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return fail(KSV_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return KSV_OK;
+}
+
 int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out) {
     if (!s) return fail(KSV_ERR_HANDLE, "null state");
     if (!shots) return KSV_OK;
@@ -737,6 +777,27 @@ int ksv_sample(ksv_state* s, const double* r, size_t shots, uint64_t* out) {
     HIPCHK(hipMemcpyAsync(out, d_out, 8 * shots, hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     return KSV_OK;
+}
+
+int ksv_sample_noalloc(ksv_state* s, const double* r, size_t shots, uint64_t* out) {
+    if (!s) return fail(KSV_ERR_HANDLE, "null state");
+    if (!shots) return KSV_OK;
+    if (!r || !out) return fail(KSV_ERR_TYPE, "null buffer");
+    if (s->n > NOALLOC_MAX_QUBITS)
+        return fail(KSV_ERR_RANGE, "ksv_sample_noalloc: " + std::to_string(s->n) + " qubits exceeds its scratch (at most " +
+                                       std::to_string(NOALLOC_MAX_QUBITS) + ")");
+    if (shots > NOALLOC_MAX_SHOTS)
+        return fail(KSV_ERR_RANGE, "ksv_sample_noalloc: " + std::to_string(shots) + " draws exceeds its scratch (at most " +
+                                       std::to_string(NOALLOC_MAX_SHOTS) + " per call)");
+    unsigned char* base = nullptr;
+    int rc = scratch_on(s->dev, &base);
+    if (rc) return rc;
+    double* d_cum = reinterpret_cast<double*>(base);
+    unsigned long long* d_ends = reinterpret_cast<unsigned long long*>(base + NOALLOC_CUM);
+    double* d_r = reinterpret_cast<double*>(base + NOALLOC_CUM + NOALLOC_ENDS);
+    uint64_t* d_out = reinterpret_cast<uint64_t*>(base + NOALLOC_CUM + NOALLOC_ENDS + 8 * NOALLOC_MAX_SHOTS);
+    // every running sum and both end indices are rewritten by this call: nothing to clear
+    return sample_seq(s, r, shots, out, d_cum, d_ends, d_r, d_out);
 }
 
 static int ensure_psi2(ksv_state* s) {
