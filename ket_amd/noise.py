@@ -51,12 +51,13 @@ def resolve_noise(noise):
 
 
 def _creg_value(shot_cregs, name):
-    bits = shot_cregs.get(name, [])
+    """cregValue (src/circuit.ts:164-166): acc | (1 << i) over the set bits, with
+    JavaScript's 32-bit << (count mod 32) and | (a signed 32-bit result)."""
     v = 0
-    for i, b in enumerate(bits):
+    for i, b in enumerate(shot_cregs.get(name, [])):
         if b:
-            v |= 1 << i
-    return v
+            v |= 1 << (i & 31)
+    return v - (1 << 32) if v & (1 << 31) else v
 
 
 class _Shot:

@@ -197,7 +197,8 @@ def collapse(n, psi, q, r):
 # run()'s per-shot path, restated (src/circuit.ts:164-207 applyOps,
 # :289-304 dep1/dep2, :55-81 collapseQubit/sampleSV, :3455-3481 run's
 # per-shot loop, :813-819 makePrng).  Calls are ket Circuit-method tuples
-# plus ("measure", (q, creg, bit)) and ("reset", (q,)).
+# plus ("measure", (q, creg, bit)), ("reset", (q,)) and
+# ("if", (creg, value, inner_calls)).
 # ---------------------------------------------------------------------------
 _SINGLE = {m for m, k in NQ.items() if k == 1}
 _TWO_KIND = {"cnot", "cx", "swap", "xx", "yy", "zz", "xy", "iswap", "srswap", "ms", "cy", "cz", "ch", "crx",
@@ -217,8 +218,60 @@ def xorshift(seed):
         yield s / 4294967296.0
 
 
+def creg_value(bits):
+    """cregValue (src/circuit.ts:164-166): acc | (1 << i) over the set bits, with
+    JavaScript's 32-bit << (count mod 32) and | (a signed 32-bit result)."""
+    v = 0
+    for i, b in enumerate(bits):
+        if b:
+            v |= 1 << (i & 31)
+    return v - (1 << 32) if v & (1 << 31) else v
+
+
+def _apply_ops(n, psi, calls, bits, rng, p1, p2, pm, prune):
+    """applyOps (src/circuit.ts:169-207) on one shot's state, recursive for `if`:
+    ("if", (creg, value, inner_calls)) runs inner_calls when cregValue == value."""
+    for name, args in calls:
+        if name == "measure":
+            q, reg, b = args
+            o = collapse(n, psi, q, next(rng))
+            if pm and next(rng) < pm:
+                o = 1 - o
+            if reg in bits:
+                bits[reg][b] = o == 1
+            continue
+        if name == "reset":
+            (q,) = args
+            if collapse(n, psi, q, next(rng)) == 1:
+                apply_named(n, psi, [("x", (q,))], prune)
+            continue
+        if name == "if":
+            reg, value, inner = args
+            if creg_value(bits.get(reg, [])) == value:
+                _apply_ops(n, psi, inner, bits, rng, p1, p2, pm, prune)
+            continue
+        if name == "barrier":
+            continue
+        apply_named(n, psi, [(name, args)], prune)
+        qs = args[len(args) - NQ[name]:] if name in NQ else ()
+        if name in _SINGLE and p1:                      # dep1
+            rand = next(rng)
+            if rand < p1:
+                r = rand / p1
+                apply_named(n, psi, [("x" if r < 1 / 3 else "y" if r < 2 / 3 else "z", (qs[0],))], prune)
+        elif name in _TWO_KIND and p2:                  # dep2
+            rand = next(rng)
+            if rand < p2:
+                pa, pb = _PAIRS[min(int(rand / p2 * 15), 14)]
+                if pa:
+                    apply_named(n, psi, [(pa, (qs[0],))], prune)
+                if pb:
+                    apply_named(n, psi, [(pb, (qs[1],))], prune)
+
+
 def run_shots(n, calls, shots, seed, noise=None, init=0, cregs=None, prune=True):
-    """Per-shot simulation; returns (counts {index: n} in first-seen order, creg bit counts)."""
+    """Per-shot simulation (run()'s per-shot loop, src/circuit.ts:3455-3481);
+    returns (counts {index: n} in first-seen order, creg bit counts)."""
     rng = xorshift(seed)
     p1 = (noise or {}).get("p1", 0) or 0
     p2 = (noise or {}).get("p2", 0) or 0
@@ -228,35 +281,7 @@ def run_shots(n, calls, shots, seed, noise=None, init=0, cregs=None, prune=True)
     for _ in range(shots):
         psi = zero_state(n, init)
         bits = {k: [False] * v for k, v in cregs.items()}
-        for name, args in calls:
-            if name == "measure":
-                q, reg, b = args
-                o = collapse(n, psi, q, next(rng))
-                if pm and next(rng) < pm:
-                    o = 1 - o
-                if reg in bits:
-                    bits[reg][b] = o == 1
-                continue
-            if name == "reset":
-                (q,) = args
-                if collapse(n, psi, q, next(rng)) == 1:
-                    apply_named(n, psi, [("x", (q,))], prune)
-                continue
-            apply_named(n, psi, [(name, args)], prune)
-            qs = args[len(args) - NQ[name]:] if name in NQ else ()
-            if name in _SINGLE and p1:                      # dep1
-                rand = next(rng)
-                if rand < p1:
-                    r = rand / p1
-                    apply_named(n, psi, [("x" if r < 1 / 3 else "y" if r < 2 / 3 else "z", (qs[0],))], prune)
-            elif name in _TWO_KIND and p2:                  # dep2
-                rand = next(rng)
-                if rand < p2:
-                    pa, pb = _PAIRS[min(int(rand / p2 * 15), 14)]
-                    if pa:
-                        apply_named(n, psi, [(pa, (qs[0],))], prune)
-                    if pb:
-                        apply_named(n, psi, [(pb, (qs[1],))], prune)
+        _apply_ops(n, psi, calls, bits, rng, p1, p2, pm, prune)
         idx = sample_sv(n, psi, next(rng))
         if pm:
             for q in range(n):

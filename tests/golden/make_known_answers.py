@@ -77,6 +77,52 @@ for i, (n, calls, H, params, want) in enumerate(grads):
     cases.append({"id": f"gradient_{i}", "source": "src/circuit.test.ts:4165-4228", "n": n, "calls": calls,
                   "query": "gradient", "hamiltonian": H, "params": params, "expected": want, "tol": 1e-10})
 
+# src/circuit.test.ts:1864-1978 -- run() with mid-circuit measure, reset and if
+# (query "run": counts from run({shots, seed}); "probs" are exact outcome
+# frequencies, "bit" = [q0-leftmost position, value, frequency, tolerance] and
+# "cregs" the register bit frequencies ket asserts)
+def meas(q, reg, bit):
+    return ["measure", [q, reg, bit]]
+
+
+def cond(reg, value, calls):
+    return ["if", [reg, value, calls]]
+
+
+two_bits = [["x", [1]], meas(0, "c", 0), meas(1, "c", 1)]
+teleport = [["h", [1]], ["cnot", [1, 2]], ["cnot", [0, 1]], ["h", [0]], meas(0, "c0", 0), meas(1, "c1", 0),
+            cond("c1", 1, [["x", [2]]]), cond("c0", 1, [["z", [2]]])]
+runs = [
+    ("if_register_zero_fires", "src/circuit.test.ts:1865-1869", 1, {"c": 1},
+     [meas(0, "c", 0), cond("c", 0, [["x", [0]]])], 100, 1, {"probs": {"1": 1.0}}),
+    ("if_fires_on_one", "src/circuit.test.ts:1871-1875", 1, {"c": 1},
+     [["x", [0]], meas(0, "c", 0), cond("c", 1, [["x", [0]]])], 100, 1, {"probs": {"0": 1.0}}),
+    ("if_skips_on_one", "src/circuit.test.ts:1877-1881", 1, {"c": 1},
+     [["x", [0]], meas(0, "c", 0), cond("c", 0, [["x", [0]]])], 100, 1, {"probs": {"1": 1.0}}),
+    ("if_corrects_collapse", "src/circuit.test.ts:1883-1887", 1, {"c": 1},
+     [["h", [0]], meas(0, "c", 0), cond("c", 1, [["x", [0]]])], 1000, 42, {"probs": {"0": 1.0}}),
+    ("if_multibit_value_2", "src/circuit.test.ts:1889-1899", 2, {"c": 2},
+     two_bits + [cond("c", 2, [["x", [0]]])], 100, 1, {"probs": {"11": 1.0}}),
+    ("if_multibit_not_1", "src/circuit.test.ts:1901-1911", 2, {"c": 2},
+     two_bits + [cond("c", 1, [["x", [0]]])], 100, 1, {"probs": {"01": 1.0}}),
+    ("if_multibit_value_3", "src/circuit.test.ts:1913-1923", 3, {"c": 2},
+     [["x", [0]], ["x", [1]], meas(0, "c", 0), meas(1, "c", 1), cond("c", 3, [["x", [2]]])], 100, 1,
+     {"probs": {"111": 1.0}}),
+    ("teleport_0", "src/circuit.test.ts:1934-1948", 3, {"c0": 1, "c1": 1}, teleport, 1000, 1,
+     {"bit": [2, "0", 1.0, 1e-5]}),
+    ("teleport_1", "src/circuit.test.ts:1934-1956", 3, {"c0": 1, "c1": 1}, [["x", [0]]] + teleport, 1000, 1,
+     {"bit": [2, "1", 1.0, 1e-5]}),
+    ("teleport_plus", "src/circuit.test.ts:1934-1963", 3, {"c0": 1, "c1": 1}, [["h", [0]]] + teleport, 1000, 1,
+     {"bit": [2, "0", 0.5, 0.05]}),
+    ("measure_then_reset", "src/circuit.test.ts:1967-1971", 1, {"c": 1},
+     [["x", [0]], meas(0, "c", 0), ["reset", [0]]], 100, 1, {"probs": {"0": 1.0}, "cregs": {"c": [1.0]}}),
+    ("reset_then_reencode", "src/circuit.test.ts:1974-1977", 1, {"c": 1},
+     [["x", [0]], meas(0, "c", 0), ["reset", [0]], ["x", [0]]], 100, 1, {"probs": {"1": 1.0}}),
+]
+for cid, src, n, cregs, calls, shots, seed, want in runs:
+    cases.append({"id": cid, "source": src, "n": n, "cregs": cregs, "calls": calls, "query": "run",
+                  "shots": shots, "seed": seed, "expected": want, "tol": 1e-10})
+
 if __name__ == "__main__":
     with open(os.path.join(HERE, "known_answers.json"), "w") as f:
         json.dump({"about": "known answers of ket's statevector tests, restated as data", "cases": cases}, f,

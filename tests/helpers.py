@@ -54,16 +54,25 @@ def random_unitary(dim, rnd):
     return [[complex(v) for v in row] for row in q]
 
 
-def build(n, calls, cregs=None):
-    c = ket_amd.Circuit(n)
-    for name, size in (cregs or {}).items():
-        c = c.creg(name, size)
+def _add_calls(c, calls):
     for name, args in calls:
         if name == "unitary":
             c = c.unitary(args[0], *args[1:])
+        elif name == "if":
+            reg, value, inner = args
+            c = c.if_(reg, value, lambda k, inner=inner: _add_calls(k, inner))
         else:
             c = getattr(c, name)(*args)
     return c
+
+
+def build(n, calls, cregs=None):
+    """ket's Circuit from (method, args) calls; ("if", (creg, value, inner_calls))
+    becomes Circuit.if(creg, value, builder)."""
+    c = ket_amd.Circuit(n)
+    for name, size in (cregs or {}).items():
+        c = c.creg(name, size)
+    return _add_calls(c, calls)
 
 
 def dense_ref(n, calls, prune=True, init=0):
@@ -138,3 +147,39 @@ def pauli_apply(n, psi, terms):
                 v = np.moveaxis(np.tensordot(mats[ch], v, axes=([1], [a])), 0, a)
         out += t["coeff"] * v.reshape(-1)
     return out
+
+
+def if_calls(n, count, seed, noise_kinds=True):
+    """A seeded random circuit with mid-circuit measurements into two multi-bit
+    registers, resets, and `if` blocks (some nested) whose bodies hold gates,
+    measurements and resets: the per-shot path's draw count then depends on the
+    measured values."""
+    rnd = random.Random(seed)
+    regs = {"a": 3, "b": 2}
+
+    def gates(k):
+        return random_calls(n, k, seed=rnd.randrange(1 << 30), with_unitary=False)
+
+    def measure():
+        reg = rnd.choice(sorted(regs))
+        return ("measure", (rnd.randrange(n), reg, rnd.randrange(regs[reg])))
+
+    calls = []
+    for _ in range(count):
+        r = rnd.random()
+        if r < 0.45:
+            calls += gates(rnd.randint(1, 4))
+        elif r < 0.7:
+            calls.append(measure())
+        elif r < 0.78:
+            calls.append(("reset", (rnd.randrange(n),)))
+        else:
+            reg = rnd.choice(sorted(regs))
+            body = gates(rnd.randint(1, 3))
+            if rnd.random() < 0.4:
+                body.append(measure())
+            if rnd.random() < 0.3:
+                reg2 = rnd.choice(sorted(regs))
+                body.append(("if", (reg2, rnd.randrange(1 << regs[reg2]), gates(2))))
+            calls.append(("if", (reg, rnd.randrange(1 << regs[reg]), body)))
+    return calls, regs

@@ -23,6 +23,32 @@ def _bind(calls, params):
     return out
 
 
+def _calls(calls):
+    """JSON calls -> (name, args) tuples; an `if` carries its inner calls."""
+    out = []
+    for name, args in calls:
+        if name == "if":
+            out.append(("if", (args[0], args[1], _calls(args[2]))))
+        else:
+            out.append((name, tuple(args)))
+    return out
+
+
+def _check_run(case, counts, creg_counts, shots):
+    """counts {index: n}: the outcome frequencies ket's test asserts."""
+    n, want = case["n"], case["expected"]
+    probs = {format(int(i), "b").zfill(n)[::-1]: c / shots for i, c in counts.items()}
+    if "probs" in want:
+        for bs, p in want["probs"].items():
+            assert abs(probs.get(bs, 0.0) - p) < case["tol"]
+    if "bit" in want:
+        pos, val, p, tol = want["bit"]
+        got = sum(v for bs, v in probs.items() if bs[pos] == val)
+        assert abs(got - p) <= tol
+    for reg, fr in want.get("cregs", {}).items():
+        assert np.max(np.abs(np.array(creg_counts[reg]) / shots - np.array(fr))) < case["tol"]
+
+
 def _init_index(case):
     bs = case.get("initialState")
     return int(bs[::-1], 2) if bs else 0
@@ -60,6 +86,10 @@ def test_oracle_reproduces_known_answer(case):
         assert abs(O.expectation(n, _oracle_state(case), case["pauli"]) - case["expected"]) < tol
     elif q == "marginals":
         assert np.max(np.abs(O.marginals(n, _oracle_state(case)) - np.array(case["expected"]))) < tol
+    elif q == "run":
+        counts, creg_counts = O.run_shots(n, _calls(case["calls"]), case["shots"], case["seed"],
+                                          cregs=case["cregs"])
+        _check_run(case, counts, creg_counts, case["shots"])
     elif q == "gradient":
         ps = case["params"]
         for i, want in enumerate(case["expected"]):
@@ -85,6 +115,17 @@ def test_product_path_reproduces_known_answer(case):
         calls = case["calls"]
         got = gradient(lambda ps: build(n, _bind(calls, ps)), case["hamiltonian"], case["params"])
         assert np.max(np.abs(np.array(got) - np.array(case["expected"]))) < tol
+        return
+    if q == "run":
+        c = build(n, _calls(case["calls"]), cregs=case["cregs"])
+        d = c.run(shots=case["shots"], seed=case["seed"])
+        shots = case["shots"]
+        counts = {int(k): round(v * shots) for k, v in d.histogram.items()}
+        creg_counts = {k: [round(v * shots) for v in fr] for k, fr in d.cregs.items()}
+        _check_run(case, counts, creg_counts, shots)
+        want = O.run_shots(n, _calls(case["calls"]), shots, case["seed"], cregs=case["cregs"])
+        assert list(counts.items()) == list(want[0].items())       # identical counts, first-seen order
+        assert creg_counts == want[1]
         return
     c = build(n, [(nm, tuple(a)) for nm, a in case["calls"]])
     if q == "amplitudes":

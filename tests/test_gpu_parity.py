@@ -323,15 +323,52 @@ def _expected_dist(n, shots, counts, cr):
 
 
 @pytest.mark.parametrize("n,noise,seed", [(3, "harmony", 1), (5, {"p1": 0.02, "p2": 0.05, "pMeas": 0.03}, 2),
-                                          (6, None, 3), (9, "aria-1", 4)])
+                                          (6, None, 3), (9, "aria-1", 4), (14, "harmony", 5),
+                                          (15, {"p1": 0.01, "p2": 0.03, "pMeas": 0.02}, 6)])
 def test_batched_shots_equal_oracle(n, noise, seed):
     """run() without `if`: all shots as copies of one device state (ksv_shots_*)
-    give the oracle's per-shot counts, creg frequencies and first-seen order."""
+    give the oracle's per-shot counts, creg frequencies and first-seen order.  At
+    14 and 15 qubits a copy exceeds one tile: Pauli events run as their own
+    launches and every measurement or reset splits the gate segments."""
     from ket_amd.noise import resolve_noise
     calls = shot_calls(n, 40, seed)
     d = build(n, calls, cregs={"c": 2}).run(shots=300, seed=seed, noise=noise)
     counts, cr = O.run_shots(n, calls, 300, seed, resolve_noise(noise), cregs={"c": 2})
     e = _expected_dist(n, 300, counts, cr)
+    assert list(d.probs.items()) == list(e.probs.items())
+    assert d.cregs == e.cregs
+
+
+def test_noisy_bench_shape_equals_oracle():
+    """The noisy benchmark's shape (12 qubits, the 20-layer random circuit, every qubit
+    measured into a register, 'harmony' noise) through the register-round copy path:
+    counts, register frequencies and first-seen order equal the oracle's per-shot
+    restatement."""
+    from ket_amd.noise import resolve_noise
+    n, shots = 12, 200
+    calls = workloads.random_layer_calls(n, 20, seed=42) + [("measure", (q, "m", q)) for q in range(n)]
+    d = build(n, calls, cregs={"m": n}).run(shots=shots, seed=42, noise="harmony")
+    counts, cr = O.run_shots(n, calls, shots, 42, resolve_noise("harmony"), cregs={"m": n})
+    e = _expected_dist(n, shots, counts, cr)
+    assert list(d.probs.items()) == list(e.probs.items())
+    assert d.cregs == e.cregs
+
+
+@pytest.mark.parametrize("n,seed,noise", [(3, 1, None), (5, 2, "harmony"), (7, 3, {"p1": 0.02, "p2": 0.05, "pMeas": 0.03}),
+                                          (12, 4, "aria-1"), (14, 5, "harmony")])
+def test_if_circuits_equal_oracle(n, seed, noise):
+    """run() with `if` (the shot-by-shot driver; each shot's draw count depends on its
+    measured values): seeded circuits with mid-circuit measurements into multi-bit
+    registers, resets, nested `if` blocks holding gates and measurements, and noise,
+    give the oracle's per-shot counts, register frequencies and first-seen order."""
+    from helpers import if_calls
+    from ket_amd.noise import resolve_noise
+    calls, regs = if_calls(n, 16, seed)
+    assert any(name == "if" for name, _ in calls)
+    shots = 150
+    d = build(n, calls, cregs=regs).run(shots=shots, seed=seed, noise=noise)
+    counts, cr = O.run_shots(n, calls, shots, seed, resolve_noise(noise), cregs=regs)
+    e = _expected_dist(n, shots, counts, cr)
     assert list(d.probs.items()) == list(e.probs.items())
     assert d.cregs == e.cregs
 
@@ -354,9 +391,9 @@ def test_batched_shots_equal_per_shot_path():
 @pytest.mark.parametrize("n,precision,wide", [(13, "f64", False), (13, "f32", False), (10, "f32", False),
                                                (9, "f64", True)])
 def test_batched_shots_equal_per_shot_path_variants(n, precision, wide):
-    """Batched and shot-by-shot drivers agree exactly: copies that fit one LDS tile
-    (per-copy passes; FP32 up to 13 qubits), 13 FP64 qubits (segmented passes), and a
-    5-qubit unitary that splits a per-copy segment."""
+    """Batched and shot-by-shot drivers agree exactly: copies that fit one tile
+    (per-copy passes: LDS passes up to 10 qubits, register rounds for 11..13, FP64 and
+    FP32), and a 5-qubit unitary that splits a per-copy segment."""
     import random
     from helpers import random_unitary
     calls = shot_calls(n, 50, 11 + n, measures=2, resets=1)

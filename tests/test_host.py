@@ -417,3 +417,27 @@ def test_qaoa_batch_rejects_repeated_edges(edges):
     from ket_amd._native import QaoaBatch, RangeError
     with pytest.raises(RangeError, match="repeated edge"):
         QaoaBatch([(14, [(0, 3)]), (14, edges)], p=2)
+
+
+def test_creg_value_is_javascripts_32_bit_integer():
+    """cregValue (src/circuit.ts:164-166) ORs 1 << i with JavaScript's 32-bit shift
+    (count mod 32) and |: bit 31 gives a negative value and bit 32 aliases bit 0.
+    The host driver and the oracle agree with that arithmetic, restated here from
+    the ECMAScript definitions of << and |."""
+    import ketsv_oracle as O
+    from ket_amd.noise import _creg_value
+
+    def js(bits):
+        acc = 0
+        for i, b in enumerate(bits):
+            x = ((1 << (i % 32)) & 0xFFFFFFFF) if b else 0       # 1 << i, ToInt32 bit pattern
+            acc = (acc | x) & 0xFFFFFFFF                          # ToInt32(acc) | ToInt32(x)
+        return acc - (1 << 32) if acc >= 1 << 31 else acc
+
+    cases = [[], [True], [False, True], [True, True, False, True], [False] * 31 + [True],
+             [False] * 32 + [True], [True] + [False] * 31 + [True], [True] * 40]
+    for bits in cases:
+        assert _creg_value({"c": bits}, "c") == js(bits) == O.creg_value(bits)
+    assert js([False] * 31 + [True]) == -(1 << 31)
+    assert js([False] * 32 + [True]) == 1
+    assert _creg_value({}, "missing") == 0
