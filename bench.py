@@ -59,6 +59,43 @@ def traffic_from_profiles(kernel_tag):
     return None
 
 
+def streaming_ceiling(st, reps=3):
+    """Read + write rate of the library's own streaming copy kernel (ksv_copy_state)
+    on a state of the same size: 2 x state bytes per copy over the best of `reps`
+    synchronous copies (GB/s), the measured ceiling beside the 8 TB/s peak."""
+    from ket_amd._native import State
+    cp = State(st.n, st.precision, prune=False)
+    cp.copy_from(st)                       # first touch
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cp.copy_from(st)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    cp.free()
+    return 2 * (16 if st.precision == "f64" else 8) * (1 << st.n) / best / 1e9
+
+
+def prune_mode_section(n, ops, mats, steps=2):
+    """The library's default mode on the same circuit: ket's per-gate pruning
+    (accumulate's 1e-14 rule after every gate, every product and sum rounded in
+    ket's order), which forbids fusing gates, so it runs as register-round
+    passes (VALU).  Reported beside the exact-mode line, not as `value`."""
+    from ket_amd._native import State
+    st = State(n, "f64", prune=True)
+    prog = st.program((ops, mats))
+    info = prog.info()
+    dt, stats = run_program_steps(st, prog, steps, 1)
+    norm = st.norm2()
+    prog.free()
+    st.free()
+    avg_ms = stats["pass_ms"] / max(1, stats["pass_launches"])
+    gbs = stats["pass_bytes"] / max(1, stats["pass_launches"]) / (avg_ms * 1e-3) / 1e9
+    return {"gate_updates_per_s": round(len(ops) * steps / dt, 2), "ms_per_step": round(dt / steps * 1e3, 2),
+            "passes": info["passes"], "kernel": "ksv::k_rpass", "avg_launch_ms": round(avg_ms, 3),
+            "hbm_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "norm2": norm}
+
+
 def run_program_steps(st, prog, steps, warmup):
     """Reset to |0...0> and run the program `warmup` + `steps` times; time the last `steps`."""
     for _ in range(warmup):
@@ -128,10 +165,9 @@ def workload_f32(args):
     mfma = {"achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
             "frac": round(tflops / FP32_MFMA_PEAK_TFS, 4), "executed_tflops": round(tflops * 6 / 8, 2),
             "executed_frac": round(tflops * 6 / 8 / FP32_MFMA_PEAK_TFS, 4)}
-    mfma_bound = flops * 6 / 8 / (FP32_MFMA_PEAK_TFS * 1e12) > per_launch / (HBM_PEAK_GBS * 1e9)
-    main = dict(mfma if mfma_bound else hbm)
-    roofline = {"bound": "mfma" if mfma_bound else "hbm", **main, "traffic": None, "kernel": "ksv::k_bpass",
-                "avg_launch_ms": round(avg_ms, 4), "hbm": hbm, "mfma": mfma}
+    mfma["binding_at_published_peaks"] = flops * 6 / 8 / (FP32_MFMA_PEAK_TFS * 1e12) > per_launch / (HBM_PEAK_GBS * 1e9)
+    roofline = {"bound": "hbm", **hbm, "traffic": None, "kernel": "ksv::k_bpass", "avg_launch_ms": round(avg_ms, 4),
+                "mfma": mfma}
     # accuracy at 30 qubits: the same seeded circuit in FP64 and in FP32
     na = 30
     ca = workloads.random_layers(na, args.layers, seed=42).tape()
@@ -394,9 +430,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 24-qubit adjoint vs parameter-shift section")
+    ap.add_argument("--no-prune-line", action="store_true", help="skip the ket-pruning-mode timing of the same circuit")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default schedule")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="end with one 16 GiB device-to-device state copy (HBM counter calibration)")
     ap.add_argument("--shots", type=int, default=0, help="noisy workload: shots (default 10000)")
     ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy", "adjoint"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
@@ -459,11 +494,8 @@ def main():
     if abs(norm - 1) > 1e-10:
         raise SystemExit(f"bench: norm check failed ({norm})")
     prog.free()
-    if args.copy_probe:
-        cp = State(n, "f64", prune=False)
-        cp.copy_from(st)
-        cp.free()
-        log("bench: copy probe done")
+    ceiling = streaming_ceiling(st, reps=3)
+    log(f"bench: own streaming copy (read + write of the state): {ceiling:.1f} GB/s")
     st.free()
 
     ms_per_step = dt / args.steps * 1e3
@@ -492,11 +524,20 @@ def main():
         mfma_bound = executed / (FP64_MFMA_PEAK_TFS * 1e12) > per_launch / (HBM_PEAK_GBS * 1e9)
     else:
         mfma, mfma_bound = None, False
-    main_part = dict(mfma, bound="mfma") if mfma_bound else dict(hbm, bound="hbm")
-    main_part.pop("flops_per_launch", None)
-    roofline = {"bound": main_part.pop("bound"), **main_part, "traffic": traffic, "kernel": kname,
-                "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
-                "launches_per_step": launches // args.steps, "hbm": hbm, "mfma": mfma}
+    if mfma:
+        mfma.pop("flops_per_launch", None)
+        mfma["binding_at_published_peaks"] = bool(mfma_bound)
+    # the pass streams the state once each way: HBM is the roofline (algorithmic bytes
+    # 2 x 2^n x 16 per launch over the HIP-event launch time); the fused blocks' MFMA
+    # work is reported beside it
+    roofline = {"bound": "hbm", **hbm, "traffic": traffic,
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and workload, "
+                                   "calibrated by a plain copy (profiles/hbm_traffic_k_bpass_random30.md)")
+                if traffic else None,
+                "kernel": kname, "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
+                "launches_per_step": launches // args.steps,
+                "ceiling_own_copy_gbs": round(ceiling, 1), "frac_of_own_copy": round(achieved / ceiling, 4),
+                "mfma": mfma}
     line = {"metric": METRIC, "value": round(value, 2), "unit": "gate-updates/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -506,8 +547,12 @@ def main():
                        "unfused_bound_gates_per_s": round(HBM_PEAK_GBS * 1e9 / (2 * 16 * 2 ** n), 1),
                        "schedule": {"kernel": args.kernel, "bq": args.bq, "btile": args.btile, "rtile": args.rtile,
                                     "bthreads": args.bthreads},
-                       "autotune": tuned, "parallelism": "single GPU"},
+                       "autotune": tuned, "parallelism": "single GPU",
+                       "prune": False, "mode": "exact (no per-gate pruning; the fused-block schedule)"},
             "roofline": roofline}
+    if not args.no_prune_line:
+        log("bench: ket-pruning mode")
+        line["ket_pruning_mode"] = prune_mode_section(n, ops, mats)
     if not args.no_adjoint:
         log("bench: adjoint section")
         line["adjoint"] = adjoint_section()

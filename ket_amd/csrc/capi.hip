@@ -242,7 +242,7 @@ int ksv_copy_state(ksv_state* dst, const ksv_state* src) {
     if (!dst || !src) return fail(KSV_ERR_HANDLE, "null state");
     if (dst->n != src->n || dst->prec != src->prec) return fail(KSV_ERR_TYPE, "state shapes differ");
     HIPCHK(hipStreamSynchronize(src->st));
-    HIPCHK(hipMemcpyAsync(dst->psi, src->psi, state_bytes(dst), hipMemcpyDeviceToDevice, dst->st));
+    HIPCHK(k_copy(dst->n, dst->prec, dst->psi, src->psi, dst->st));
     HIPCHK(hipStreamSynchronize(dst->st));
     return KSV_OK;
 }

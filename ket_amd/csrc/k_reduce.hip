@@ -4,6 +4,8 @@
 // histograms for top-k.  Every sum accumulates in FP64 whatever the state
 // precision.  Reductions use a fixed grid and a fixed combine order, so a
 // result is bit-reproducible run to run.
+#include <algorithm>
+
 #include "kcommon.h"
 
 namespace ksv {
@@ -261,9 +263,33 @@ hipError_t k_project(int n, int prec, void* psi, int q, int outcome, double scal
     return hipGetLastError();
 }
 
+// Device-to-device state copy: each thread moves four 16-byte words per step
+// (all four loads issued before the stores), consecutive threads take
+// consecutive words, each workgroup walks its own contiguous span.  Its
+// read + write rate is the streaming ceiling the pass kernels are measured
+// against (a pass also reads and writes every amplitude once).
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    uint64_t words, uint64_t span) {
+    const uint64_t b0 = (uint64_t)blockIdx.x * span;
+    const uint64_t b1 = b0 + span < words ? b0 + span : words;
+    uint64_t i = b0 + threadIdx.x;
+    for (; i + 768 < b1; i += 1024) {
+        const uint4 a = src[i], b = src[i + 256], c = src[i + 512], d = src[i + 768];
+        dst[i] = a;
+        dst[i + 256] = b;
+        dst[i + 512] = c;
+        dst[i + 768] = d;
+    }
+    for (; i < b1; i += 256) dst[i] = src[i];
+}
+
 hipError_t k_copy(int n, int prec, void* dst, const void* src, hipStream_t st) {
-    size_t bytes = (size_t)(prec == KSV_F64 ? 16 : 8) << n;
-    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+    const uint64_t words = ((uint64_t)(prec == KSV_F64 ? 16 : 8) << n) / 16;   // n >= 1: at least one word
+    const uint64_t blocks = std::min<uint64_t>(8192, (words + 1023) / 1024);
+    const uint64_t span = ((words + blocks - 1) / blocks + 1023) / 1024 * 1024;
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((words + span - 1) / span)), dim3(256), 0, st,
+                       (const uint4*)src, (uint4*)dst, words, span);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
