@@ -626,6 +626,30 @@ class Circuit:
 
         return Circuit(self.qubits, rebuild(self._ops()), self._cregs, self._gates)
 
+    # ---- wire formats (src/circuit.ts:1960-2090, :3768-3803) --------------------
+    def toJSON(self):
+        """CircuitJSON: {ket: 1, qubits, cregs, gates, ops}; gate matrices are
+        rebuilt from each op's name and params on load."""
+        from .wire import to_json
+        return to_json(self)
+
+    @staticmethod
+    def fromJSON(j):
+        """A circuit from a CircuitJSON object or its JSON text."""
+        from .wire import from_json
+        return from_json(j)
+
+    def toQASM(self):
+        """OpenQASM 2.0 text of this circuit."""
+        from .wire import to_qasm
+        return to_qasm(self)
+
+    @staticmethod
+    def fromQASM(source):
+        """A circuit from OpenQASM 2.0 or 3.0 text."""
+        from .wire import from_qasm
+        return from_qasm(source)
+
     # ---- simulation -------------------------------------------------------------
     def _is_pure(self):
         return not any(op[0] in _PURE_BLOCKERS for op in self._ops())
