@@ -232,6 +232,27 @@ int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const double* mats
                 const uint64_t* xmask, const uint64_t* zmask, const double* coeff, size_t nterms,
                 double* grad, double* energy);
 
+/* Batched adjoint: one tape with B = 2^(s->n - n) parameter sets at once (the
+ * inner loop of vqe() / gradient() / minimize() over many parameter vectors,
+ * src/algorithms.ts:301-336, :536-547; ket runs each set separately).  Copy t of
+ * the state (amplitudes [t 2^n, (t + 1) 2^n)) runs the tape with its own
+ * matrix array, mats + t * nmats, in the layout of the tape's offsets (gate and
+ * derivative matrices alike); the records, the Pauli terms (on qubits < n) and
+ * the tape's structure are shared.  Every run starts each copy from |0...0>,
+ * evaluates E_t = <psi_t|H|psi_t> and, with want_grad, dE_t/dparam by the
+ * adjoint method; the launch sequence is captured once in a HIP graph and
+ * replayed with each run's matrices.  energy[t], grad[t * nparam + j].
+ * Unitaries wider than three qubits are not supported.  An op whose matrices
+ * were diagonal in every copy at creation must stay diagonal (KSV_ERR_TYPE
+ * otherwise: create the batch again).  The batch uses the state's buffers:
+ * destroy it before the state. */
+typedef struct ksv_adj_batch ksv_adj_batch;
+int ksv_adjoint_batch_create(ksv_state* s, int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                             const ksv_param* params, size_t nrec, size_t nparam, const uint64_t* xmask,
+                             const uint64_t* zmask, const double* coeff, size_t nterms, ksv_adj_batch** out);
+int ksv_adjoint_batch_run(ksv_adj_batch* b, const double* mats, int want_grad, double* energy, double* grad);
+int ksv_adjoint_batch_destroy(ksv_adj_batch* b);
+
 /* ---- measurement ---------------------------------------------------------------- */
 /* collapseQubit (src/circuit.ts:55-70): outcome = r < p1; project and scale
  * by 1/sqrt(p_outcome) in FP64. */

@@ -7,11 +7,11 @@ ABI of include/ketsv.h.
 """
 from . import gates
 from ._native import RangeError, State, device_count
-from .algorithms import (AnsatzFn, efficientSU2, energy_and_gradient, gradient, iqft, maxCutHamiltonian, minimize,
-                         qaoa, qft, realAmplitudes, vqe)
+from .algorithms import (AnsatzFn, efficientSU2, energy_and_gradient, energy_and_gradient_batch, gradient, iqft,
+                         maxCutHamiltonian, minimize, qaoa, qft, realAmplitudes, vqe, vqe_batch)
 from .circuit import Circuit, Distribution, configure, make_prng, prng_draws
 from .noise import DEVICES
 
 __all__ = ["Circuit", "Distribution", "RangeError", "State", "gates", "qft", "iqft", "qaoa", "maxCutHamiltonian",
-           "realAmplitudes", "efficientSU2", "vqe", "gradient", "energy_and_gradient", "minimize", "AnsatzFn",
+           "realAmplitudes", "efficientSU2", "vqe", "gradient", "energy_and_gradient", "energy_and_gradient_batch", "vqe_batch", "minimize", "AnsatzFn",
            "configure", "make_prng", "prng_draws", "DEVICES", "device_count"]

@@ -75,6 +75,10 @@ SIGNATURES = [
     ("ksv_sample_noalloc", c_int, [_P, _P, c_size_t, _P]),
     ("ksv_expect_pauli", c_int, [_P, _P, _P, _P, c_size_t, c_int, _P, _DP]),
     ("ksv_adjoint", c_int, [_P, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P, c_size_t, _P, _DP]),
+    ("ksv_adjoint_batch_create", c_int, [_P, c_int, _P, c_size_t, _P, c_size_t, _P, c_size_t, c_size_t, _P, _P, _P,
+                                         c_size_t, POINTER(c_void_p)]),
+    ("ksv_adjoint_batch_run", c_int, [_P, _P, c_int, _P, _P]),
+    ("ksv_adjoint_batch_destroy", c_int, [_P]),
     ("ksv_collapse", c_int, [_P, c_int, c_double, POINTER(c_int)]),
     ("ksv_shots_init", c_int, [_P, c_int, c_uint64]),
     ("ksv_shots_init_identity", c_int, [_P, c_int]),
@@ -403,6 +407,57 @@ class Program:
         if getattr(self, "_h", None) is not None:
             lib().ksv_program_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class AdjointBatch:
+    """B parameter sets of one tape on the copies of one device state
+    (ksv_adjoint_batch_*): energies and adjoint gradients per set, one graph
+    replay per evaluation."""
+
+    def __init__(self, n, copies_log, precision, tape, mats_all, records, nparam, xmask, zmask, coeff,
+                 adj_tile=0, adj_threads=0):
+        ops, _ = tape
+        self.B = 1 << copies_log
+        self.nparam = int(nparam)
+        self.nmats = mats_all.shape[1]
+        self.state = State(n + copies_log, precision, prune=False, adj_tile=adj_tile, adj_threads=adj_threads)
+        rec = np.ascontiguousarray(records, dtype=PARAM_DTYPE)
+        x = np.ascontiguousarray(xmask, dtype=np.uint64)
+        z = np.ascontiguousarray(zmask, dtype=np.uint64)
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        m = np.ascontiguousarray(mats_all, dtype=np.float64)
+        h = c_void_p()
+        try:
+            check(lib().ksv_adjoint_batch_create(self.state.handle, int(n), ptr(ops), len(ops), ptr(m), self.nmats,
+                                                 ptr(rec), len(rec), self.nparam, ptr(x), ptr(z), ptr(c), len(c),
+                                                 ctypes.byref(h)))
+        except Exception:
+            self.state.free()
+            raise
+        self._h = h
+
+    def run(self, mats_all, grad=True):
+        m = np.ascontiguousarray(mats_all, dtype=np.float64)
+        if m.shape != (self.B, self.nmats):
+            raise TypeError(f"matrices must be {self.B} x {self.nmats}")
+        e = np.empty(self.B)
+        g = np.zeros((self.B, max(1, self.nparam)))
+        check(lib().ksv_adjoint_batch_run(self._h, ptr(m), 1 if grad else 0, ptr(e), ptr(g)))
+        return (e, g[:, :self.nparam]) if grad else e
+
+    def free(self):
+        if getattr(self, "_h", None) is not None:
+            lib().ksv_adjoint_batch_destroy(self._h)
+            self._h = None
+        if getattr(self, "state", None) is not None:
+            self.state.free()
+            self.state = None
 
     def __del__(self):
         try:

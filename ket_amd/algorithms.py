@@ -292,6 +292,95 @@ def energy_and_gradient(ansatz, hamiltonian, params):
     return float(e), [float(g) for g in grad]
 
 
+# ---- many parameter sets at once ---------------------------------------------------
+_BATCHES = {}            # structure key -> AdjointBatch (the few most recent)
+_BATCH_KEEP = 4
+
+
+def _packed_with_derivatives(flat, records):
+    """(ops, mats with every record's dG appended, records as PARAM_DTYPE)."""
+    ops, mats = pack(flat)
+    extra = []
+    rec = np.zeros(len(records), dtype=PARAM_DTYPE)
+    base = len(mats)
+    for j, (oi, pi, coeff, d) in enumerate(records):
+        rec[j] = (oi, pi, coeff, base + len(extra))
+        for v in d:
+            extra.append(complex(v).real)
+            extra.append(complex(v).imag)
+    return ops, np.concatenate([np.asarray(mats, dtype=np.float64), np.asarray(extra, dtype=np.float64)]), rec
+
+
+def _batch_tapes(ansatz, params_list):
+    """Trace every parameter set; all must share one structure (ops, qubits,
+    parameter records).  Returns (n, ops, mats_all [B x nmats], records)."""
+    ops0 = rec0 = None
+    mats_all = []
+    n = None
+    for ps in params_list:
+        tr = trace(ansatz, [float(p) for p in ps])
+        if tr is None:
+            raise TypeError("a batched gradient needs an ansatz built from affine functions of its parameters")
+        flat, records, _, nq = tr
+        ops, mats, rec = _packed_with_derivatives(flat, records)
+        if ops0 is None:
+            ops0, rec0, n = ops, rec, nq
+        elif nq != n or not np.array_equal(ops, ops0) or not np.array_equal(rec, rec0) or len(mats) != len(mats_all[0]):
+            raise TypeError("every parameter set must give the ansatz the same structure")
+        mats_all.append(mats)
+    return n, ops0, np.stack(mats_all), rec0
+
+
+def _batch_run(ansatz, hamiltonian, params_list, grad):
+    from ._native import AdjointBatch
+    params_list = [list(p) for p in params_list]
+    if not params_list:
+        return ([], []) if grad else []
+    count = len(params_list)
+    b = max(0, (count - 1).bit_length())
+    padded = params_list + [params_list[-1]] * ((1 << b) - count)
+    n, ops, mats_all, rec = _batch_tapes(ansatz, padded)
+    nparam = len(params_list[0])
+    x, z, c = _terms(hamiltonian, n)
+    key = (id(ansatz), n, b, OPTIONS["precision"], ops.tobytes(), rec.tobytes(), x.tobytes(), z.tobytes(),
+           c.tobytes(), nparam, OPTIONS["adj_tile"], OPTIONS["adj_threads"])
+    bt = _BATCHES.pop(key, None)
+    for attempt in (0, 1):
+        if bt is None:
+            while len(_BATCHES) >= _BATCH_KEEP:
+                _BATCHES.pop(next(iter(_BATCHES))).free()
+            bt = AdjointBatch(n, b, OPTIONS["precision"], (ops, mats_all[0]), mats_all, rec, nparam, x, z, c,
+                              adj_tile=OPTIONS["adj_tile"], adj_threads=OPTIONS["adj_threads"])
+        try:
+            out = bt.run(mats_all, grad=grad)
+            break
+        except TypeError as e:
+            # an op that was diagonal for every set at creation is not any more
+            if attempt or "diagonal" not in str(e):
+                raise
+            bt.free()
+            bt = None
+    _BATCHES[key] = bt
+    if grad:
+        e, g = out
+        return [float(v) for v in e[:count]], [[float(v) for v in row] for row in g[:count]]
+    return [float(v) for v in out[:count]]
+
+
+def energy_and_gradient_batch(ansatz, hamiltonian, params_list):
+    """(energies, gradients) for many parameter sets of one ansatz in one batched
+    evaluation: each set runs on its own copy of one device state, both adjoint
+    sweeps cover every copy per launch, and the launch sequence is a HIP graph
+    replayed per call (no ket counterpart; ket evaluates sets one by one,
+    src/algorithms.ts:301-336, :536-547).  Exact (no pruning)."""
+    return _batch_run(ansatz, hamiltonian, params_list, True)
+
+
+def vqe_batch(ansatz, hamiltonian, params_list):
+    """<H> for many parameter sets of one ansatz in one batched evaluation (exact)."""
+    return _batch_run(ansatz, hamiltonian, params_list, False)
+
+
 def _shifted_matrix(flat_op, pi, delta):
     """Tape matrix of a traced op with parameter `pi` moved by `delta`."""
     from .circuit import _resolve_parametric

@@ -12,6 +12,7 @@
 // index).  Unitaries wider than three qubits run alone through the general
 // engine, on phi and then on lambda.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -27,6 +28,7 @@ struct AItem {
     std::vector<int> tg, ctl;   // targets (MSB first), controls
     std::vector<cd> m;          // G^dag (AK_DENSE and wide), row-major
     std::vector<int> recs;      // parameter records of the op
+    int src = -1;               // tape index of the op
     uint64_t need = 0, touch = 0;
     bool diag = false;          // AK_DENSE: G^dag and every record's dG diagonal
 };
@@ -232,6 +234,7 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
             a.m = dagger(m, dim);
         }
         a.recs = recs_of[i];
+        a.src = (int)i;
         if (a.kind == AK_DENSE) {
             const int d = 1 << a.tg.size();
             a.diag = offdiag_zero(a.m, d);
@@ -324,6 +327,10 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
                 }
             }
             a.nrec = (int32_t)it.recs.size();
+            plan.aop_src.push_back(it.src);
+            std::array<int32_t, 3> ar = {-1, -1, -1};
+            for (size_t r = 0; r < it.recs.size() && r < 3; r++) ar[r] = it.recs[r];
+            plan.aop_rec.push_back(ar);
             const int dim = 1 << a.k;
             for (size_t r = 0; r < it.recs.size(); r++) {
                 const ksv_param& p = params[it.recs[r]];
@@ -346,6 +353,144 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
     if (plan.mats.size() / 2 > (size_t)INT32_MAX) {
         err = "adjoint tape too large";
         return KSV_ERR_RANGE;
+    }
+    return KSV_OK;
+}
+
+// csrswap's 4x4 on (a, b) where the control is 1 (src/statevector.ts:177-207)
+static std::vector<cd> csrswap_block() {
+    const double s = 1 / std::sqrt(2.0);
+    std::vector<cd> m(16, cd(0, 0));
+    m[0] = m[15] = 1;
+    m[5] = m[10] = cd(s, 0);
+    m[6] = m[9] = cd(0, s);
+    return m;
+}
+
+int adjoint_plan_mats(const AdjPlan& plan, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                      const ksv_param* params, std::vector<double>& out, std::string& err) {
+    out.clear();
+    out.reserve(plan.mats.size());
+    std::vector<cd> m, dm;
+    for (size_t x = 0; x < plan.aops.size(); x++) {
+        const AOp& a = plan.aops[x];
+        const int src = plan.aop_src[x];
+        if (src < 0 || (size_t)src >= nops) {
+            err = "adjoint plan and tape differ";
+            return KSV_ERR_TYPE;
+        }
+        const ksv_op& o = ops[src];
+        const int dim = 1 << a.k;
+        if (a.kind == AK_DENSE) {
+            if (a.mat != (int32_t)(out.size() / 2)) {
+                err = "adjoint plan and tape differ";
+                return KSV_ERR_TYPE;
+            }
+            if (o.kind == KSV_CSRSWAP) {
+                m = csrswap_block();
+            } else if (!read_mat(mats, nmats, o.mat, dim, m)) {
+                err = "op " + std::to_string(src) + ": bad matrix offset";
+                return KSV_ERR_TYPE;
+            }
+            m = dagger(m, dim);
+            if ((a.flags & AOP_DIAG) && !offdiag_zero(m, dim)) {
+                err = "op " + std::to_string(src) + " is no longer diagonal: the batch must be created again";
+                return KSV_ERR_TYPE;
+            }
+            for (const cd& z : m) {
+                out.push_back(z.real());
+                out.push_back(z.imag());
+            }
+        }
+        for (int r = 0; r < a.nrec; r++) {
+            const ksv_param& p = params[plan.aop_rec[x][r]];
+            if (!read_mat(mats, nmats, p.dmat, dim, dm)) {
+                err = "bad derivative matrix offset";
+                return KSV_ERR_TYPE;
+            }
+            if ((a.flags & AOP_DIAG) && !offdiag_zero(dm, dim)) {
+                err = "op " + std::to_string(src) + " is no longer diagonal: the batch must be created again";
+                return KSV_ERR_TYPE;
+            }
+            for (const cd& z : dm) {
+                out.push_back(z.real());
+                out.push_back(z.imag());
+            }
+        }
+    }
+    if (out.size() != plan.mats.size()) {
+        err = "adjoint plan and tape differ";
+        return KSV_ERR_TYPE;
+    }
+    return KSV_OK;
+}
+
+void adjoint_plan_undiag(AdjPlan& plan, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                         const ksv_param* params) {
+    std::vector<cd> m;
+    for (size_t x = 0; x < plan.aops.size(); x++) {
+        AOp& a = plan.aops[x];
+        if (!(a.flags & AOP_DIAG)) continue;
+        const int src = plan.aop_src[x];
+        const int dim = 1 << a.k;
+        bool diag = src >= 0 && (size_t)src < nops;
+        if (diag && ops[src].kind != KSV_CSRSWAP) diag = read_mat(mats, nmats, ops[src].mat, dim, m) && offdiag_zero(m, dim);
+        for (int r = 0; diag && r < a.nrec; r++)
+            diag = read_mat(mats, nmats, params[plan.aop_rec[x][r]].dmat, dim, m) && offdiag_zero(m, dim);
+        if (!diag) a.flags &= (uint8_t)~AOP_DIAG;
+    }
+}
+
+int invert_tape(const ksv_op* ops, size_t nops, const double* mats, size_t nmats, std::vector<ksv_op>& iops,
+                std::vector<double>& imats, std::string& err) {
+    iops.clear();
+    imats.clear();
+    std::vector<cd> m;
+    for (size_t i = nops; i-- > 0;) {
+        ksv_op o = ops[i];
+        int dim = 0;
+        switch (o.kind) {
+        case KSV_SINGLE: case KSV_CONTROLLED: dim = 2; break;
+        case KSV_TWO: dim = 4; break;
+        case KSV_UNITARY:
+            if (o.nq < 1 || o.nq > KSV_MAX_OPERANDS) { err = "unitary: bad operand count"; return KSV_ERR_TYPE; }
+            dim = 1 << o.nq;
+            break;
+        case KSV_CSRSWAP: {
+            // the inverse as a 3-qubit unitary on (control, a, b): identity where the
+            // control is 0, the block's conjugate transpose where it is 1
+            const std::vector<cd> v = dagger(csrswap_block(), 4);
+            ksv_op u;
+            std::memset(&u, 0, sizeof u);
+            u.kind = KSV_UNITARY;
+            u.nq = 3;
+            u.q[0] = o.q[0];
+            u.q[1] = o.q[1];
+            u.q[2] = o.q[2];
+            u.mat = (int64_t)imats.size();
+            for (int r = 0; r < 8; r++)
+                for (int c = 0; c < 8; c++) {
+                    cd z = r < 4 ? (r == c ? cd(1, 0) : cd(0, 0)) : (c < 4 ? cd(0, 0) : v[(size_t)(r - 4) * 4 + (c - 4)]);
+                    imats.push_back(z.real());
+                    imats.push_back(z.imag());
+                }
+            iops.push_back(u);
+            continue;
+        }
+        default: break;     // permutations and barriers are their own inverses
+        }
+        if (dim) {
+            if (!read_mat(mats, nmats, o.mat, dim, m)) { err = "op " + std::to_string(i) + ": bad matrix offset"; return KSV_ERR_TYPE; }
+            m = dagger(m, dim);
+            o.mat = (int64_t)imats.size();
+            for (const cd& z : m) {
+                imats.push_back(z.real());
+                imats.push_back(z.imag());
+            }
+        } else {
+            o.mat = -1;
+        }
+        iops.push_back(o);
     }
     return KSV_OK;
 }

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 
 #include "kcommon.h"
 
@@ -39,9 +40,13 @@ struct AdjPassArgs {
     int nops;
     int nslot;
     int k;
-    int pad;
+    int single;                // 1: phi only (lambda is neither read nor written)
     uint64_t ntiles;
     uint64_t tq_lo, tq_hi;     // physical bit of tile bit i, ascending
+    uint64_t mstride;          // doubles between the matrix arrays of consecutive copies
+    int cb;                    // 2^cb copies: the workgroups are split evenly among them, each
+                               // taking only its copy's tiles and reading its copy's matrices
+    int pad;
 };
 
 // Fields of chain member c, read when its step starts (scalar loads), so
@@ -226,15 +231,19 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
     V* phi = reinterpret_cast<V*>(a.phi);
     V* lam = reinterpret_cast<V*>(a.lam);
     const KSV_CONST uint64_t* ops = to_const(reinterpret_cast<const uint64_t*>(a.ops));
-    const KSV_CONST double* mats = to_const(a.mats);
+    // copies: workgroup w serves copy w / gc and walks that copy's tiles
+    const uint64_t gc = (uint64_t)gridDim.x >> a.cb;
+    const uint64_t copy = blockIdx.x / gc;
+    const uint64_t tc = a.ntiles >> a.cb;
+    const KSV_CONST double* mats = to_const(a.mats + copy * a.mstride);
 
     uint64_t g_t = 0;   // physical offset of the thread bits
 #pragma unroll
     for (int b = 0; b < LNT; b++)
         if (b < k) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(b < 8 ? a.tq_lo : a.tq_hi, b & 7);
 
-    for (uint64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-        uint64_t base = tile;
+    for (uint64_t t = blockIdx.x % gc; t < tc; t += gc) {
+        uint64_t base = copy * tc + t;
 #pragma unroll
         for (int i = 0; i < 16; i++)
             if (i < k) base = ins0_64(base, byte_at(i < 8 ? a.tq_lo : a.tq_hi, i & 7));
@@ -246,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                 for (int b = LNT; b < KT; b++)
                     if (b < k && ((xh >> b) & 1)) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
                 P[l] = phi[base | g_t | g_h];
-                L[l] = lam[base | g_t | g_h];
+                if (!a.single) L[l] = lam[base | g_t | g_h];
             }
         }
         __syncthreads();
@@ -324,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                 for (int b = LNT; b < KT; b++)
                     if (b < k && ((xh >> b) & 1)) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
                 phi[base | g_t | g_h] = P[l];
-                lam[base | g_t | g_h] = L[l];
+                if (!a.single) lam[base | g_t | g_h] = L[l];
             }
         }
     }
@@ -337,17 +346,55 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
     }
 }
 
-// out[r] = sum_g part[r * G + g], one workgroup per record.
+// out[r * C + c] = sum of part[r * G + g] over copy c's workgroups
+// g in [c G / C, (c + 1) G / C), one workgroup per (record r, copy c).
 __global__ __launch_bounds__(256) void k_recsum(const double* part, int G, double* out) {
     __shared__ double sh[4];
-    const size_t r = blockIdx.x;
+    const size_t r = blockIdx.x, C = gridDim.y, c = blockIdx.y;
+    const int gc = G / (int)C;
     double v = 0;
-    for (int g = threadIdx.x; g < G; g += 256) v += part[r * G + g];
+    for (int g = threadIdx.x; g < gc; g += 256) v += part[r * G + c * gc + g];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) out[r] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (threadIdx.x == 0) out[r * C + c] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Re <a|b> per copy: block (x, c) sums its share of copy c's N amplitudes
+// into part[c * gridDim.x + x]; k_rowsum then adds each copy's row in order.
+template <typename V>
+__global__ __launch_bounds__(256) void k_dotre_copies(const V* a, const V* b, uint64_t N, double* part) {
+    __shared__ double sh[4];
+    const uint64_t off = (uint64_t)blockIdx.y * N;
+    double s = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x) {
+        V x = a[off + i], y = b[off + i];
+        s += (double)x.x * y.x + (double)x.y * y.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void k_rowsum(const double* part, int cols, int rows, double* out) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        double v = 0;
+        for (int c = 0; c < cols; c++) v += part[(size_t)r * cols + c];
+        out[r] = v;
+    }
+}
+
+template <typename V>
+__global__ void k_fill_zero(V* p, uint64_t N) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N; i += (uint64_t)gridDim.x * blockDim.x) {
+        V z;
+        z.x = 0;
+        z.y = 0;
+        p[i] = z;
+    }
 }
 
 // lambda = sum_t c_t P_t psi for up to 8 terms per launch, accumulated into
@@ -591,12 +638,318 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
         AHIP(launch_adjpass(prec, atile, athreads, G, a, st));
     }
     if (nrec) {
-        hipLaunchKernelGGL(k_recsum, dim3((unsigned)nrec), dim3(256), 0, st, d_part, G, d_sum);
+        hipLaunchKernelGGL(k_recsum, dim3((unsigned)nrec, 1), dim3(256), 0, st, d_part, G, d_sum);
         AHIP(hipGetLastError());
     }
     std::vector<double> sums(nr, 0.0);
     AHIP(hipMemcpyAsync(sums.data(), d_sum, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
     AHIP(hipStreamSynchronize(st));
     for (size_t r = 0; r < nrec; r++) grad[params[r].param] += 2.0 * params[r].coeff * sums[r];
+    return KSV_OK;
+}
+
+// ---- batched adjoint: many parameter sets of one tape ------------------------
+// The state holds B = 2^cb copies of n qubits (copy t = amplitudes
+// [t 2^n, (t + 1) 2^n)); copy t runs the tape with its own matrix array.  Both
+// sweeps are tile passes of k_adjpass over all copies at once: the forward
+// sweep is the backward planner's output for the inverse tape (its passes
+// apply G_1 .. G_N), on phi only; every workgroup serves one copy and reads
+// that copy's matrices, so per-record partials and the per-copy sums stay
+// apart.  The whole launch sequence is captured once in a HIP graph and
+// replayed with each call's matrices.
+struct ksv_adj_batch {
+    ksv_state* s = nullptr;
+    int n = 0, cb = 0, prec = KSV_F64, atile = 0, athreads = 256;
+    uint64_t B = 1;
+    void *psi = nullptr, *psi2 = nullptr;
+    hipStream_t st = nullptr;
+    std::vector<ksv_op> ops;
+    std::vector<ksv_param> params;
+    size_t nmats = 0, nrec = 0, nparam = 0, nterms = 0;
+    AdjPlan fwd, bwd;
+    size_t fm = 0, bm = 0;               // doubles of one copy's forward / backward matrices
+    int G = 0, gd = 0;                   // adjoint-pass workgroups; dot blocks per copy
+    void* blob = nullptr;
+    size_t off_faops = 0, off_baops = 0, off_slot = 0, off_mats = 0, off_part = 0, off_sums = 0, off_epart = 0,
+           off_energy = 0, off_terms = 0;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<double> hmats;           // host staging of every copy's plan matrices
+
+    ~ksv_adj_batch() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (int g = 0; g < 2; g++) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
+        if (blob) (void)hipFree(blob);
+    }
+};
+
+namespace {
+
+// every copy's forward and backward plan matrices, copy t at hmats[t * (fm + bm)]
+int batch_mats(ksv_adj_batch* b, const double* mats) {
+    std::string err;
+    std::vector<ksv_op> iops;
+    std::vector<double> imats, f, g;
+    b->hmats.resize(b->B * (b->fm + b->bm));
+    for (uint64_t t = 0; t < b->B; t++) {
+        const double* m = mats + t * b->nmats;
+        int rc = invert_tape(b->ops.data(), b->ops.size(), m, b->nmats, iops, imats, err);
+        if (!rc) rc = adjoint_plan_mats(b->fwd, iops.data(), iops.size(), imats.data(), imats.size(), nullptr, f, err);
+        if (!rc) rc = adjoint_plan_mats(b->bwd, b->ops.data(), b->ops.size(), m, b->nmats, b->params.data(), g, err);
+        if (rc) return adj_fail(rc, "copy " + std::to_string(t) + ": " + err);
+        std::copy(f.begin(), f.end(), b->hmats.begin() + t * (b->fm + b->bm));
+        std::copy(g.begin(), g.end(), b->hmats.begin() + t * (b->fm + b->bm) + b->fm);
+    }
+    return KSV_OK;
+}
+
+hipError_t batch_passes(ksv_adj_batch* b, const AdjPlan& plan, const AOp* d_aops, const double* d_mats, int single,
+                        const int32_t* d_slot, double* d_part) {
+    char* dev = static_cast<char*>(b->blob);
+    (void)dev;
+    for (const AdjStep& sp : plan.steps) {
+        AdjPassArgs a;
+        std::memset(&a, 0, sizeof a);
+        a.phi = b->psi;
+        a.lam = b->psi2;
+        a.ops = d_aops + sp.op_begin;
+        a.mats = d_mats;
+        a.slot_rec = d_slot ? d_slot + sp.slot_begin : nullptr;
+        a.part = d_part;
+        a.nops = sp.op_count;
+        a.nslot = single ? 0 : sp.slot_count;
+        a.k = sp.k;
+        a.single = single;
+        a.ntiles = 1ull << (b->n + b->cb - sp.k);
+        a.mstride = b->fm + b->bm;
+        a.cb = b->cb;
+        for (int i = 0; i < 8; i++) {
+            a.tq_lo |= (uint64_t)(uint8_t)sp.tq[i] << (8 * i);
+            a.tq_hi |= (uint64_t)(uint8_t)sp.tq[8 + i] << (8 * i);
+        }
+        hipError_t e = launch_adjpass(b->prec, b->atile, b->athreads, b->G, a, b->st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// the whole launch sequence of one evaluation (captured into a graph)
+hipError_t batch_schedule(ksv_adj_batch* b, bool grad) {
+    char* dev = static_cast<char*>(b->blob);
+    const int ntot = b->n + b->cb;
+    const uint64_t N = 1ull << ntot;
+    const double* d_mats = reinterpret_cast<const double*>(dev + b->off_mats);
+    hipError_t e = shots_basis(ntot, b->n, b->prec, b->psi, 0, b->st);
+    if (e == hipSuccess)
+        e = batch_passes(b, b->fwd, reinterpret_cast<const AOp*>(dev + b->off_faops), d_mats, 1, nullptr, nullptr);
+    if (e != hipSuccess) return e;
+    const int g = (int)std::min<uint64_t>((N + 255) / 256, 8192);
+    uint64_t* d_x = reinterpret_cast<uint64_t*>(dev + b->off_terms);
+    const size_t nt = std::max<size_t>(b->nterms, 1);
+    uint64_t* d_z = d_x + nt;
+    double* d_cr = reinterpret_cast<double*>(d_z + nt);
+    double* d_ci = d_cr + nt;
+    if (b->nterms == 0) {
+        if (b->prec == KSV_F64) hipLaunchKernelGGL((k_fill_zero<double2>), dim3(g), dim3(256), 0, b->st, (double2*)b->psi2, N);
+        else hipLaunchKernelGGL((k_fill_zero<float2>), dim3(g), dim3(256), 0, b->st, (float2*)b->psi2, N);
+    }
+    for (size_t t0 = 0; t0 < b->nterms; t0 += 8) {
+        const int m = (int)std::min<size_t>(8, b->nterms - t0);
+        if (b->prec == KSV_F64)
+            hipLaunchKernelGGL((k_hpsi<double2>), dim3(g), dim3(256), 0, b->st, (const double2*)b->psi, (double2*)b->psi2,
+                               N, d_x + t0, d_z + t0, d_cr + t0, d_ci + t0, m, t0 == 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k_hpsi<float2>), dim3(g), dim3(256), 0, b->st, (const float2*)b->psi, (float2*)b->psi2,
+                               N, d_x + t0, d_z + t0, d_cr + t0, d_ci + t0, m, t0 == 0 ? 1 : 0);
+    }
+    double* d_epart = reinterpret_cast<double*>(dev + b->off_epart);
+    double* d_energy = reinterpret_cast<double*>(dev + b->off_energy);
+    const uint64_t Nc = 1ull << b->n;
+    if (b->prec == KSV_F64)
+        hipLaunchKernelGGL((k_dotre_copies<double2>), dim3(b->gd, (unsigned)b->B), dim3(256), 0, b->st,
+                           (const double2*)b->psi, (const double2*)b->psi2, Nc, d_epart);
+    else
+        hipLaunchKernelGGL((k_dotre_copies<float2>), dim3(b->gd, (unsigned)b->B), dim3(256), 0, b->st,
+                           (const float2*)b->psi, (const float2*)b->psi2, Nc, d_epart);
+    hipLaunchKernelGGL(k_rowsum, dim3((unsigned)((b->B + 255) / 256)), dim3(256), 0, b->st, d_epart, b->gd, (int)b->B,
+                       d_energy);
+    e = hipGetLastError();
+    if (e != hipSuccess || !grad) return e;
+    double* d_part = reinterpret_cast<double*>(dev + b->off_part);
+    e = batch_passes(b, b->bwd, reinterpret_cast<const AOp*>(dev + b->off_baops), d_mats + b->fm, 0,
+                     reinterpret_cast<const int32_t*>(dev + b->off_slot), d_part);
+    if (e != hipSuccess) return e;
+    if (b->nrec)
+        hipLaunchKernelGGL(k_recsum, dim3((unsigned)b->nrec, (unsigned)b->B), dim3(256), 0, b->st, d_part, b->G,
+                           reinterpret_cast<double*>(dev + b->off_sums));
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int ksv_adjoint_batch_create(ksv_state* s, int n, const ksv_op* ops, size_t nops, const double* mats,
+                                        size_t nmats, const ksv_param* params, size_t nrec, size_t nparam,
+                                        const uint64_t* xmask, const uint64_t* zmask, const double* coeff,
+                                        size_t nterms, ksv_adj_batch** out) {
+    if (!out) return adj_fail(KSV_ERR_TYPE, "null output pointer");
+    *out = nullptr;
+    int ntot, prec, prune, tile, atile, athreads;
+    void *psi, *psi2;
+    hipStream_t st;
+    double* part;
+    int rc = adj_context(s, &ntot, &prec, &psi, &psi2, &st, &part, &prune, &tile, &atile, &athreads);
+    if (rc) return rc;
+    if (n < 1 || n > ntot) return adj_fail(KSV_ERR_RANGE, "copy width must be 1.." + std::to_string(ntot) + " qubits");
+    if (ntot - n > 16) return adj_fail(KSV_ERR_RANGE, "at most 65536 copies");
+    const int adef = prec == KSV_F64 ? ADJ_TILE_F64 : ADJ_TILE_F32;
+    if (athreads == 512 && atile != adef) return adj_fail(KSV_ERR_RANGE, "512-thread adjoint passes need the default tile");
+    if (!ops && nops) return adj_fail(KSV_ERR_TYPE, "null tape");
+    if (!mats && nmats) return adj_fail(KSV_ERR_TYPE, "null matrix buffer");
+    if (nrec && !params) return adj_fail(KSV_ERR_TYPE, "null parameter records");
+    if (nterms && (!xmask || !zmask || !coeff)) return adj_fail(KSV_ERR_TYPE, "null term buffer");
+    const uint64_t cmask = (1ull << n) - 1;
+    for (size_t t = 0; t < nterms; t++)
+        if ((xmask[t] | zmask[t]) & ~cmask) return adj_fail(KSV_ERR_RANGE, "Pauli term acts outside the copy's qubits");
+    std::map<int, int> per_op;
+    for (size_t r = 0; r < nrec; r++) {
+        const ksv_param& p = params[r];
+        if (p.op < 0 || (size_t)p.op >= nops) return adj_fail(KSV_ERR_TYPE, "parameter record names a missing op");
+        if (p.param < 0 || (size_t)p.param >= nparam) return adj_fail(KSV_ERR_TYPE, "parameter index out of range");
+        const ksv_op& o = ops[p.op];
+        const int dim = o.kind == KSV_SINGLE || o.kind == KSV_CONTROLLED ? 2 : (o.kind == KSV_TWO ? 4 : 0);
+        if (!dim) return adj_fail(KSV_ERR_TYPE, "only single, controlled and two-qubit gates can carry parameters");
+        if (p.dmat < 0 || (size_t)p.dmat + 2 * dim * dim > nmats) return adj_fail(KSV_ERR_TYPE, "bad derivative matrix offset");
+        if (++per_op[p.op] > 3) return adj_fail(KSV_ERR_TYPE, "more than three parameter records on one op");
+    }
+    std::unique_ptr<ksv_adj_batch> b(new ksv_adj_batch());
+    b->s = s;
+    b->n = n;
+    b->cb = ntot - n;
+    b->B = 1ull << b->cb;
+    b->prec = prec;
+    b->atile = atile;
+    b->athreads = athreads;
+    b->psi = psi;
+    b->psi2 = psi2;
+    b->st = st;
+    b->ops.assign(ops, ops + nops);
+    b->params.assign(params, params + nrec);
+    b->nmats = nmats;
+    b->nrec = nrec;
+    b->nparam = nparam;
+    b->nterms = nterms;
+    {
+        std::string err;
+        std::vector<ksv_op> iops;
+        std::vector<double> imats;
+        rc = invert_tape(ops, nops, mats, nmats, iops, imats, err);
+        if (!rc) rc = plan_adjoint(n, atile, iops.data(), iops.size(), imats.data(), imats.size(), nullptr, 0, b->fwd, err);
+        if (!rc) rc = plan_adjoint(n, atile, ops, nops, mats, nmats, params, nrec, b->bwd, err);
+        if (rc) return adj_fail(rc, err);
+        for (const AdjStep& sp : b->fwd.steps)
+            if (sp.big) return adj_fail(KSV_ERR_TYPE, "batched adjoint: unitaries of more than three qubits are not supported");
+        for (const AdjStep& sp : b->bwd.steps)
+            if (sp.big) return adj_fail(KSV_ERR_TYPE, "batched adjoint: unitaries of more than three qubits are not supported");
+        // an op is treated as diagonal only if it is diagonal in every copy
+        for (uint64_t t = 0; t < b->B; t++) {
+            const double* m = mats + t * nmats;
+            rc = invert_tape(ops, nops, m, nmats, iops, imats, err);
+            if (rc) return adj_fail(rc, err);
+            adjoint_plan_undiag(b->fwd, iops.data(), iops.size(), imats.data(), imats.size(), nullptr);
+            adjoint_plan_undiag(b->bwd, ops, nops, m, nmats, params);
+        }
+    }
+    b->fm = b->fwd.mats.size();
+    b->bm = b->bwd.mats.size();
+    const uint64_t tc = 1ull << (n - std::min(b->fwd.k, b->bwd.k));
+    const uint64_t gc = std::max<uint64_t>(1, std::min<uint64_t>(tc, std::max<uint64_t>(1, 2048 / b->B)));
+    const size_t nr = std::max<size_t>(nrec, 1);
+    if ((uint64_t)nr * gc * b->B > (1ull << 27)) return adj_fail(KSV_ERR_RANGE, "too many parameter records for the batch");
+    b->G = (int)(gc * b->B);
+    b->gd = (int)std::min<uint64_t>(((1ull << n) + 255) / 256, 64);
+    const size_t nt = std::max<size_t>(nterms, 1);
+    size_t off = 0;
+    b->off_faops = off; off = align256(off + sizeof(AOp) * std::max<size_t>(1, b->fwd.aops.size()));
+    b->off_baops = off; off = align256(off + sizeof(AOp) * std::max<size_t>(1, b->bwd.aops.size()));
+    b->off_slot = off; off = align256(off + 4 * std::max<size_t>(1, b->bwd.slot_rec.size()));
+    b->off_terms = off; off = align256(off + 4 * 8 * nt);
+    const size_t host_end = off;
+    b->off_mats = off; off = align256(off + 8 * std::max<size_t>(1, b->B * (b->fm + b->bm)));
+    b->off_part = off; off = align256(off + 8 * nr * (size_t)b->G);
+    b->off_sums = off; off = align256(off + 8 * nr * b->B);
+    b->off_epart = off; off = align256(off + 8 * (size_t)b->gd * b->B);
+    b->off_energy = off; off = align256(off + 8 * b->B);
+    if (hipMalloc(&b->blob, off) != hipSuccess) {
+        b->blob = nullptr;
+        return adj_fail(KSV_ERR_DEVICE, "out of device memory for the batched adjoint");
+    }
+    {
+        std::vector<char> h(host_end, 0);
+        if (!b->fwd.aops.empty()) std::memcpy(h.data() + b->off_faops, b->fwd.aops.data(), sizeof(AOp) * b->fwd.aops.size());
+        if (!b->bwd.aops.empty()) std::memcpy(h.data() + b->off_baops, b->bwd.aops.data(), sizeof(AOp) * b->bwd.aops.size());
+        if (!b->bwd.slot_rec.empty()) std::memcpy(h.data() + b->off_slot, b->bwd.slot_rec.data(), 4 * b->bwd.slot_rec.size());
+        uint64_t* x = reinterpret_cast<uint64_t*>(h.data() + b->off_terms);
+        double* cr = reinterpret_cast<double*>(x + 2 * nt);
+        for (size_t t = 0; t < nterms; t++) {
+            const int ny = __builtin_popcountll(xmask[t] & zmask[t]) & 3;
+            const double c = coeff[t];
+            const double re[4] = {c, 0, -c, 0}, im[4] = {0, c, 0, -c};   // c * i^ny
+            x[t] = xmask[t];
+            x[nt + t] = zmask[t];
+            cr[t] = re[ny];
+            cr[nt + t] = im[ny];
+        }
+        AHIP(hipMemcpyAsync(b->blob, h.data(), h.size(), hipMemcpyHostToDevice, st));
+        AHIP(hipStreamSynchronize(st));
+    }
+    *out = b.release();
+    return KSV_OK;
+}
+
+extern "C" int ksv_adjoint_batch_run(ksv_adj_batch* b, const double* mats, int want_grad, double* energy, double* grad) {
+    if (!b) return adj_fail(KSV_ERR_HANDLE, "null batch");
+    if (!mats && b->nmats) return adj_fail(KSV_ERR_TYPE, "null matrix buffer");
+    if (want_grad && b->nparam && !grad) return adj_fail(KSV_ERR_TYPE, "null gradient buffer");
+    int rc = batch_mats(b, mats);
+    if (rc) return rc;
+    char* dev = static_cast<char*>(b->blob);
+    if (!b->hmats.empty())
+        AHIP(hipMemcpyAsync(dev + b->off_mats, b->hmats.data(), 8 * b->hmats.size(), hipMemcpyHostToDevice, b->st));
+    const int g = want_grad ? 1 : 0;
+    if (!b->exec[g]) {
+        AHIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = batch_schedule(b, want_grad != 0);
+        hipGraph_t gr = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->st, &gr);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return adj_fail(KSV_ERR_DEVICE, std::string("batched adjoint capture: ") +
+                                                hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        b->graph[g] = gr;
+        AHIP(hipGraphInstantiate(&b->exec[g], gr, nullptr, nullptr, 0));
+    }
+    AHIP(hipGraphLaunch(b->exec[g], b->st));
+    std::vector<double> e(b->B), sums(want_grad ? b->nrec * b->B : 0);
+    AHIP(hipMemcpyAsync(e.data(), dev + b->off_energy, 8 * b->B, hipMemcpyDeviceToHost, b->st));
+    if (!sums.empty())
+        AHIP(hipMemcpyAsync(sums.data(), dev + b->off_sums, 8 * sums.size(), hipMemcpyDeviceToHost, b->st));
+    AHIP(hipStreamSynchronize(b->st));
+    if (energy) std::copy(e.begin(), e.end(), energy);
+    if (want_grad && b->nparam) {
+        std::fill(grad, grad + b->B * b->nparam, 0.0);
+        for (size_t r = 0; r < b->nrec; r++)
+            for (uint64_t t = 0; t < b->B; t++)
+                grad[t * b->nparam + b->params[r].param] += 2.0 * b->params[r].coeff * sums[r * b->B + t];
+    }
+    return KSV_OK;
+}
+
+extern "C" int ksv_adjoint_batch_destroy(ksv_adj_batch* b) {
+    if (!b) return adj_fail(KSV_ERR_HANDLE, "null batch");
+    delete b;
     return KSV_OK;
 }

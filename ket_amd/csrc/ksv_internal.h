@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <complex>
 #include <string>
 #include <vector>
@@ -320,7 +321,23 @@ struct AdjPlan {
     std::vector<int32_t> slot_rec;   // global parameter record of each pass slot
     std::vector<ksv_op> bops;        // wide inverse ops (KSV_UNITARY with conj-transposed matrix)
     std::vector<double> bmats;
+    std::vector<int32_t> aop_src;    // tape op of every AOp
+    std::vector<std::array<int32_t, 3>> aop_rec;   // its parameter records (-1: none)
 };
+
+// plan.mats rebuilt for another matrix array of the same tape (same layout, so
+// the plan's AOps stay valid); KSV_ERR_TYPE if an op the plan treats as
+// diagonal is not diagonal under the new matrices.
+int adjoint_plan_mats(const AdjPlan& plan, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                      const ksv_param* params, std::vector<double>& out, std::string& err);
+// Clear the diagonal flag of every AOp whose G^dag or dG is not diagonal under
+// `mats` (a plan made for one matrix array, used for several).
+void adjoint_plan_undiag(AdjPlan& plan, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
+                         const ksv_param* params);
+// The inverse tape G_N^dag, ..., G_1^dag (csrswap as a 3-qubit unitary): planned
+// with plan_adjoint, its passes apply G_1, ..., G_N, the forward evolution.
+int invert_tape(const ksv_op* ops, size_t nops, const double* mats, size_t nmats, std::vector<ksv_op>& iops,
+                std::vector<double>& imats, std::string& err);
 
 // adjplan.cpp: the inverse of `ops` split into tile passes of K qubits.
 // params[r].op names the tape op of record r (already validated).
