@@ -353,6 +353,45 @@ def workload_noisy(args):
             "most_likely": d.most, "entropy_bits": round(d.entropy, 4), "cpu_baseline": cpu}
 
 
+def workload_batch(args):
+    """Many parameter sets of one ansatz per evaluation: 64 sets of
+    efficientSU2(16, 8) (272 parameters, transverse-field Ising H) as copies of
+    one 22-qubit device state, energies and adjoint gradients of all sets per
+    graph replay, against the same sets evaluated one by one."""
+    import numpy as np
+    from ket_amd import workloads
+    from ket_amd.algorithms import efficientSU2, energy_and_gradient, energy_and_gradient_batch
+    import ket_amd
+    ket_amd.configure(prune=False)
+    n, reps, B = 16, 8, 64
+    ans = efficientSU2(n, reps)
+    H = workloads.tfim_hamiltonian(n)
+    npar = 2 * n * (reps + 1)
+    rng = np.random.default_rng(2026)
+    sets = [list(rng.uniform(-np.pi, np.pi, npar)) for _ in range(B)]
+    energy_and_gradient_batch(ans, H, sets)                 # plan, capture, warm-up
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        es, gs = energy_and_gradient_batch(ans, H, sets)
+    dt = (time.perf_counter() - t0) / args.steps
+    energy_and_gradient(ans, H, sets[0])
+    sample = 8
+    t0 = time.perf_counter()
+    errs = []
+    for i in range(sample):
+        e1, g1 = energy_and_gradient(ans, H, sets[i])
+        errs.append(max(abs(e1 - es[i]), float(np.max(np.abs(np.array(g1) - np.array(gs[i]))))))
+    t1 = (time.perf_counter() - t0) / sample
+    return {"metric": "parameter sets/s (energy + adjoint gradient), batched", "value": round(B / dt, 2),
+            "unit": "parameter sets/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{B} parameter sets of efficientSU2({n},{reps}) ({npar} parameters each), "
+                                   f"transverse-field Ising H, one {n + 6}-qubit state of {B} copies"},
+            "single_set_s": round(t1, 5), "speedup_vs_single_sets": round(t1 * B / dt, 1),
+            "max_abs_err_vs_single_sets": max(errs)}
+
+
 def workload_qaoa(args):
     """512 seeded Erdos-Renyi Max-Cut instances (n uniform in 14..22, edge
     probability 0.3, p = 4, gamma and beta uniform in [0, pi)) in one batch:
@@ -433,11 +472,12 @@ def main():
     ap.add_argument("--no-prune-line", action="store_true", help="skip the ket-pruning-mode timing of the same circuit")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default schedule")
     ap.add_argument("--shots", type=int, default=0, help="noisy workload: shots (default 10000)")
-    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy", "adjoint"],
+    ap.add_argument("--workload", default="random", choices=["random", "f32", "qft", "qaoa", "noisy", "adjoint", "batch"],
                     help="random: 30-qubit FP64 random circuit (default line); f32: complex64 32-qubit run "
                          "with accuracy against FP64; qft: qft(n) on a seeded basis state; qaoa: 512 batched "
                          "Max-Cut QAOA instances with adjoint gradients; noisy: run() with device noise, "
-                         "batched trajectories; adjoint: the 24-qubit adjoint-gradient section alone")
+                         "batched trajectories; adjoint: the 24-qubit adjoint-gradient section alone; batch: 64 "
+                         "parameter sets of one ansatz per evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -446,7 +486,7 @@ def main():
 
     if args.workload != "random":
         line = {"f32": workload_f32, "qft": workload_qft, "qaoa": workload_qaoa,
-                "noisy": workload_noisy, "adjoint": workload_adjoint}[args.workload](args)
+                "noisy": workload_noisy, "adjoint": workload_adjoint, "batch": workload_batch}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
 

@@ -311,24 +311,72 @@ def _packed_with_derivatives(flat, records):
     return ops, np.concatenate([np.asarray(mats, dtype=np.float64), np.asarray(extra, dtype=np.float64)]), rec
 
 
+def _scatter(mats_all, offs, res, m, d):
+    """res (m * B x d, op-major) into columns offs[o] .. of every row, interleaved (re, im)."""
+    B = mats_all.shape[0]
+    cols = (np.asarray(offs, dtype=np.int64)[:, None] + 2 * np.arange(d)[None, :]).reshape(-1)
+    r = res.reshape(m, B, d).transpose(1, 0, 2).reshape(B, m * d)
+    mats_all[:, cols] = r.real
+    mats_all[:, cols + 1] = r.imag
+
+
 def _batch_tapes(ansatz, params_list):
-    """Trace every parameter set; all must share one structure (ops, qubits,
-    parameter records).  Returns (n, ops, mats_all [B x nmats], records)."""
-    ops0 = rec0 = None
-    mats_all = []
-    n = None
-    for ps in params_list:
-        tr = trace(ansatz, [float(p) for p in ps])
-        if tr is None:
-            raise TypeError("a batched gradient needs an ansatz built from affine functions of its parameters")
-        flat, records, _, nq = tr
-        ops, mats, rec = _packed_with_derivatives(flat, records)
-        if ops0 is None:
-            ops0, rec0, n = ops, rec, nq
-        elif nq != n or not np.array_equal(ops, ops0) or not np.array_equal(rec, rec0) or len(mats) != len(mats_all[0]):
-            raise TypeError("every parameter set must give the ansatz the same structure")
-        mats_all.append(mats)
-    return n, ops0, np.stack(mats_all), rec0
+    """The tape of an ansatz for many parameter sets: traced once (the trace
+    fails when the structure depends on parameter values), then every
+    parametrised gate's matrix and derivative evaluated for all sets at once
+    from its affine angles, one vectorised call per gate kind.  Returns
+    (n, ops, mats_all [B x nmats], records)."""
+    from . import gatevec as V
+    theta = np.asarray(params_list, dtype=np.float64)
+    tr = trace(ansatz, [float(p) for p in params_list[0]])
+    if tr is None:
+        raise TypeError("a batched gradient needs an ansatz built from affine functions of its parameters")
+    flat, records, _, n = tr
+    ops, mats0, rec = _packed_with_derivatives(flat, records)
+    B = theta.shape[0]
+    mats_all = np.tile(mats0, (B, 1))
+    gates, derivs = {}, {}         # name -> [(offset, vals)]; (name, which) -> [(offset, vals)]
+    ri = 0
+    for i, op in enumerate(flat):
+        meta = op[-1] if op[0] in ("single", "two", "controlled") else None
+        if not meta or not isinstance(meta, tuple):
+            continue
+        name, ps = meta
+        if not any(isinstance(p, Sym) for p in ps):
+            continue
+        vals = []
+        for p in ps:
+            if isinstance(p, Sym):
+                v = np.full(B, float(p.const))
+                for k, cf in p.terms.items():
+                    v = v + cf * theta[:, k]
+                vals.append(v)
+            else:
+                vals.append(np.full(B, float(p)))
+        gates.setdefault(name, []).append((int(ops["mat"][i]), vals))
+        for which, p in enumerate(ps):
+            if not isinstance(p, Sym):
+                continue
+            for pi, coeff in p.terms.items():
+                if coeff == 0:
+                    continue
+                if int(rec[ri]["op"]) != i or int(rec[ri]["param"]) != pi:
+                    raise AssertionError("parameter records out of step with the trace")
+                derivs.setdefault((name, which), []).append((int(rec[ri]["dmat"]), vals))
+                ri += 1
+    if ri != len(rec):
+        raise AssertionError("parameter records out of step with the trace")
+    for name, items in gates.items():
+        fn, k = V.MATRIX[name]
+        args = [np.concatenate([v[j] for _, v in items]) for j in range(k)]
+        res = fn(*args)
+        _scatter(mats_all, [o for o, _ in items], res, len(items), res.shape[1])
+    for (name, which), items in derivs.items():
+        k = V.MATRIX[name][1]
+        args = [np.concatenate([v[j] for _, v in items]) for j in range(k)]
+        res = V.derivative(name, args, which)
+        _scatter(mats_all, [o for o, _ in items], res, len(items), res.shape[1])
+    return n, ops, mats_all, rec
 
 
 def _batch_run(ansatz, hamiltonian, params_list, grad):

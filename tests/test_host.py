@@ -441,3 +441,45 @@ def test_creg_value_is_javascripts_32_bit_integer():
     assert js([False] * 31 + [True]) == -(1 << 31)
     assert js([False] * 32 + [True]) == 1
     assert _creg_value({}, "missing") == 0
+
+
+def test_vectorised_gate_matrices_equal_the_scalar_constructors():
+    """gatevec (many angles at once) against gates.py entry by entry, matrices and
+    derivatives of every parametrised gate method."""
+    import math as m
+    from ket_amd import gatevec as V
+    from ket_amd.algorithms import _dmat
+    from ket_amd.circuit import _resolve_parametric
+    rng = np.random.default_rng(5)
+    B = 64
+    for name, (fn, k) in V.MATRIX.items():
+        vals = [rng.uniform(-7, 7, B) for _ in range(k)]
+        got = fn(*vals)
+        qs = (0,) if name in ("rx", "ry", "rz", "vz", "u1", "p", "u2", "u3", "gpi", "gpi2") else (0, 1)
+        for t in range(B):
+            want = _resolve_parametric(name, [float(v[t]) for v in vals], qs)
+            g = want[2] if want[0] == "single" else want[3]
+            assert np.max(np.abs(np.array(g) - got[t])) <= 4e-16, name
+        for which in range(k):
+            dv = V.derivative(name, vals, which)
+            for t in range(0, B, 7):
+                d = _dmat(name, [float(v[t]) for v in vals], which)
+                assert np.max(np.abs(np.array(d) - dv[t])) <= 4e-16, (name, which)
+
+
+def test_batched_tapes_equal_per_set_packing():
+    """The batched path traces an ansatz once and fills every set's matrices and
+    derivatives vectorised: identical ops and records, and every set's matrix array
+    equal to tracing and packing that set alone (efficientSU2 and a random ansatz
+    with shared parameters and affine angles over every parametrised gate kind)."""
+    from helpers import random_ansatz
+    from ket_amd.algorithms import _batch_tapes, _packed_with_derivatives, efficientSU2, trace
+    rng = np.random.default_rng(3)
+    for ans, npar in ((efficientSU2(6, 2), 36), (random_ansatz(7, 80, 5, 11, wide=False), 5)):
+        sets = [list(rng.uniform(-4, 4, npar)) for _ in range(6)]
+        n, ops, mats_all, rec = _batch_tapes(ans, sets)
+        for t, ps in enumerate(sets):
+            flat, records, _, n1 = trace(ans, ps)
+            o, m, r = _packed_with_derivatives(flat, records)
+            assert n1 == n and np.array_equal(o, ops) and np.array_equal(r, rec)
+            assert np.max(np.abs(m - mats_all[t])) <= 1e-15
