@@ -58,7 +58,30 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("libketsv link failed:\n" + r.stdout)
     os.replace(tmp, LIB)
+    build_napi(verbose)
     return LIB
+
+
+NAPI_SRC = os.path.join(HERE, "napi", "ketsv_napi.c")
+NAPI_OUT = os.path.join(LIBDIR, "ketsv.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi(verbose=False):
+    """The Node-API addon (plain C over libketsv.so, no HIP code), when Node's
+    headers are installed; returns its path or None."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return None
+    tmp = NAPI_OUT + ".tmp"
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-std=c11", "-Wall", "-I", NODE_INCLUDE, "-I", os.path.join(ROOT, "include"),
+           NAPI_SRC, "-L", LIBDIR, "-lketsv", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("ketsv.node build failed:\n" + r.stdout)
+    os.replace(tmp, NAPI_OUT)
+    return NAPI_OUT
 
 
 if __name__ == "__main__":

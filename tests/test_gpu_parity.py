@@ -787,3 +787,20 @@ def test_adjoint_without_terms_large_state():
     ket_amd.configure(prune=False)
     e, g = energy_and_gradient(ans, [], params)
     assert e == 0.0 and all(v == 0.0 for v in g)
+
+
+def test_node_api_addon_on_the_device():
+    """The Node-API addon drives the device from JavaScript: a Bell pair through
+    create / apply / probs / sample / expect (draws from makePrng written in JS),
+    a 10-qubit uniform state through amplitudes, error classes on the device path."""
+    import os
+    import shutil
+    import subprocess
+    from ket_amd.build import NAPI_OUT, NAPI_SRC
+    if not shutil.which("node"):
+        pytest.skip("node is not installed")
+    assert os.path.exists(NAPI_OUT), "ketsv.node is not built"
+    r = subprocess.run(["node", os.path.join(os.path.dirname(NAPI_SRC), "test_napi.js"), NAPI_OUT, "gpu"],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout

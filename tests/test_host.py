@@ -483,3 +483,21 @@ def test_batched_tapes_equal_per_set_packing():
             o, m, r = _packed_with_derivatives(flat, records)
             assert n1 == n and np.array_equal(o, ops) and np.array_equal(r, rec)
             assert np.max(np.abs(m - mats_all[t])) <= 1e-15
+
+
+def test_node_api_addon_error_classes():
+    """The Node-API addon (ket_amd/napi/ketsv_napi.c, what ket's TypeScript host
+    loads) under this image's Node: ket's error classes reach JavaScript
+    (RangeError for sizes and qubit indices, TypeError for malformed tapes and
+    arguments, Error without a device), the dry-run scheduler answers, and the
+    library's draw stream equals makePrng written in JavaScript."""
+    import shutil
+    import subprocess
+    from ket_amd.build import NAPI_OUT, NAPI_SRC
+    if not shutil.which("node") or not os.path.exists(NAPI_OUT):
+        pytest.skip("node or the built addon is not present")
+    if ket_amd.device_count() > 0:
+        pytest.skip("a GPU is present (the GPU suite runs the device half)")
+    r = subprocess.run(["node", os.path.join(os.path.dirname(NAPI_SRC), "test_napi.js"), NAPI_OUT],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
