@@ -23,6 +23,10 @@
  * zeroed, which is exactly the set of keys ket's accumulate() deletes.
  *
  * State layout: interleaved (re, im) doubles, index bit q = qubit q.
+ *
+ * Built twice (oracle/Makefile): plainly, and with -fopenmp as the
+ * all-core dense CPU baseline.  The OpenMP pragmas split only loops whose
+ * iterations own disjoint amplitudes, so both builds give the same bits.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -181,6 +185,7 @@ static inline uint64_t ins0(uint64_t p, int q) {   /* insert a 0 bit at position
 static void apply_single(int n, double* psi, int q, const double* m, int prune) {
     cx a = {m[0], m[1]}, b = {m[2], m[3]}, c = {m[4], m[5]}, d = {m[6], m[7]};
     uint64_t half = 1ull << (n - 1), mk = 1ull << q;
+#pragma omp parallel for schedule(static)
     for (uint64_t p = 0; p < half; p++) {
         uint64_t i0 = ins0(p, q), i1 = i0 | mk;
         cx x0 = ld(psi, i0), x1 = ld(psi, i1);
@@ -194,6 +199,7 @@ static void apply_single(int n, double* psi, int q, const double* m, int prune) 
 static void apply_controlled(int n, double* psi, int ctl, int q, const double* m, int prune) {
     cx a = {m[0], m[1]}, b = {m[2], m[3]}, c = {m[4], m[5]}, d = {m[6], m[7]};
     uint64_t half = 1ull << (n - 1), mk = 1ull << q, cm = 1ull << ctl;
+#pragma omp parallel for schedule(static)
     for (uint64_t p = 0; p < half; p++) {
         uint64_t i0 = ins0(p, q), i1 = i0 | mk;
         if (!(i0 & cm)) continue;
@@ -209,33 +215,37 @@ static void apply_k(int n, double* psi, int k, const int* qs, const double* m, i
     uint64_t dim = 1ull << k, all = 0;
     uint64_t masks[KSV_MAX_OPERANDS];
     for (int b = 0; b < k; b++) { masks[b] = 1ull << qs[b]; all |= masks[b]; }
-    uint64_t* base = (uint64_t*)malloc(dim * sizeof(uint64_t));
-    cx* amp = (cx*)malloc(dim * sizeof(cx));
-    cx* out = (cx*)malloc(dim * sizeof(cx));
     uint64_t N = 1ull << n;
-    for (uint64_t ctx = 0; ctx < N; ctx++) {
-        if (ctx & all) continue;
-        for (uint64_t i = 0; i < dim; i++) {
-            uint64_t g = ctx;
-            for (int b = 0; b < k; b++)
-                if ((i >> (k - 1 - b)) & 1) g |= masks[b];
-            base[i] = g;
-            amp[i] = ld(psi, g);
-        }
-        for (uint64_t r = 0; r < dim; r++) {
-            cx acc = CZERO;
-            for (uint64_t c = 0; c < dim; c++) {
-                const double* e = m + 2 * (r * dim + c);
-                cx me = {e[0], e[1]};
-                acc = cadd(acc, cmul(me, amp[c]));
+#pragma omp parallel
+    {
+        uint64_t* base = (uint64_t*)malloc(dim * sizeof(uint64_t));
+        cx* amp = (cx*)malloc(dim * sizeof(cx));
+        cx* out = (cx*)malloc(dim * sizeof(cx));
+#pragma omp for schedule(static)
+        for (uint64_t ctx = 0; ctx < N; ctx++) {
+            if (ctx & all) continue;
+            for (uint64_t i = 0; i < dim; i++) {
+                uint64_t g = ctx;
+                for (int b = 0; b < k; b++)
+                    if ((i >> (k - 1 - b)) & 1) g |= masks[b];
+                base[i] = g;
+                amp[i] = ld(psi, g);
             }
-            out[r] = acc;
+            for (uint64_t r = 0; r < dim; r++) {
+                cx acc = CZERO;
+                for (uint64_t c = 0; c < dim; c++) {
+                    const double* e = m + 2 * (r * dim + c);
+                    cx me = {e[0], e[1]};
+                    acc = cadd(acc, cmul(me, amp[c]));
+                }
+                out[r] = acc;
+            }
+            for (uint64_t r = 0; r < dim; r++) st(psi, base[r], keep(out[r], prune));
         }
-        for (uint64_t r = 0; r < dim; r++) st(psi, base[r], keep(out[r], prune));
+        free(base);
+        free(amp);
+        free(out);
     }
-    free(base);
-    free(amp);
-    free(out);
 }
 
 /* applyCsrSwap, src/statevector.ts:177-207 */
@@ -243,6 +253,7 @@ static void apply_csrswap(int n, double* psi, int ctl, int a, int b, int prune) 
     const double sq2 = 1 / sqrt(2.0);
     cx r2 = {sq2, 0}, i2 = {0, sq2};
     uint64_t N = 1ull << n, cm = 1ull << ctl, am = 1ull << a, bm = 1ull << b;
+#pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < N; i++) {
         if (!(i & cm) || (i & am) || (i & bm)) continue;
         uint64_t i01 = i | bm, i10 = i | am;
@@ -271,6 +282,7 @@ static int apply_op(int n, double* psi, const ksv_op* op, const double* mats, in
     case KSV_CNOT: {                                   /* src/statevector.ts:70-80 */
         if (q[0] == q[1]) return KSV_ERR_TYPE;
         uint64_t cm = 1ull << q[0], tm = 1ull << q[1];
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; i++)
             if ((i & cm) && !(i & tm)) swp(psi, i, i | tm);
         return 0;
@@ -278,6 +290,7 @@ static int apply_op(int n, double* psi, const ksv_op* op, const double* mats, in
     case KSV_SWAP: {                                   /* src/statevector.ts:85-102 */
         if (q[0] == q[1]) return 0;
         uint64_t am = 1ull << q[0], bm = 1ull << q[1];
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; i++)
             if ((i & am) && !(i & bm)) swp(psi, i, i ^ am ^ bm);
         return 0;
@@ -293,6 +306,7 @@ static int apply_op(int n, double* psi, const ksv_op* op, const double* mats, in
     case KSV_TOFFOLI: {                                /* src/statevector.ts:138-147 */
         if (q[2] == q[0] || q[2] == q[1]) return KSV_ERR_TYPE;
         uint64_t c1 = 1ull << q[0], c2 = 1ull << q[1], tm = 1ull << q[2];
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; i++)
             if ((i & c1) && (i & c2) && !(i & tm)) swp(psi, i, i | tm);
         return 0;
@@ -301,6 +315,7 @@ static int apply_op(int n, double* psi, const ksv_op* op, const double* mats, in
         if (q[1] == q[2]) return 0;
         if (q[0] == q[1] || q[0] == q[2]) return KSV_ERR_TYPE;
         uint64_t cm = 1ull << q[0], am = 1ull << q[1], bm = 1ull << q[2];
+#pragma omp parallel for schedule(static)
         for (uint64_t i = 0; i < N; i++)
             if ((i & cm) && (i & am) && !(i & bm)) swp(psi, i, i ^ am ^ bm);
         return 0;

@@ -27,12 +27,40 @@ def build():
 _lib = None
 
 
+OMP_PATH = os.path.join(HERE, "libketsv_oracle_omp.so")
+_omp = None
+
+
+def lib_omp():
+    """The same oracle with its apply loops split over all host cores (OpenMP);
+    bit-identical results, used as the all-core dense CPU baseline."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(OMP_PATH):
+            build()
+        _omp = _configure(ctypes.CDLL(OMP_PATH))
+    return _omp
+
+
+def apply_tape_omp(n, psi, ops, mats, prune=True):
+    m = mats if len(mats) else np.zeros(1)
+    rc = lib_omp().orc_apply(n, _p(psi), _p(ops), len(ops), _p(m), 1 if prune else 0)
+    if rc:
+        raise RuntimeError(f"oracle rejected the tape (code {rc})")
+    return psi
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        _lib = _configure(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _configure(L):
+    if True:
         L.orc_gate_matrix.argtypes = [c_char_p, POINTER(c_double), POINTER(c_double)]
         L.orc_gate_matrix.restype = c_int
         L.orc_apply.argtypes = [c_int, c_void_p, c_void_p, c_size_t, c_void_p, c_int]
@@ -61,8 +89,7 @@ def lib():
         L.orc_op_struct_size.restype = c_size_t
         assert L.orc_gate_struct_size() == ctypes.sizeof(OrcGate)
         assert L.orc_op_struct_size() == 64
-        _lib = L
-    return _lib
+    return L
 
 
 def _p(a):
