@@ -61,6 +61,93 @@ def cpu_baseline(n, ops, mats, budget_s):
                             "sample": f"first {k1} gates, scalar build, {dt1:.1f} s"}}
 
 
+# statevector() wall times read off ket's published charts (benchmark/charts/
+# uniform.svg, qft.svg: GitHub Actions ubuntu-latest, 2 cores, Node 22, one
+# timed run after warm-up), in ms
+KET_CHART_MS = {"uniform_20": 1940.0, "qft_20": 6100.0, "uniform_16": 50.4, "qft_16": 162.0}
+
+
+def ket_algorithm_baseline(scaling=False):
+    """ket's own CPU algorithm (the sparse Map<bigint> statevector with per-gate
+    pruning, restated in JavaScript and run under Node on one host core, as ket
+    runs) timed on the same inputs as the GPU path in its default ket-pruning
+    mode:
+      - the 12-qubit 20-layer random circuit (830 gates): exactProbs plus 10 000
+        seeded shots, one warm-up and the median of five, against the GPU path's
+        Circuit.exactProbs() + Circuit.run(10000) end to end;
+      - H on 20 qubits and qft(20): one timed statevector() each, beside ket's
+        published chart points for the same circuits and the GPU's simulate()
+        (tape packing, planning and the passes, until a read of the state);
+      - the 40-layer random circuit at 16 qubits (and 20, 22 with `scaling`):
+        gate-updates/s once the state has filled in (the first layer untimed).
+    The JavaScript restatement's amplitudes and seeded counts equal the C
+    oracle's bit for bit (tested on the CPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ket_sparse as K
+    if K.node() is None:
+        return {"skipped": "node is not installed on this host"}
+    import ket_amd
+    from ket_amd import workloads
+    from ket_amd.algorithms import qft
+
+    old = ket_amd.configure(prune=True, precision="f64")
+    out = {"kind": "port", "cores": 1, "algorithm": "ket's sparse Map<bigint> statevector, per-gate 1e-14 prune, "
+                                                   "JavaScript under Node, one thread",
+           "gpu_mode": "ket pruning (the library default)"}
+    try:
+        n, seed, shots = 12, 42, 10000
+        c = workloads.random_layers(n, 20, seed=seed)
+        ops, mats = c.tape()
+        log("bench: ket algorithm (Node) on the 12-qubit circuit")
+        r = K.run(n, ops, mats, task="probs_run", shots=shots, seed=seed, warmup=1, runs=5)
+        out["node"] = r["node"]
+        ts = []
+        for i in range(6):
+            t0 = time.perf_counter()
+            c.exactProbs()
+            c.run(shots=shots, seed=seed)
+            if i:
+                ts.append(time.perf_counter() - t0)
+        gpu_ms = sorted(ts)[len(ts) // 2] * 1e3
+        out["random_12q_830g_probs_10k_shots"] = {
+            "cpu_median_ms": round(r["median_ms"], 1), "gpu_median_ms": round(gpu_ms, 2),
+            "speedup": round(r["median_ms"] / gpu_ms, 1), "gates": len(ops),
+            "cpu_gate_updates_per_s": round(len(ops) / (r["median_ms"] * 1e-3), 1)}
+        for name, circ in (("uniform_20", _uniform(20)), ("qft_20", qft(20))):
+            o, m = circ.tape()
+            log(f"bench: ket algorithm (Node) on {name}")
+            r = K.run(20, o, m, warmup=0, runs=1)
+            circ.simulate().free()
+            t0 = time.perf_counter()
+            st = circ.simulate()
+            st.amplitude(0)     # a blocking read: the state is complete
+            g = (time.perf_counter() - t0) * 1e3
+            st.free()
+            out[name] = {"cpu_ms": round(r["median_ms"], 1), "ket_published_ms": KET_CHART_MS[name],
+                         "cpu_over_published": round(r["median_ms"] / KET_CHART_MS[name], 2),
+                         "gpu_simulate_ms": round(g, 3), "speedup": round(r["median_ms"] / g, 1), "gates": len(o)}
+        rates = {}
+        for nq in ((16, 20, 22) if scaling else (16,)):
+            circ = workloads.random_layers(nq, 40, seed=42)
+            o, m = circ.tape()
+            log(f"bench: ket algorithm (Node) rate at {nq} qubits")
+            r = K.run(nq, o, m, task="rate", skip=3 * nq + nq // 2, budget_ms=4000.0)
+            rates[str(nq)] = {"gate_updates_per_s": round(r["gates"] / (r["ms"] * 1e-3), 3), "gates_timed": r["gates"],
+                              "ms": round(r["ms"], 1), "map_entries": r["entries"]}
+        out["random_layers_rate"] = rates
+    finally:
+        ket_amd.configure(**old)
+    return out
+
+
+def _uniform(n):
+    from ket_amd.circuit import Circuit
+    c = Circuit(n)
+    for q in range(n):
+        c = c.h(q)
+    return c
+
+
 def traffic_from_profiles(kernel_tag):
     """Measured HBM bytes per launch of `kernel_tag` from the committed counter summary, or None."""
     import re
@@ -485,6 +572,9 @@ def main():
     ap.add_argument("--rtile", type=int, default=0, help="register-round tile qubits (11..13)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ket-js", action="store_true", help="skip the timing of ket's own algorithm under Node")
+    ap.add_argument("--ket-js-scaling", action="store_true",
+                    help="add the 20- and 22-qubit points of ket's algorithm (about three minutes more)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 24-qubit adjoint vs parameter-shift section")
     ap.add_argument("--no-prune-line", action="store_true", help="skip the ket-pruning-mode timing of the same circuit")
     ap.add_argument("--no-autotune", action="store_true", help="use the library's default schedule")
@@ -616,6 +706,8 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(n, ops, mats, args.cpu_seconds)
+    if not args.no_cpu and not args.no_ket_js:
+        line["ket_algorithm_cpu"] = ket_algorithm_baseline(args.ket_js_scaling)
     print(json.dumps(line), flush=True)
 
 
