@@ -283,6 +283,26 @@ int ksv_qaoa_batch_info(const ksv_batch* b, uint64_t* total_amplitudes, uint64_t
 int ksv_qaoa_batch_destroy(ksv_batch* b);
 int ksv_batch_last_error(char* buf, size_t len);
 
+/* ---- density matrices (circuit.dm()) ------------------------------------------------ *
+ * Replaces runDM (src/density.ts:475-588) over ket's sparse map
+ * (src/density.ts:5-35).  rho lives dense on the device, entry key
+ * (row << n) | col, 1 <= n <= 15, starting as |0..0><0..0|.  Every channel
+ * follows ket's arithmetic (U rho U^+ as rows by U then columns by U^+, every
+ * product and sum rounded on its own, entries with |v|^2 < 1e-14 removed) and
+ * every entry carries its place in ket's Map order, so depolarize2's
+ * order-dependent partial-sum pruning (dmAcc, src/density.ts:29-35) comes out
+ * as ket's.  ksv_dm_apply runs a pure tape (unitaries up to 6 qubits) with
+ * depolarize1(p1) after every single-qubit gate and depolarize2(p2) after every
+ * cnot / swap / two-qubit / controlled gate, as runDM does.  ksv_dm_read
+ * writes rho (2 * 4^n doubles, interleaved re/im; NULL to skip), the keys of
+ * the present entries in Map order (4^n capacity; NULL to skip) and their
+ * count.                                                                     */
+typedef struct ksv_dm ksv_dm;
+int ksv_dm_create(int n, ksv_dm** out);
+int ksv_dm_apply(ksv_dm* d, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, double p1, double p2);
+int ksv_dm_read(ksv_dm* d, double* rho, uint64_t* order, uint64_t* count);
+int ksv_dm_destroy(ksv_dm* d);
+
 /* ---- instrumentation -------------------------------------------------------------- */
 /* ---- batched trajectories -------------------------------------------- *
  * Replaces run()'s per-shot loop (src/circuit.ts:3455-3481) over applyOps

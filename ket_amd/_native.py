@@ -94,6 +94,10 @@ SIGNATURES = [
     ("ksv_qaoa_batch_info", c_int, [_P, _U64P, _U64P]),
     ("ksv_qaoa_batch_destroy", c_int, [_P]),
     ("ksv_batch_last_error", c_int, [ctypes.c_char_p, c_size_t]),
+    ("ksv_dm_create", c_int, [c_int, POINTER(c_void_p)]),
+    ("ksv_dm_apply", c_int, [_P, _P, c_size_t, _P, c_size_t, c_double, c_double]),
+    ("ksv_dm_read", c_int, [_P, _P, _P, _U64P]),
+    ("ksv_dm_destroy", c_int, [_P]),
     ("ksv_get_stats", c_int, [_P, POINTER(KsvStats)]),
     ("ksv_reset_stats", c_int, [_P]),
 ]
@@ -539,3 +543,41 @@ def xorshift_fill(state, count):
     out = np.empty(count, dtype=np.float64)
     check(lib().ksv_xorshift32(ptr(state), count, ptr(out)))
     return out
+
+
+class DensityState:
+    """A density matrix on the device in ket's Map semantics (ksv_dm_*): rho as
+    (row << n) | col plus every entry's place in ket's Map order."""
+
+    def __init__(self, n):
+        self.n = n
+        h = c_void_p()
+        check(lib().ksv_dm_create(int(n), ctypes.byref(h)))
+        self._h = h
+
+    def apply(self, tape, p1=0.0, p2=0.0):
+        ops, mats = tape
+        ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+        mats = np.ascontiguousarray(mats, dtype=np.float64)
+        check(lib().ksv_dm_apply(self._h, ptr(ops), len(ops), ptr(mats) if len(mats) else None, len(mats),
+                                 float(p1), float(p2)))
+
+    def read(self):
+        """(rho as a dense 2^n x 2^n complex128 array, keys in Map order)."""
+        dim = 1 << self.n
+        rho = np.empty((dim, dim), dtype=np.complex128)
+        order = np.empty(dim * dim, dtype=np.uint64)
+        cnt = c_uint64(0)
+        check(lib().ksv_dm_read(self._h, ptr(rho), ptr(order), ctypes.byref(cnt)))
+        return rho, order[:cnt.value].copy()
+
+    def free(self):
+        if self._h:
+            lib().ksv_dm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libketsv.so")
-SOURCES = ["capi.hip", "k_pass.hip", "k_rpass.hip", "k_bpass.hip", "k_reduce.hip", "k_adjoint.hip", "k_qaoa.hip", "k_shots.hip", "schedule.cpp", "adjplan.cpp"]
+SOURCES = ["capi.hip", "k_pass.hip", "k_rpass.hip", "k_bpass.hip", "k_reduce.hip", "k_adjoint.hip", "k_qaoa.hip", "k_shots.hip", "k_dm.hip", "schedule.cpp", "adjplan.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -727,7 +727,7 @@ class Circuit:
                 raise TypeError(f"Unknown device profile '{noise}'. Known: {', '.join(DM_DEVICE_NOISE)}")
         else:
             params = noise
-        return run_dm(self.qubits, _flatten(self._ops()), params, OPTIONS["precision"], OPTIONS["prune"])
+        return run_dm(self.qubits, _flatten(self._ops()), params)
 
     def amplitude(self, bitstring):
         if not isinstance(bitstring, str) or len(bitstring) != self.qubits or any(ch not in "01" for ch in bitstring) or not bitstring:

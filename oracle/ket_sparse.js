@@ -332,4 +332,6 @@ function main() {
   process.stdout.write(JSON.stringify(out) + '\n')
 }
 
-main()
+if (require.main === module) main()
+
+module.exports = { KIND, readTape }

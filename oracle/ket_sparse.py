@@ -50,3 +50,30 @@ def run(n, ops, mats, task="statevector", shots=0, seed=1, warmup=1, runs=5, dum
         if dump:
             out["amplitudes"] = np.fromfile(pd, dtype=np.complex128)
         return out
+
+
+def run_dm(n, ops, mats, p1=0.0, p2=0.0, timeout=3600):
+    """ket's runDM restated over a Map (oracle/ket_dm.js) on the tape.
+
+    Returns (rho, order, info): the dense 2^n x 2^n rho (row, col), the keys
+    (row << n) | col of the final Map in its insertion order, and the job's
+    JSON result (entries, dropped partial sums and their total magnitude, ms)."""
+    exe = node()
+    if exe is None:
+        raise RuntimeError("node is not on PATH")
+    with tempfile.TemporaryDirectory(prefix="ket_dm_") as tmp:
+        po, pm, pj, pr, pk = (os.path.join(tmp, f) for f in ("ops.bin", "mats.bin", "job.json", "rho.bin",
+                                                              "order.bin"))
+        np.ascontiguousarray(ops).tofile(po)
+        np.ascontiguousarray(mats, dtype=np.float64).tofile(pm)
+        with open(pj, "w") as f:
+            json.dump({"n": int(n), "ops": po, "mats": pm, "p1": float(p1), "p2": float(p2), "dump_rho": pr,
+                       "dump_order": pk}, f)
+        proc = subprocess.run([exe, "--max-old-space-size=16384", os.path.join(HERE, "ket_dm.js"), pj],
+                              capture_output=True, text=True, timeout=timeout)
+        if proc.returncode != 0:
+            raise RuntimeError(f"ket_dm.js failed ({proc.returncode}): {proc.stderr.strip()[-2000:]}")
+        info = json.loads(proc.stdout.strip().splitlines()[-1])
+        rho = np.fromfile(pr, dtype=np.complex128).reshape(1 << n, 1 << n)
+        order = np.fromfile(pk, dtype=np.uint64)
+        return rho, order, info

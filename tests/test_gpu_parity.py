@@ -479,7 +479,7 @@ def test_circuit_matrix_equals_oracle_columns(n):
 
 
 def test_dm_known_answers():
-    """ket's dm() cases (src/circuit.test.ts:5017-5250): rho as a 2n-qubit device state."""
+    """ket's dm() cases (src/circuit.test.ts:5017-5250) on the device density matrix."""
     sq2 = 1 / math.sqrt(2)
     for c in (Circuit(1), Circuit(1).x(0), Circuit(1).h(0), Circuit(2).h(0).cnot(0, 1)):
         assert abs(c.dm().purity() - 1) < 1e-10
@@ -520,15 +520,19 @@ def test_dm_known_answers():
                                           (5, {"p1": 0.02, "p2": 0.03}, 4)])
 def test_dm_equals_oracle(n, noise, seed):
     """rho of seeded circuits over every op kind (csrswap, 2- and 4-qubit unitaries,
-    toffoli, cswap) with depolarising noise, against the oracle's runDM."""
+    toffoli, cswap) with depolarising noise, against the oracle's dense runDM, which
+    forms depolarize2's sums whole: 1e-12 without two-qubit noise, else within 2e-7
+    (ket drops running sums below 1e-7; the bit-exact comparison with a Map-order
+    restatement of ket is in test_gpu_dm.py)."""
     from ket_amd.density import DM_DEVICE_NOISE
     calls = random_calls(n, 30, seed=seed, with_unitary=True)
     calls.append(("csrswap", (0, 1, 2)))
     d = build(n, calls).dm(noise=noise)
     params = DM_DEVICE_NOISE[noise] if isinstance(noise, str) else noise
     ref = O.run_dm(n, calls, params)
-    assert np.max(np.abs(d._rho - ref)) < TOL
-    assert abs(d.purity() - float(np.sum(np.abs(ref) ** 2))) < 1e-12
+    tol = 2e-7 if (params or {}).get("p2") else TOL
+    assert np.max(np.abs(d._rho - ref)) < tol
+    assert abs(d.purity() - float(np.sum(np.abs(ref) ** 2))) < (1e-6 if tol > TOL else 1e-12)
 
 
 @pytest.mark.parametrize("precision,tol", [("f64", TOL), ("f32", 1e-6)])

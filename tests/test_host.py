@@ -314,29 +314,6 @@ def test_prng_long_fill_is_the_serial_stream():
     assert b() == c()
 
 
-@pytest.mark.parametrize("noise,seed", [(None, 1), ({"p1": 0.05, "p2": 0.1}, 2), ({"p1": 0.02, "p2": 0.03}, 3)])
-def test_dm_tape_on_the_oracle(noise, seed):
-    """The 2n-qubit tape dm() sends to the device (U (x) conj(U) per gate on row and
-    column qubits, depolarising maps, ket's csrswap block form), run through the
-    oracle's tape engine, equals the oracle's runDM restatement."""
-    import ketsv_oracle as O
-    from helpers import build, random_calls
-    from ket_amd.circuit import _flatten
-    from ket_amd.density import dm_ops
-    from ket_amd.tape import pack
-    n = 3
-    calls = random_calls(n, 25, seed=seed, with_unitary=True) + [("csrswap", (0, 1, 2))]
-    c = build(n, calls)
-    p1 = (noise or {}).get("p1", 0)
-    p2 = (noise or {}).get("p2", 0)
-    ops, mats = pack(dm_ops(n, _flatten(c._ops()), p1, p2))
-    vec = O.zero_state(2 * n)
-    O.apply_tape(2 * n, vec, np.ascontiguousarray(ops, dtype=OP_DTYPE), mats, prune=True)
-    rho = vec.reshape(1 << n, 1 << n)
-    ref = O.run_dm(n, calls, noise)
-    assert np.max(np.abs(rho - ref)) < 1e-12
-
-
 class _OracleCopies:
     """CPU stand-in for a batched device state (the ksv_shots_* methods), every copy
     simulated by the oracle; lets the host driver's event logic run without a GPU."""
