@@ -18,7 +18,8 @@
 // per lane from the host.  A block whose ops are all diagonal is instead one
 // phase per row: its fragments hold the diagonal and each lane multiplies
 // the amplitudes it reads in place.  Columns differ between waves, so blocks need a
-// barrier only between them.  The LDS address of tile index x is x with its
+// barrier only between them; within a block a wave reads its next slice from LDS
+// while the current one runs on MFMA.  The LDS address of tile index x is x with its
 // low four bits XORed by a per-pass word of its higher bits, chosen so that
 // each 16-lane group hits 16 distinct 16-byte bank groups.  HBM sees one
 // coalesced read and one coalesced write of the tile per pass.
@@ -63,147 +64,184 @@ __device__ __forceinline__ uint32_t swz_part(uint32_t x, uint64_t w, int lo, int
     return g;
 }
 
+// Tile index x = tid + i * NT of a workgroup: the thread part and the uniform
+// part of its physical offset and of its swizzled LDS address.
+template <int K, int NT>
+struct TileMap {
+    static constexpr int LNT = NT == 512 ? 9 : 8;
+    static constexpr int PER = (1 << K) / NT;   // amplitudes per thread at load / store
+    uint64_t g_t;
+    uint32_t l_t;
+    __device__ __forceinline__ TileMap(const BPassArgs& a, uint32_t tid) {
+        g_t = 0;
+#pragma unroll
+        for (int b = 0; b < LNT; b++)
+            g_t |= (uint64_t)((tid >> b) & 1) << byte_at(b < 8 ? a.tq_lo : a.tq_hi, b & 7);
+        l_t = tid ^ swz_part(tid, a.swz, 4, LNT);
+    }
+    static __device__ __forceinline__ uint64_t base(const BPassArgs& a, uint64_t tile) {
+        uint64_t b = tile;
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < K) b = ins0_64(b, byte_at(i < 8 ? a.tq_lo : a.tq_hi, i & 7));
+        return b;
+    }
+    __device__ __forceinline__ uint64_t gaddr(const BPassArgs& a, uint64_t base, int i) const {
+        const uint32_t xh = (uint32_t)i * NT;
+        uint64_t g_h = 0;
+#pragma unroll
+        for (int b = LNT; b < K; b++)
+            if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
+        return base | g_t | g_h;
+    }
+    __device__ __forceinline__ uint32_t laddr(const BPassArgs& a, int i) const {
+        const uint32_t xh = (uint32_t)i * NT;
+        return l_t ^ xh ^ swz_part(xh, a.swz, LNT, K);
+    }
+};
+
+// One block of a pass on the tile in LDS (record w, fragments af).
 template <typename T, int BQ, int K, int NT>
-__global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
+__device__ __forceinline__ void bpass_block(typename BVec2<T>::type* lds, const KSV_CONST uint64_t* w,
+                                            const T (&af)[3 * ((1 << BQ) / 16) * ((1 << BQ) / 4)], uint32_t lane,
+                                            uint32_t wave) {
     using V = typename BVec2<T>::type;
     using M = Mfma<T>;
     using acc_t = typename M::acc;
-    constexpr int NW = NT / 64;         // waves
-    constexpr int LNT = NT == 512 ? 9 : 8;
-    constexpr int R = 1 << BQ;          // complex rows
-    constexpr int MB = R / 16;          // output 16-row tiles of each product
-    constexpr int KS = R / 4;           // k-steps
-    constexpr int NF = 3 * MB * KS;     // A fragments: Ur, Ui, Ur + Ui
-    constexpr int NC = 1 << (K - BQ);   // columns
-    constexpr int NBW = NC / 16 / NW;   // 16-column slices per wave
-    constexpr int PER = (1 << K) / NT;  // amplitudes per thread at load / store
+    constexpr int NW = NT / 64;
+    constexpr int R = 1 << BQ;
+    constexpr int MB = R / 16;
+    constexpr int KS = R / 4;
+    constexpr int NC = 1 << (K - BQ);
+    constexpr int NBW = NC / 16 / NW;
     static_assert(NBW >= 1, "tile too small for the block width");
-    __shared__ V lds[1 << K];
+    uint16_t ar[8], ac[16];
+    {
+        const uint64_t r0 = w[0], r1 = w[1], r2 = w[2], r3 = w[3], r4 = w[4], r5 = w[5];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ar[i] = (uint16_t)(r0 >> (16 * i));
+            ar[4 + i] = (uint16_t)(r1 >> (16 * i));
+            ac[i] = (uint16_t)(r2 >> (16 * i));
+            ac[4 + i] = (uint16_t)(r3 >> (16 * i));
+            ac[8 + i] = (uint16_t)(r4 >> (16 * i));
+            ac[12 + i] = (uint16_t)(r5 >> (16 * i));
+        }
+    }
+    const bool diag = (w[7] & BREC_DIAG) != 0;     // a phase per row (fragments hold the diagonal)
+    const uint32_t rl = lane >> 4, cl = lane & 15;
+    // read rows r = 4t + rl: bits 0,1 from rl, the rest from t
+    const uint32_t rd_l = ((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u);
+    // write rows r = 16 h + (4 v + rl) (f64) or 16 h + (4 rl + v) (f32)
+    constexpr bool LOW = M::lane_is_row_low;
+    const uint32_t wr_l = LOW ? (((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u))
+                              : (((rl & 1) ? ar[2] : 0u) ^ ((rl & 2) ? ar[3] : 0u));
+    const uint32_t col_l = ((cl & 1) ? ac[0] : 0u) ^ ((cl & 2) ? ac[1] : 0u) ^ ((cl & 4) ? ac[2] : 0u) ^
+                           ((cl & 8) ? ac[3] : 0u);
+    auto col_of = [&](int m) {
+        const uint32_t nb = wave + NW * m;
+        uint32_t col = col_l;
+#pragma unroll
+        for (int b = 0; b < K - BQ - 4; b++)
+            if ((nb >> b) & 1) col ^= ac[4 + b];
+        return col;
+    };
+    uint32_t ra[R / 4];
+#pragma unroll
+    for (int t = 0; t < R / 4; t++) {
+        ra[t] = rd_l;
+#pragma unroll
+        for (int b = 0; b < BQ - 2; b++)
+            if ((t >> b) & 1) ra[t] ^= ar[2 + b];
+    }
+    if (diag) {
+#pragma unroll
+        for (int m = 0; m < NBW; m++) {
+            const uint32_t col = col_of(m);
+#pragma unroll
+            for (int t = 0; t < R / 4; t++) {
+                const V x = lds[ra[t] ^ col];
+                const T pr = af[2 * t], pi = af[2 * t + 1];
+                V y;
+                y.x = x.x * pr - x.y * pi;
+                y.y = x.x * pi + x.y * pr;
+                lds[ra[t] ^ col] = y;
+            }
+        }
+        return;
+    }
+    // the slices own disjoint columns: slice m + 1 is read from LDS while
+    // slice m runs on MFMA
+    V xn[R / 4];
+#pragma unroll
+    for (int t = 0; t < R / 4; t++) xn[t] = lds[ra[t] ^ col_of(0)];
+#pragma unroll
+    for (int m = 0; m < NBW; m++) {
+        const uint32_t col = col_of(m);
+        V x[R / 4];
+#pragma unroll
+        for (int t = 0; t < R / 4; t++) x[t] = xn[t];
+        if (m + 1 < NBW) {
+            const uint32_t cn = col_of(m + 1);
+#pragma unroll
+            for (int t = 0; t < R / 4; t++) xn[t] = lds[ra[t] ^ cn];
+        }
+        acc_t t1[MB], t2[MB], t3[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++) t1[mb] = t2[mb] = t3[mb] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < KS; kk++) {
+            const T xr = x[kk].x, xi = x[kk].y, xs = xr + xi;
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                t1[mb] = M::step(af[mb * KS + kk], xr, t1[mb]);
+                t2[mb] = M::step(af[(MB + mb) * KS + kk], xi, t2[mb]);
+                t3[mb] = M::step(af[(2 * MB + mb) * KS + kk], xs, t3[mb]);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < MB; h++) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                uint32_t wa = wr_l ^ (LOW ? (((v & 1) ? ar[2] : 0u) ^ ((v & 2) ? ar[3] : 0u))
+                                          : (((v & 1) ? ar[0] : 0u) ^ ((v & 2) ? ar[1] : 0u)));
+                if (h) wa ^= ar[4];
+                V y;
+                y.x = t1[h][v] - t2[h][v];
+                y.y = t3[h][v] - t1[h][v] - t2[h][v];
+                lds[wa ^ col] = y;
+            }
+        }
+    }
+}
 
+// One tile per workgroup: load, every block with a barrier after it, store.
+template <typename T, int BQ, int K, int NT>
+__global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
+    using V = typename BVec2<T>::type;
+    constexpr int NF = 3 * ((1 << BQ) / 16) * ((1 << BQ) / 4);   // A fragments: Ur, Ui, Ur + Ui
+    using TM = TileMap<K, NT>;
+    __shared__ V lds[1 << K];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     V* psi = reinterpret_cast<V*>(a.psi);
     const T* frags = reinterpret_cast<const T*>(a.frags);
     const KSV_CONST uint64_t* recs = to_const(reinterpret_cast<const uint64_t*>(a.recs));
-
-    uint64_t base = blockIdx.x;
+    const TM tm(a, tid);
+    const uint64_t base = TM::base(a, blockIdx.x);
 #pragma unroll
-    for (int i = 0; i < 16; i++)
-        if (i < K) base = ins0_64(base, byte_at(i < 8 ? a.tq_lo : a.tq_hi, i & 7));
-
-    // tile index x = tid + i * NT: thread part and uniform part of the
-    // physical offset and of the swizzled LDS address
-    uint64_t g_t = 0;
-#pragma unroll
-    for (int b = 0; b < LNT; b++) g_t |= (uint64_t)((tid >> b) & 1) << byte_at(b < 8 ? a.tq_lo : a.tq_hi, b & 7);
-    const uint32_t l_t = tid ^ swz_part(tid, a.swz, 4, LNT);
-#pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const uint32_t xh = (uint32_t)i * NT;
-        uint64_t g_h = 0;
-#pragma unroll
-        for (int b = LNT; b < K; b++)
-            if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
-        lds[l_t ^ xh ^ swz_part(xh, a.swz, LNT, K)] = psi[base | g_t | g_h];
-    }
+    for (int i = 0; i < TM::PER; i++) lds[tm.laddr(a, i)] = psi[tm.gaddr(a, base, i)];
     __syncthreads();
-
-    const uint32_t rl = lane >> 4, cl = lane & 15;
     for (int blk = 0; blk < a.nblk; blk++) {
         const KSV_CONST uint64_t* w = recs + 8 * blk;
-        uint16_t ar[8], ac[16];
-        {
-            const uint64_t r0 = w[0], r1 = w[1], r2 = w[2], r3 = w[3], r4 = w[4], r5 = w[5];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                ar[i] = (uint16_t)(r0 >> (16 * i));
-                ar[4 + i] = (uint16_t)(r1 >> (16 * i));
-                ac[i] = (uint16_t)(r2 >> (16 * i));
-                ac[4 + i] = (uint16_t)(r3 >> (16 * i));
-                ac[8 + i] = (uint16_t)(r4 >> (16 * i));
-                ac[12 + i] = (uint16_t)(r5 >> (16 * i));
-            }
-        }
         const int64_t foff = (int64_t)w[6];
-        const bool diag = (w[7] & BREC_DIAG) != 0;     // a phase per row (fragments hold the diagonal)
         T af[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) af[f] = frags[foff + f * 64 + lane];
-
-        // read rows r = 4t + rl: bits 0,1 from rl, the rest from t
-        const uint32_t rd_l = ((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u);
-        // write rows r = 16 h + (4 v + rl) (f64) or 16 h + (4 rl + v) (f32)
-        constexpr bool LOW = M::lane_is_row_low;
-        const uint32_t wr_l = LOW ? (((rl & 1) ? ar[0] : 0u) ^ ((rl & 2) ? ar[1] : 0u))
-                                  : (((rl & 1) ? ar[2] : 0u) ^ ((rl & 2) ? ar[3] : 0u));
-        const uint32_t col_l = ((cl & 1) ? ac[0] : 0u) ^ ((cl & 2) ? ac[1] : 0u) ^ ((cl & 4) ? ac[2] : 0u) ^
-                               ((cl & 8) ? ac[3] : 0u);
-#pragma unroll
-        for (int m = 0; m < NBW; m++) {
-            const uint32_t nb = wave + NW * m;
-            uint32_t col = col_l;
-#pragma unroll
-            for (int b = 0; b < K - BQ - 4; b++)
-                if ((nb >> b) & 1) col ^= ac[4 + b];
-            V x[R / 4];
-            uint32_t ra[R / 4];
-#pragma unroll
-            for (int t = 0; t < R / 4; t++) {
-                ra[t] = rd_l;
-#pragma unroll
-                for (int b = 0; b < BQ - 2; b++)
-                    if ((t >> b) & 1) ra[t] ^= ar[2 + b];
-                x[t] = lds[ra[t] ^ col];
-            }
-            if (diag) {
-#pragma unroll
-                for (int t = 0; t < R / 4; t++) {
-                    const T pr = af[2 * t], pi = af[2 * t + 1];
-                    V y;
-                    y.x = x[t].x * pr - x[t].y * pi;
-                    y.y = x[t].x * pi + x[t].y * pr;
-                    lds[ra[t] ^ col] = y;
-                }
-                continue;
-            }
-            acc_t t1[MB], t2[MB], t3[MB];
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++) t1[mb] = t2[mb] = t3[mb] = acc_t{0, 0, 0, 0};
-#pragma unroll
-            for (int kk = 0; kk < KS; kk++) {
-                const T xr = x[kk].x, xi = x[kk].y, xs = xr + xi;
-#pragma unroll
-                for (int mb = 0; mb < MB; mb++) {
-                    t1[mb] = M::step(af[mb * KS + kk], xr, t1[mb]);
-                    t2[mb] = M::step(af[(MB + mb) * KS + kk], xi, t2[mb]);
-                    t3[mb] = M::step(af[(2 * MB + mb) * KS + kk], xs, t3[mb]);
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < MB; h++) {
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    uint32_t wa = wr_l ^ (LOW ? (((v & 1) ? ar[2] : 0u) ^ ((v & 2) ? ar[3] : 0u))
-                                              : (((v & 1) ? ar[0] : 0u) ^ ((v & 2) ? ar[1] : 0u)));
-                    if (h) wa ^= ar[4];
-                    V y;
-                    y.x = t1[h][v] - t2[h][v];
-                    y.y = t3[h][v] - t1[h][v] - t2[h][v];
-                    lds[wa ^ col] = y;
-                }
-            }
-        }
+        bpass_block<T, BQ, K, NT>(lds, w, af, lane, wave);
         __syncthreads();
     }
-
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const uint32_t xh = (uint32_t)i * NT;
-        uint64_t g_h = 0;
-#pragma unroll
-        for (int b = LNT; b < K; b++)
-            if ((xh >> b) & 1) g_h |= 1ull << byte_at(a.tq_hi, b - 8);
-        psi[base | g_t | g_h] = lds[l_t ^ xh ^ swz_part(xh, a.swz, LNT, K)];
-    }
+    for (int i = 0; i < TM::PER; i++) psi[tm.gaddr(a, base, i)] = lds[tm.laddr(a, i)];
 }
 
 template <typename T>
