@@ -207,8 +207,9 @@ def trace(ansatz, params):
         return None
     if ctx.escaped or not isinstance(circ, Circuit):
         return None
+    from . import gatevec as V
     flat = _flatten(circ._ops())
-    records = []
+    pending = []            # (op, param, coeff, name, which, angle values)
     uses = {}
     exact = True
     for i, op in enumerate(flat):
@@ -216,35 +217,42 @@ def trace(ansatz, params):
         if not meta or not isinstance(meta, tuple):
             continue
         name, ps = meta
-        vals = [p.value if isinstance(p, Sym) else float(p) for p in ps]
+        vals = None
         for which, p in enumerate(ps):
             if not isinstance(p, Sym):
                 continue
             for pi, coeff in p.terms.items():
                 if coeff == 0:
                     continue
-                d = _dmat(name, vals, which)
-                if d is None:
+                if name not in V.MATRIX:
                     return None
-                records.append((i, pi, coeff, d))
+                if vals is None:
+                    vals = [p.value if isinstance(p, Sym) else float(p) for p in ps]
+                pending.append((i, pi, coeff, name, which, vals))
                 uses[pi] = uses.get(pi, 0) + 1
                 if name not in _PSR_EXACT or abs(coeff) != 1.0:
                     exact = False
     if any(v != 1 for v in uses.values()):
         exact = False
+    # every derivative matrix of one gate kind and angle in one vectorised call
+    groups = {}
+    for j, rec in enumerate(pending):
+        groups.setdefault((rec[3], rec[4]), []).append(j)
+    dmats = [None] * len(pending)
+    for (name, which), js in groups.items():
+        k = V.MATRIX[name][1]
+        args = [np.array([pending[j][5][a] for j in js], dtype=np.float64) for a in range(k)]
+        res = V.derivative(name, args, which)
+        if res is None:
+            return None
+        for j, row in zip(js, res):
+            dmats[j] = row
+    records = [(i, pi, coeff, d) for (i, pi, coeff, _, _, _), d in zip(pending, dmats)]
     return flat, records, exact, circ.qubits
 
 
 def _adjoint(flat, records, nparam, n, hamiltonian):
-    ops, mats = pack(flat)
-    mats = list(mats)
-    rec = np.zeros(len(records), dtype=PARAM_DTYPE)
-    for j, (oi, pi, coeff, d) in enumerate(records):
-        rec[j] = (oi, pi, coeff, len(mats))
-        for v in d:
-            mats.append(complex(v).real)
-            mats.append(complex(v).imag)
-    mats = np.asarray(mats, dtype=np.float64)
+    ops, mats, rec = _packed_with_derivatives(flat, records)
     x, z, c = _terms(hamiltonian, n)
     st = State(n, OPTIONS["precision"], prune=False, tile=OPTIONS["tile"], adj_tile=OPTIONS["adj_tile"],
                adj_threads=OPTIONS["adj_threads"])
@@ -300,15 +308,17 @@ _BATCH_KEEP = 4
 def _packed_with_derivatives(flat, records):
     """(ops, mats with every record's dG appended, records as PARAM_DTYPE)."""
     ops, mats = pack(flat)
-    extra = []
     rec = np.zeros(len(records), dtype=PARAM_DTYPE)
-    base = len(mats)
-    for j, (oi, pi, coeff, d) in enumerate(records):
-        rec[j] = (oi, pi, coeff, base + len(extra))
-        for v in d:
-            extra.append(complex(v).real)
-            extra.append(complex(v).imag)
-    return ops, np.concatenate([np.asarray(mats, dtype=np.float64), np.asarray(extra, dtype=np.float64)]), rec
+    if not records:
+        return ops, mats, rec
+    ds = [np.asarray(d, dtype=np.complex128).reshape(-1) for _, _, _, d in records]
+    sizes = np.array([2 * len(d) for d in ds], dtype=np.int64)
+    rec["op"] = [r[0] for r in records]
+    rec["param"] = [r[1] for r in records]
+    rec["coeff"] = [r[2] for r in records]
+    rec["dmat"] = len(mats) + np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    extra = np.concatenate(ds).view(np.float64)
+    return ops, np.concatenate([mats, extra]), rec
 
 
 def _scatter(mats_all, offs, res, m, d):

@@ -38,14 +38,19 @@ def _flat_matrix(m):
 
 
 def pack(flat_ops):
-    """flat ops -> (ops: structured array of OP_DTYPE, mats: float64 array)."""
+    """flat ops -> (ops: structured array of OP_DTYPE, mats: float64 array).
+
+    Matrices get their offsets in op order; gates of 4 and of 16 entries
+    (the 2x2 and 4x4 gate tuples) are converted to doubles in bulk."""
     n = len(flat_ops)
     ops = np.zeros(n, dtype=OP_DTYPE)
     kinds = ops["kind"]
     nqs = ops["nq"]
     qs = ops["q"]
     matoff = ops["mat"]
-    mats = []
+    total = 0
+    by_size = {4: ([], []), 16: ([], [])}   # entries -> (offsets, matrices)
+    other = []                                # (offset, matrix) of any other shape
     cache = {}
     for i, op in enumerate(flat_ops):
         k = op[0]
@@ -59,8 +64,16 @@ def pack(flat_ops):
             key = id(g)
             hit = cache.get(key)
             if hit is None or hit[0] is not g:
-                off = len(mats)
-                mats.extend(_flat_matrix(g))
+                off = total
+                flat = isinstance(g, tuple) and len(g) in by_size and not isinstance(g[0], (list, tuple))
+                if flat:
+                    by_size[len(g)][0].append(off)
+                    by_size[len(g)][1].append(g)
+                    total += 2 * len(g)
+                else:
+                    m = _flat_matrix(g)
+                    other.append((off, m))
+                    total += len(m)
                 cache[key] = (g, off)
             else:
                 off = hit[1]
@@ -69,8 +82,10 @@ def pack(flat_ops):
             q = tuple(op[1])
             if len(q) > KSV_MAX_OPERANDS:
                 raise TypeError(f"unitary on {len(q)} qubits exceeds the {KSV_MAX_OPERANDS}-qubit limit")
-            matoff[i] = len(mats)
-            mats.extend(_flat_matrix(op[2]))
+            matoff[i] = total
+            m = _flat_matrix(op[2])
+            other.append((total, m))
+            total += len(m)
         elif k == "barrier":
             q = ()
             matoff[i] = -1
@@ -86,4 +101,11 @@ def pack(flat_ops):
         nqs[i] = len(q)
         for j, v in enumerate(q):
             qs[i, j] = v
-    return ops, np.asarray(mats, dtype=np.float64)
+    mats = np.empty(total, dtype=np.float64)
+    for size, (offs, gs) in by_size.items():
+        if offs:
+            vals = np.array(gs, dtype=np.complex128).view(np.float64)          # rows of 2 * size doubles
+            mats[np.asarray(offs, dtype=np.int64)[:, None] + np.arange(2 * size)[None, :]] = vals
+    for off, m in other:
+        mats[off:off + len(m)] = m
+    return ops, mats
