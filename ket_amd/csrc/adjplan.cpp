@@ -70,11 +70,25 @@ bool same_support(const AItem& a, const AItem& b) {
     return mask_of(a.ctl) == mask_of(b.ctl);
 }
 
+// An X (cnot, Toffoli) whose controls all lie in tile T can head a chain of
+// uncontrolled one-qubit dense ops on its target: the kernel swaps each pair
+// whose control bits are set, then runs the chain on it, in one LDS round trip.
+bool x_head(const AItem& a, uint64_t T) {
+    return a.kind == AK_X && a.tg.size() == 1 && !a.ctl.empty() && (mask_of(a.ctl) & ~T) == 0;
+}
+
+bool plain_1q(const AItem& b, int t) {
+    return b.kind == AK_DENSE && b.tg.size() == 1 && b.tg[0] == t && b.ctl.empty();
+}
+
 // Order the ops of a pass into chains: op j joins the chain of an earlier op
 // i with the same support when no op left between them touches those
-// qubits (j then commutes with everything it moves past).  run[x] = chain
-// length at a chain head, 0 inside a chain.
-void make_chains(const std::vector<AItem>& it, const std::vector<int>& chosen, std::vector<int>& order,
+// qubits (j then commutes with everything it moves past).  An X with local
+// controls heads a chain of the uncontrolled one-qubit ops on its target that
+// follow it, under the same rule on the target (ops between them may touch
+// the controls: the X reads them at its own place).  run[x] = chain length at
+// a chain head, 0 inside a chain.
+void make_chains(const std::vector<AItem>& it, const std::vector<int>& chosen, uint64_t T, std::vector<int>& order,
                  std::vector<int>& run) {
     order.clear();
     run.clear();
@@ -86,6 +100,28 @@ void make_chains(const std::vector<AItem>& it, const std::vector<int>& chosen, s
         const size_t head = order.size();
         order.push_back(chosen[i]);
         run.push_back(1);
+        if (x_head(a, T)) {
+            const int t = a.tg[0];
+            const uint64_t tm = 1ull << t;
+            const AItem* sup = nullptr;
+            uint64_t blk = 0;
+            for (size_t j = i + 1; j < chosen.size() && run[head] < 1 + ACHAIN_1Q; j++) {
+                if (used[j]) continue;
+                const AItem& b = it[chosen[j]];
+                const bool fits = sup ? same_support(*sup, b) : plain_1q(b, t);
+                if (fits && !(b.touch & blk)) {
+                    used[j] = 1;
+                    order.push_back(chosen[j]);
+                    run.push_back(0);
+                    run[head]++;
+                    if (!sup) sup = &b;
+                } else {
+                    blk |= b.touch;
+                    if (blk & tm) break;
+                }
+            }
+            continue;
+        }
         if (a.kind != AK_DENSE || a.tg.size() > 2) continue;
         const int cap = a.tg.size() == 1 ? ACHAIN_1Q : ACHAIN_2Q;
         uint64_t blk = 0;
@@ -282,7 +318,7 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
         chosen.clear();
         rest.clear();
         take(items, rem, T, all, 0, &chosen, &rest);
-        make_chains(items, chosen, order, run);
+        make_chains(items, chosen, T, order, run);
         AdjStep s;
         s.k = K;
         int pos[64];

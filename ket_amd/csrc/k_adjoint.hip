@@ -79,7 +79,7 @@ __device__ __forceinline__ ChainOp chain_op(const KSV_CONST uint64_t* opw, int c
 template <typename V, int KK, int G, int CMAX, int NT>
 __device__ __forceinline__ void adj_chain(V* P, V* L, const AOp& op, const KSV_CONST uint64_t* opw, int run,
                                           const KSV_CONST double* mats, uint32_t ne, uint32_t tid,
-                                          double (&acc)[ACHAIN_1Q][3]) {
+                                          double (&acc)[ACHAIN_1Q][3], uint32_t xctl) {
     constexpr int D = 1 << KK;
     for (uint32_t e0 = tid; e0 < ne; e0 += G * NT) {
         uint32_t id[G][D];
@@ -104,6 +104,19 @@ __device__ __forceinline__ void adj_chain(V* P, V* L, const AOp& op, const KSV_C
                 xr[g][r] = x.x; xi[g][r] = x.y;
                 yr[g][r] = y.x; yi[g][r] = y.y;
             }
+        if (KK == 1 && xctl) {
+            // the heading X: swap the pair where its control bits are set
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                if ((id[g][0] & xctl) == xctl) {
+                    double t;
+                    t = xr[g][0]; xr[g][0] = xr[g][1]; xr[g][1] = t;
+                    t = xi[g][0]; xi[g][0] = xi[g][1]; xi[g][1] = t;
+                    t = yr[g][0]; yr[g][0] = yr[g][1]; yr[g][1] = t;
+                    t = yi[g][0]; yi[g][0] = yi[g][1]; yi[g][1] = t;
+                }
+            }
+        }
 #pragma unroll
         for (int c = 0; c < CMAX; c++) {
             if (c >= run) break;
@@ -204,14 +217,14 @@ __device__ __forceinline__ void adj_chain(V* P, V* L, const AOp& op, const KSV_C
 template <typename V, int KK, int GMAX, int CMAX, int NT>
 __device__ __forceinline__ void adj_chain_g(V* P, V* L, const AOp& op, const KSV_CONST uint64_t* opw, int run,
                                             const KSV_CONST double* mats, uint32_t ne, uint32_t tid,
-                                            double (&acc)[ACHAIN_1Q][3]) {
+                                            double (&acc)[ACHAIN_1Q][3], uint32_t xctl) {
     if constexpr (GMAX >= 4) {
-        if (ne >= 4 * NT) return adj_chain<V, KK, 4, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+        if (ne >= 4 * NT) return adj_chain<V, KK, 4, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc, xctl);
     }
     if constexpr (GMAX >= 2) {
-        if (ne >= 2 * NT) return adj_chain<V, KK, 2, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+        if (ne >= 2 * NT) return adj_chain<V, KK, 2, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc, xctl);
     }
-    adj_chain<V, KK, 1, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+    adj_chain<V, KK, 1, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc, xctl);
 }
 
 // One backward pass: the workgroup walks tiles blockIdx.x, +gridDim.x, ...;
@@ -268,11 +281,11 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                 __builtin_memcpy(&op, w, sizeof op);
             }
             if ((base & op.gctl) != op.gctl) {   // a chain shares its head's controls
-                if (op.kind == AK_DENSE) o += op.run - 1;
+                if (op.run > 1) o += op.run - 1;
                 continue;
             }
-            const uint32_t ne = tsize >> op.nins;
-            if (op.kind == AK_X) {
+            uint32_t ne = tsize >> op.nins;
+            if (op.kind == AK_X && op.run <= 1) {
                 const uint32_t tb = 1u << op.t[0];
                 for (uint32_t e = tid; e < ne; e += NT) {
                     const uint32_t l0 = aop_elem(e, op), l1 = l0 | tb;
@@ -289,9 +302,21 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                     L[la] = y1; L[lb] = y0;
                 }
             } else {
-                // a chain of `run` dense ops from here
-                const int run = op.run;
-                const KSV_CONST uint64_t* opw = ops + (sizeof(AOp) / 8) * o;
+                // a chain of dense ops from here, possibly headed by an X with local
+                // controls (its members are then uncontrolled one-qubit ops on the X's
+                // target, enumerated by the first member)
+                const int xh = op.kind == AK_X ? 1 : 0;
+                const uint32_t xctl = xh ? op.lctl : 0u;
+                const int span = op.run;
+                const int run = span - xh;
+                const KSV_CONST uint64_t* opw = ops + (sizeof(AOp) / 8) * (o + xh);
+                if (xh) {
+                    uint64_t w[sizeof(AOp) / 8];
+#pragma unroll
+                    for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = opw[i];
+                    __builtin_memcpy(&op, w, sizeof op);
+                    ne = tsize >> op.nins;
+                }
                 double acc[ACHAIN_1Q][3];
 #pragma unroll
                 for (int c = 0; c < ACHAIN_1Q; c++)
@@ -299,9 +324,9 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                     for (int j = 0; j < 3; j++) acc[c][j] = 0;
                 constexpr int G1 = (1 << (KT - 1)) / NT >= 4 ? 4 : (1 << (KT - 1)) / NT;
                 constexpr int G2 = (1 << (KT - 2)) / NT >= 2 ? 2 : 1;
-                if (op.k == 1) adj_chain_g<V, 1, G1, ACHAIN_1Q, NT>(P, L, op, opw, run, mats, ne, tid, acc);
-                else if (op.k == 2) adj_chain_g<V, 2, G2, ACHAIN_2Q, NT>(P, L, op, opw, run, mats, ne, tid, acc);
-                else adj_chain<V, 3, 1, 1, NT>(P, L, op, opw, run, mats, ne, tid, acc);
+                if (op.k == 1) adj_chain_g<V, 1, G1, ACHAIN_1Q, NT>(P, L, op, opw, run, mats, ne, tid, acc, xctl);
+                else if (op.k == 2) adj_chain_g<V, 2, G2, ACHAIN_2Q, NT>(P, L, op, opw, run, mats, ne, tid, acc, 0u);
+                else adj_chain<V, 3, 1, 1, NT>(P, L, op, opw, run, mats, ne, tid, acc, 0u);
 #pragma unroll
                 for (int c = 0; c < ACHAIN_1Q; c++) {
                     if (c >= run) break;
@@ -321,7 +346,7 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
                         if (lane == 0) red[m.slot[j]][wave] += v;
                     }
                 }
-                o += run - 1;
+                o += span - 1;
             }
             __syncthreads();
         }

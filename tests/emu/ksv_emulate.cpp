@@ -668,10 +668,36 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
                 L[l] = lam[off];
             }
             for (int o = 0; o < s.op_count; o++) {
+                // an X with local controls may head a chain of one-qubit ops on its
+                // target: each pair is swapped where the controls are set, then the
+                // chain runs on it
+                uint32_t xctl = 0;
+                int xpos = -1;
+                const AOp& h = plan.aops[s.op_begin + o];
+                if (h.kind == AK_X && h.run > 1) {
+                    CHECK(h.k == 1 && h.gctl == 0 && h.lctl != 0 && h.run <= 1 + ACHAIN_1Q && o + h.run <= s.op_count,
+                          "bad X-headed chain");
+                    const AOp& m0 = plan.aops[s.op_begin + o + 1];
+                    CHECK(m0.kind == AK_DENSE && m0.k == 1 && m0.t[0] == h.t[0] && m0.lctl == 0 && m0.gctl == 0 &&
+                              m0.run == 0,
+                          "X-headed chain member");
+                    xctl = h.lctl;
+                    xpos = h.t[0];
+                    o++;
+                }
                 const AOp& op = plan.aops[s.op_begin + o];
                 CHECK((op.gctl & tmask) == 0, "global control inside the tile");
                 int run = 1;
-                if (op.kind == AK_DENSE) {
+                if (xpos >= 0) {
+                    run = h.run - 1;
+                    for (int c = 1; c < run; c++) {
+                        const AOp& m = plan.aops[s.op_begin + o + c];
+                        CHECK(m.kind == AK_DENSE && m.run == 0 && m.k == op.k && m.nins == op.nins &&
+                                  m.lctl == op.lctl && m.gctl == op.gctl &&
+                                  std::memcmp(m.t, op.t, sizeof op.t) == 0 && std::memcmp(m.ins, op.ins, sizeof op.ins) == 0,
+                              "chain member with another support");
+                    }
+                } else if (op.kind == AK_DENSE) {
                     run = op.run;
                     const int cap = op.k == 1 ? ACHAIN_1Q : (op.k == 2 ? ACHAIN_2Q : 1);
                     CHECK(run >= 1 && run <= cap && o + run <= s.op_count, "bad chain length");
@@ -723,6 +749,13 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
                         id[r] = g;
                         x[r] = P[g];
                         y[r] = L[g];
+                    }
+                    if (xpos >= 0) {
+                        CHECK((xctl & (1u << xpos)) == 0 && (l & (1u << xpos)) == 0, "X control on its target");
+                        if ((l & xctl) == xctl) {
+                            std::swap(x[0], x[1]);
+                            std::swap(y[0], y[1]);
+                        }
                     }
                     // the chain, in registers
                     for (int c = 0; c < run; c++) {
