@@ -104,6 +104,9 @@ typedef struct ksv_program ksv_program;
                                    10..12 (FP32)                                               */
 #define KSV_OPT_ADJ_THREADS 10  /* ksv_adjoint backward-pass workgroup size: 0 = default (256),
                                    256 or 512 (512 on the default tile only)                   */
+#define KSV_OPT_BLOCK_PIPE  11  /* FP64 fused-block passes of 3-4 blocks on 12-qubit tiles:
+                                   1 (default) = workgroups walk tiles with the next tile loaded
+                                   during the current one's blocks, 0 = one tile per workgroup  */
 
 /* Per-state launch statistics (KSV_OPT_TIMING). */
 typedef struct ksv_stats {

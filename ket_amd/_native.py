@@ -18,7 +18,7 @@ KSV_F64, KSV_F32 = 0, 1
 KSV_MAX_OPERANDS = 12
 (KSV_OPT_PRUNE, KSV_OPT_TILE_QUBITS, KSV_OPT_TIMING, KSV_OPT_FUSE, KSV_OPT_PASS_KERNEL, KSV_OPT_ROUND_TILE,
  KSV_OPT_BLOCK_QUBITS, KSV_OPT_BLOCK_TILE, KSV_OPT_BLOCK_THREADS, KSV_OPT_ADJ_TILE,
- KSV_OPT_ADJ_THREADS) = range(11)
+ KSV_OPT_ADJ_THREADS, KSV_OPT_BLOCK_PIPE) = range(12)
 
 OP_DTYPE = np.dtype([("kind", "<i4"), ("nq", "<i4"), ("q", "<i4", (KSV_MAX_OPERANDS,)), ("mat", "<i8")])
 PARAM_DTYPE = np.dtype([("op", "<i4"), ("param", "<i4"), ("coeff", "<f8"), ("dmat", "<i8")])
@@ -175,7 +175,7 @@ class State:
     """Owning handle of one device statevector (2^n amplitudes in HBM)."""
 
     def __init__(self, n, precision="f64", prune=True, tile=0, fuse=False, timing=False, kernel="auto",
-                 rtile=0, bq=0, btile=0, bthreads=0, adj_tile=0, adj_threads=0):
+                 rtile=0, bq=0, btile=0, bthreads=0, adj_tile=0, adj_threads=0, bpipe=None):
         self.n = int(n)
         self.precision = precision
         prec = KSV_F64 if precision == "f64" else KSV_F32
@@ -203,6 +203,8 @@ class State:
             self.set_option(KSV_OPT_ADJ_TILE, adj_tile)
         if adj_threads:
             self.set_option(KSV_OPT_ADJ_THREADS, adj_threads)
+        if bpipe is not None:
+            self.set_option(KSV_OPT_BLOCK_PIPE, 1 if bpipe else 0)
 
     @property
     def handle(self):

@@ -55,6 +55,7 @@ struct ksv_state {
     int bq = 0;                     // KSV_OPT_BLOCK_QUBITS
     int btile = 0;                  // KSV_OPT_BLOCK_TILE
     int bthreads = 256;             // KSV_OPT_BLOCK_THREADS
+    int bpipe = 1;                  // KSV_OPT_BLOCK_PIPE
     int atile = 0;                  // KSV_OPT_ADJ_TILE
     int athreads = 256;             // KSV_OPT_ADJ_THREADS
     ksv_stats stats;
@@ -201,6 +202,10 @@ int ksv_set_option(ksv_state* s, int option, double value) {
         s->bthreads = t ? t : 256;
         return KSV_OK;
     }
+    case KSV_OPT_BLOCK_PIPE:
+        if (value != 0 && value != 1) return fail(KSV_ERR_RANGE, "fused-block pipelining must be 0 or 1");
+        s->bpipe = (int)value;
+        return KSV_OK;
     case KSV_OPT_ADJ_TILE: {
         int t = (int)value;
         const int hi = s->prec == KSV_F64 ? ADJ_TILE_F64 : ADJ_TILE_F32;
@@ -334,6 +339,7 @@ static int build_program(ksv_state* s, const ksv_op* ops, size_t nops, const dou
     if (rc) return rc;
     p->dev = s->dev;
     p->bthreads = s->bthreads;
+    p->bpipe = s->bpipe;
     std::vector<char> blob;
     std::vector<size_t> off;
     program_blob(s, *p, blob, off);
@@ -1169,6 +1175,7 @@ int ksv_shots_run(ksv_state* s, int n, const ksv_op* ops, size_t nops, const dou
         }
         p->dev = s->dev;
         p->bthreads = s->bthreads;
+        p->bpipe = s->bpipe;
         p->borrowed = true;
         std::vector<size_t> off;
         std::vector<char> one;

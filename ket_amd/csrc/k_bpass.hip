@@ -23,6 +23,12 @@
 // low four bits XORed by a per-pass word of its higher bits, chosen so that
 // each 16-lane group hits 16 distinct 16-byte bank groups.  HBM sees one
 // coalesced read and one coalesced write of the tile per pass.
+//
+// FP64 passes of 3-4 blocks run as k_bpass_pipe instead: two 512-thread
+// workgroups per CU each walk a strided list of tiles, keep the pass's
+// fragments in LDS, and load the next tile into registers while the current
+// one's blocks run (barriers order LDS only, so those loads stay in flight).
+#include <algorithm>
 #include <cstring>
 
 #include "kcommon.h"
@@ -35,6 +41,13 @@ template <> struct BVec2<float> { using type = float2; };
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+// the amplitude as a plain vector: an array of these held across a loop stays in
+// registers, where an array of double2 structs ends up in scratch
+template <typename T> struct Ext2;
+template <> struct Ext2<double> { using type = d2_t; };
+template <> struct Ext2<float> { using type = f2_t; };
 
 // Output layout of 16x16x4 (measured on gfx950): D[i][j] sits in lane
 // j + 16 * (i % 4), element i / 4 for f64, and in lane j + 16 * (i / 4),
@@ -230,18 +243,131 @@ __global__ __launch_bounds__(NT) void k_bpass(BPassArgs a) {
     const uint64_t base = TM::base(a, blockIdx.x);
 #pragma unroll
     for (int i = 0; i < TM::PER; i++) lds[tm.laddr(a, i)] = psi[tm.gaddr(a, base, i)];
+    // fragments of block b + 1 are loaded while block b runs
+    T af[NF];
+    if (a.nblk > 0) {
+        const int64_t foff = (int64_t)recs[6];
+#pragma unroll
+        for (int f = 0; f < NF; f++) af[f] = frags[foff + f * 64 + lane];
+    }
     __syncthreads();
     for (int blk = 0; blk < a.nblk; blk++) {
         const KSV_CONST uint64_t* w = recs + 8 * blk;
-        const int64_t foff = (int64_t)w[6];
-        T af[NF];
+        T an[NF];
+        const int64_t fn = (int64_t)recs[8 * (blk + 1 < a.nblk ? blk + 1 : blk) + 6];
 #pragma unroll
-        for (int f = 0; f < NF; f++) af[f] = frags[foff + f * 64 + lane];
+        for (int f = 0; f < NF; f++) an[f] = frags[fn + f * 64 + lane];
         bpass_block<T, BQ, K, NT>(lds, w, af, lane, wave);
+#pragma unroll
+        for (int f = 0; f < NF; f++) af[f] = an[f];
         __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < TM::PER; i++) psi[tm.gaddr(a, base, i)] = lds[tm.laddr(a, i)];
+}
+
+// Workgroup barrier that orders LDS accesses only: outstanding global loads
+// (the next tile) and stores (the previous one) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Fragments the pipelined kernel keeps in LDS per block: FP64 keeps Ur and Ui
+// and adds Ur + Ui in registers (the same double addition the host makes, so
+// the fragments are identical); FP32 keeps all three (the host rounds the FP64
+// sum to FP32, which a sum of the rounded parts would not always equal).
+template <typename T, int BQ> struct PipeFrags {
+    static constexpr int NL = (sizeof(T) == 8 ? 2 : 3) * ((1 << BQ) / 16) * ((1 << BQ) / 4);
+};
+
+// Pipelined form: each workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+// The pass's fragments sit in LDS beside the tile (read once per workgroup), so
+// the only global traffic inside the loop is the tiles themselves: tile t + G is
+// loaded into registers while the blocks of tile t run on MFMA, and the stores
+// of tile t drain while tile t + G is computed.
+template <typename T, int BQ, int K, int NT>
+__global__ __launch_bounds__(NT) void k_bpass_pipe(BPassArgs a, uint64_t ntiles) {
+    using V = typename BVec2<T>::type;
+    constexpr int NF = 3 * ((1 << BQ) / 16) * ((1 << BQ) / 4);
+    constexpr int NL = PipeFrags<T, BQ>::NL;   // fragments kept in LDS per block
+    using TM = TileMap<K, NT>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    V* lds = reinterpret_cast<V*>(smem);
+    T* flds = reinterpret_cast<T*>(smem + sizeof(V) * (1 << K));
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T* frags = reinterpret_cast<const T*>(a.frags);
+    const KSV_CONST uint64_t* recs = to_const(reinterpret_cast<const uint64_t*>(a.recs));
+    for (int b = 0; b < a.nblk; b++) {
+        const int64_t foff = (int64_t)recs[8 * b + 6];
+        for (int r = tid; r < NL * 64; r += NT) flds[b * NL * 64 + r] = frags[foff + r];
+    }
+    using E = typename Ext2<T>::type;
+    E* psie = reinterpret_cast<E*>(a.psi);
+    E* ldse = reinterpret_cast<E*>(smem);
+    const TM tm(a, tid);
+    E nxt[TM::PER];
+    {
+        const uint64_t base = TM::base(a, blockIdx.x);
+#pragma unroll
+        for (int i = 0; i < TM::PER; i++) nxt[i] = psie[tm.gaddr(a, base, i)];
+#pragma unroll
+        for (int i = 0; i < TM::PER; i++) ldse[tm.laddr(a, i)] = nxt[i];
+    }
+    lds_barrier();
+    const uint64_t g = gridDim.x;
+    const uint64_t cnt = (ntiles - blockIdx.x + g - 1) / g;   // tiles of this workgroup
+    for (uint64_t j = 0; j < cnt; j++) {
+        const uint64_t t = blockIdx.x + j * g;
+        // the next tile (the last one re-reads itself, so the loads need no branch)
+        const uint64_t bn = TM::base(a, j + 1 < cnt ? t + g : t);
+#pragma unroll
+        for (int i = 0; i < TM::PER; i++) nxt[i] = psie[tm.gaddr(a, bn, i)];
+        for (int blk = 0; blk < a.nblk; blk++) {
+            const KSV_CONST uint64_t* w = recs + 8 * blk;
+            T af[NF];
+#pragma unroll
+            for (int f = 0; f < NL; f++) af[f] = flds[(blk * NL + f) * 64 + lane];
+#pragma unroll
+            for (int f = NL; f < NF; f++) af[f] = af[f - NL] + af[f - NL / 2];   // Ur + Ui, as the host forms it
+            bpass_block<T, BQ, K, NT>(lds, w, af, lane, wave);
+            lds_barrier();
+        }
+        const uint64_t base = TM::base(a, t);
+#pragma unroll
+        for (int i = 0; i < TM::PER; i++) psie[tm.gaddr(a, base, i)] = ldse[tm.laddr(a, i)];
+        lds_barrier();
+#pragma unroll
+        for (int i = 0; i < TM::PER; i++) ldse[tm.laddr(a, i)] = nxt[i];
+        lds_barrier();
+    }
+}
+
+template <typename T, int BQ, int K, int NT>
+static hipError_t launch_pipe(const BPassArgs& a, int n, hipStream_t st, bool* done) {
+    constexpr int NL = PipeFrags<T, BQ>::NL;
+    using V = typename BVec2<T>::type;
+    const size_t bytes = sizeof(V) * ((size_t)1 << K) + sizeof(T) * (size_t)a.nblk * NL * 64;
+    int dev = 0, cus = 0, maxlds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    *done = false;
+    if (bytes > (size_t)maxlds) return hipSuccess;   // too many blocks: the one-tile kernel runs the pass
+    auto kern = k_bpass_pipe<T, BQ, K, NT>;
+    e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    int occ = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NT, bytes);
+    if (e != hipSuccess) return e;
+    if (occ < 2) return hipSuccess;   // one workgroup per CU: the one-tile kernel is faster
+    const uint64_t ntiles = 1ull << (n - K);
+    const uint64_t g = std::min<uint64_t>(ntiles, (uint64_t)occ * (uint64_t)cus);
+    hipLaunchKernelGGL(kern, dim3((unsigned)g), dim3(NT), bytes, st, a, ntiles);
+    *done = true;
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -260,6 +386,16 @@ static hipError_t launch_bpass_t(const Program& p, const Step& s, void* psi, hip
     a.swz = s.swz;
     const dim3 grid((unsigned)(1ull << (p.n - s.k)));
     const bool wide = p.bthreads == 512;
+    // FP64 passes of 3 or 4 dense blocks: their MFMA work is about the tile's HBM
+    // time, which the pipelined walk overlaps (synthetic 30-qubit passes: 7.5 -> 7.1 ms
+    // at 3 blocks, 9.8 -> 8.8 ms at 4).  With fewer blocks the one-tile kernel streams
+    // faster; with more, the fragments leave room for one workgroup per CU only, and
+    // FP32 passes gain nothing.
+    if (p.bpipe && sizeof(T) == 8 && s.bq == 4 && s.k == 12 && s.op_count >= 3 && s.op_count <= 4) {
+        bool done = false;
+        hipError_t e = launch_pipe<T, 4, 12, 512>(a, p.n, st, &done);
+        if (e != hipSuccess || done) return e;
+    }
     if (s.bq == 4 && s.k == 11) hipLaunchKernelGGL((k_bpass<T, 4, 11, 256>), grid, dim3(256), 0, st, a);
     else if (s.bq == 4 && s.k == 12 && !wide) hipLaunchKernelGGL((k_bpass<T, 4, 12, 256>), grid, dim3(256), 0, st, a);
     else if (s.bq == 4 && s.k == 12) hipLaunchKernelGGL((k_bpass<T, 4, 12, 512>), grid, dim3(512), 0, st, a);

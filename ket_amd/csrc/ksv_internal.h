@@ -212,6 +212,7 @@ struct Program {
     BRec* d_brecs = nullptr;
     void* d_frags = nullptr;
     int bthreads = 256;              // fused-block passes: workgroup size (256 or 512)
+    int bpipe = 1;                   // fused-block passes: pipelined tile walk (k_bpass_pipe) where it pays
     int nsys = 0;                    // batched copies: qubits of one copy (conditional ops)
     std::vector<uint32_t> bits;      // copy bitmaps of conditional ops
     uint32_t* d_bits = nullptr;

@@ -24,10 +24,10 @@ TOL = 1e-12
 
 
 def gpu_state(n, calls, prune=True, tile=0, init=0, fuse=False, kernel="auto", rtile=0, bq=0, btile=0,
-              precision="f64", bthreads=0):
+              precision="f64", bthreads=0, bpipe=None):
     c = build(n, calls)
     st = State(n, precision, prune=prune, tile=tile, fuse=fuse, kernel=kernel, rtile=rtile, bq=bq, btile=btile,
-               bthreads=bthreads)
+               bthreads=bthreads, bpipe=bpipe)
     if init:
         st.init_basis(init)
     st.apply(c.tape())
@@ -108,6 +108,33 @@ def test_fused_block_passes(n, btile, bq, nt):
     ref = dense_ref(n, calls, prune=False)
     got = gpu_state(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq, bthreads=nt)
     assert np.max(np.abs(got - ref)) < TOL
+
+
+@pytest.mark.parametrize("n,kind,seed", [(16, "random", 1), (20, "random", 2), (22, "random", 3), (18, "qft", 4),
+                                          (21, "calls", 5)])
+def test_pipelined_block_passes(n, kind, seed):
+    """FP64 passes of 3-4 dense blocks run as a pipelined tile walk (each workgroup
+    loads its next tile while the current one's blocks run): amplitudes bit-identical
+    to the one-tile-per-workgroup kernel (same per-tile arithmetic) and within 1e-12
+    of the oracle, for random layers, QFT (diagonal blocks) and every op kind."""
+    from ket_amd._native import plan
+    if kind == "random":
+        calls = workloads.random_layer_calls(n, 12, seed=seed)
+    elif kind == "qft":
+        calls = [("x", (q,)) for q in range(0, n, 3)]
+        for j in range(n - 1, -1, -1):
+            calls.append(("h", (j,)))
+            calls += [("cu1", (math.pi / 2 ** (j - k), k, j)) for k in range(j - 1, -1, -1)]
+        calls += [("swap", (i, n - 1 - i)) for i in range(n // 2)]
+    else:
+        calls = random_calls(n, 300, seed=seed)
+    c = build(n, calls)
+    ref = dense_ref(n, calls, prune=False)
+    a = gpu_state(n, calls, prune=False, kernel="mfma", bpipe=False)
+    b = gpu_state(n, calls, prune=False, kernel="mfma", bpipe=True)
+    assert np.array_equal(a, b)
+    assert np.max(np.abs(b - ref)) < TOL
+    assert plan(n, c.tape(), kernel="mfma", prune=False)["blocks"] > 0
 
 
 @pytest.mark.parametrize("btile,nt", [(11, 256), (12, 512)])
