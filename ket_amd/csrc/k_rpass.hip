@@ -134,9 +134,14 @@ __device__ __forceinline__ void dmul(T& yr, T& yi, T dx, T dy, T xr, T xi) {
 #define RREGS_ARGS T (&re)[RREGS], T (&im)[RREGS]
 
 // 2x2 on slot S.  SEL: a thread-bit control exists (lp: this thread is
-// selected).  RC: slot controls exist (rctl).
+// selected).  RC: slot controls exist (rctl).  Under PR, `form` (ROP_REAL,
+// ROP_RXF) drops the products with a zero factor from ket's complex products:
+// mul(a, x) with a.y == 0 is (a.x x.re, a.x x.im) and with a.x == 0 it is
+// (-(a.y x.im), a.y x.re), each exact except for the sign of a zero, so the
+// sums below equal ket's add(mul(a, x0), mul(b, x1)) bit for bit wherever the
+// result is non-zero, and zeros are pruned either way.
 template <typename T, int S, bool PR, bool SEL, bool RC, typename P>
-__device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp) {
+__device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp, uint32_t form) {
     const CM<T> m0 = ldm(m, 0), m1 = ldm(m, 1), m2 = ldm(m, 2), m3 = ldm(m, 3);
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
@@ -145,8 +150,20 @@ __device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp) {
         const int j1 = j | (1 << S);
         const T x0r = re[j], x0i = im[j], x1r = re[j1], x1i = im[j1];
         T y0r, y0i, y1r, y1i;
-        mac2<T, PR>(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
-        mac2<T, PR>(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
+        if (PR && form == ROP_REAL) {
+            y0r = rn_add(rn_mul(m0.x, x0r), rn_mul(m1.x, x1r));
+            y0i = rn_add(rn_mul(m0.x, x0i), rn_mul(m1.x, x1i));
+            y1r = rn_add(rn_mul(m2.x, x0r), rn_mul(m3.x, x1r));
+            y1i = rn_add(rn_mul(m2.x, x0i), rn_mul(m3.x, x1i));
+        } else if (PR && form == ROP_RXF) {
+            y0r = rn_sub(rn_mul(m0.x, x0r), rn_mul(m1.y, x1i));
+            y0i = rn_add(rn_mul(m0.x, x0i), rn_mul(m1.y, x1r));
+            y1r = rn_sub(rn_mul(m3.x, x1r), rn_mul(m2.y, x0i));
+            y1i = rn_add(rn_mul(m2.y, x0r), rn_mul(m3.x, x1i));
+        } else {
+            mac2<T, PR>(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
+            mac2<T, PR>(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
+        }
         prune2<T, PR>(y0r, y0i);
         prune2<T, PR>(y1r, y1i);
         if (SEL) {
@@ -478,17 +495,21 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
         const bool lp = (tid & tctl) == tctl;
         const bool sel = tctl != 0;
         switch (kind) {
-        case R_M1:
+        case R_M1: {
+            // copy programs (STG) keep the general form: the reduced forms' code
+            // costs that build registers (noisy batch 142k -> 112k shots/s)
+            const uint32_t form = STG ? 0u : (uint32_t)(h1 >> 16) & (ROP_REAL | ROP_RXF);
             if (rctl == 0) {
-#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp))
+#define F(S) (sel ? r_m1<T, S, PR, true, false>(re, im, pay, 0, lp, form) : r_m1<T, S, PR, false, false>(re, im, pay, 0, lp, form))
                 RSLOT_CASES(sa, F)
 #undef F
             } else {
-#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp))
+#define F(S) (sel ? r_m1<T, S, PR, true, true>(re, im, pay, rctl, lp, form) : r_m1<T, S, PR, false, true>(re, im, pay, rctl, lp, form))
                 RSLOT_CASES(sa, F)
 #undef F
             }
             break;
+        }
         case R_DIAG:
             if (kq == 1) {
                 const uint32_t L = (uint32_t)locw & 0xFF;

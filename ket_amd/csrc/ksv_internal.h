@@ -96,6 +96,13 @@ struct ROp {
 };
 static_assert(sizeof(ROp) == 128, "ROp layout");
 constexpr uint16_t ROP_COND = 1;
+// R_M1 matrices whose entries are each purely real or purely imaginary: ket's
+// complex products with a zero part reduce to single real products (exactly,
+// up to the sign of zero results, which ket's 1e-14 prune then clears), so the
+// kernel may skip the vanishing terms.  ROP_REAL: all four entries real (ry,
+// h, z, ...); ROP_RXF: real diagonal, imaginary off-diagonal (rx, y).
+constexpr uint16_t ROP_REAL = 2;
+constexpr uint16_t ROP_RXF = 4;
 
 // ---------------------------------------------------------------------------
 // Fused-block pass: the circuit is fused on the host into dense blocks of

@@ -626,6 +626,15 @@ static void put_inline(ROp& r, const cd* v, int count, int prec) {
     }
 }
 
+// Structure of a 2x2 matrix for the register-round kernel (ROP_REAL / ROP_RXF).
+static uint16_t m1_form(const cd* m) {
+    auto real = [](const cd& v) { return v.imag() == 0.0; };
+    auto imag = [](const cd& v) { return v.real() == 0.0; };
+    if (real(m[0]) && real(m[1]) && real(m[2]) && real(m[3])) return ROP_REAL;
+    if (real(m[0]) && imag(m[1]) && imag(m[2]) && real(m[3])) return ROP_RXF;
+    return 0;
+}
+
 // One op in the slot / thread-bit / global form of layout L.
 static void emit_rop(const LOp& o, const RLayout& L, const int* pos, Program& prog) {
     auto slot_of = [&](int q) {
@@ -653,6 +662,7 @@ static void emit_rop(const LOp& o, const RLayout& L, const int* pos, Program& pr
             r.kind = R_M1;
             r.a = (uint8_t)slot_of(o.tg[0]);
             put_inline(r, o.m.data(), 4, prog.precision);
+            r.flags = m1_form(o.m.data());
         } else {
             // dense 2^k x 2^k (k = 2..4) on slots loc[0] (MSB) .. loc[k-1]
             r.kind = R_DENSE;
@@ -687,7 +697,7 @@ static void emit_rop(const LOp& o, const RLayout& L, const int* pos, Program& pr
     }
     }
     if (o.cond) {   // copy-conditional (Pauli events): kinds whose matrix is inline
-        r.flags = ROP_COND;
+        r.flags |= ROP_COND;
         r.mat = o.cond - 1;
     }
     prog.rops.push_back(r);
@@ -1262,7 +1272,10 @@ int plan_program(Program& p, int n, int prec, const ksv_op* ops, size_t nops, co
 // Payload of an already emitted register-round record after its op's matrix changed.
 static void refresh_rop(const LOp& o, ROp& r, Program& p, std::vector<std::pair<size_t, size_t>>& ranges) {
     switch (r.kind) {
-    case R_M1: put_inline(r, o.m.data(), 4, p.precision); break;
+    case R_M1:
+        put_inline(r, o.m.data(), 4, p.precision);
+        r.flags = (uint16_t)((r.flags & ROP_COND) | m1_form(o.m.data()));
+        break;
     case R_DIAG:
         if (o.tg.size() <= 2) put_inline(r, o.m.data(), (int)o.m.size(), p.precision);
         else {
