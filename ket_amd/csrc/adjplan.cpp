@@ -56,6 +56,29 @@ bool read_mat(const double* mats, size_t nmats, int64_t off, int dim, std::vecto
     return true;
 }
 
+// A one-qubit gate that is a rotation exp(-i theta P / 2) about a Pauli axis P
+// has dG/dtheta = (-i/2) P G, so the backward sweep's derivative term
+// Re <lambda| dG G^dag |phi> is Re <lambda| (-i/2) P |phi>.  Returns P (0 = X,
+// 1 = Y, 2 = Z) when the record's derivative `d` (2x2, interleaved complex)
+// times the op's G^dag `gd` equals (-i/2) P to rounding, else -1.
+int pauli_rotation(const std::vector<cd>& gd, const double* d) {
+    cd m[4];
+    for (int r = 0; r < 2; r++)
+        for (int c = 0; c < 2; c++) {
+            cd acc = 0;
+            for (int k = 0; k < 2; k++) acc += cd(d[2 * (2 * r + k)], d[2 * (2 * r + k) + 1]) * gd[2 * k + c];
+            m[2 * r + c] = acc;
+        }
+    const cd h(0, -0.5), z(0, 0);
+    const cd want[3][4] = {{z, h, h, z}, {z, cd(-0.5, 0), cd(0.5, 0), z}, {h, z, z, -h}};
+    for (int pk = 0; pk < 3; pk++) {
+        bool ok = true;
+        for (int i = 0; i < 4 && ok; i++) ok = std::abs(m[i] - want[pk][i]) <= 1e-14;
+        if (ok) return pk;
+    }
+    return -1;
+}
+
 bool offdiag_zero(const std::vector<cd>& m, int dim) {
     for (int r = 0; r < dim; r++)
         for (int c = 0; c < dim; c++)
@@ -217,7 +240,7 @@ uint64_t choose_tile(const std::vector<AItem>& it, const std::vector<int>& rem, 
 }  // namespace
 
 int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err) {
+                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err, bool pauli) {
     plan = AdjPlan();
     K = std::min(n, std::max(K, 8));
     plan.k = K;
@@ -374,8 +397,13 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
                     err = "bad derivative matrix offset";
                     return KSV_ERR_TYPE;
                 }
-                a.dmat[r] = (int32_t)(plan.mats.size() / 2);
-                plan.mats.insert(plan.mats.end(), mats + p.dmat, mats + p.dmat + 2 * (size_t)dim * dim);
+                const int pk = pauli && it.kind == AK_DENSE && dim == 2 ? pauli_rotation(it.m, mats + p.dmat) : -1;
+                if (pk >= 0) {   // dG G^dag = (-i/2) P: the kernel needs no derivative matrix
+                    a.dmat[r] = -1 - pk;
+                } else {
+                    a.dmat[r] = (int32_t)(plan.mats.size() / 2);
+                    plan.mats.insert(plan.mats.end(), mats + p.dmat, mats + p.dmat + 2 * (size_t)dim * dim);
+                }
                 a.slot[r] = (int16_t)((int)plan.slot_rec.size() - s.slot_begin);
                 plan.slot_rec.push_back(it.recs[r]);
             }
@@ -440,6 +468,10 @@ int adjoint_plan_mats(const AdjPlan& plan, const ksv_op* ops, size_t nops, const
         }
         for (int r = 0; r < a.nrec; r++) {
             const ksv_param& p = params[plan.aop_rec[x][r]];
+            if (a.dmat[r] < 0) {
+                err = "adjoint plan holds Pauli-rotation records (made for one parameter set)";
+                return KSV_ERR_TYPE;
+            }
             if (!read_mat(mats, nmats, p.dmat, dim, dm)) {
                 err = "bad derivative matrix offset";
                 return KSV_ERR_TYPE;

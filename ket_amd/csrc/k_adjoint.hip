@@ -162,6 +162,24 @@ __device__ __forceinline__ void adj_chain(V* P, V* L, const AOp& op, const KSV_C
                 for (int j = 0; j < 3; j++) {
                     if (j >= q.nrec) break;
                     asm volatile("" ::: "memory");
+                    if (KK == 1 && q.dmat[j] < 0) {
+                        // rotation about Pauli P: Re <lambda| (-i/2) P |phi>, phi before G^dag
+                        const int pk = -1 - q.dmat[j];
+                        double s = 0;
+#pragma unroll
+                        for (int g = 0; g < G; g++) {
+                            const double x0r = xr[g][0], x0i = xi[g][0], x1r = xr[g][1], x1i = xi[g][1];
+                            const double y0r = yr[g][0], y0i = yi[g][0], y1r = yr[g][1], y1i = yi[g][1];
+                            if (pk == 0)        // X
+                                s += (y0r * x1i - y0i * x1r) + (y1r * x0i - y1i * x0r);
+                            else if (pk == 1)   // Y
+                                s += (y1r * x0r + y1i * x0i) - (y0r * x1r + y0i * x1i);
+                            else                // Z
+                                s += (y0r * x0i - y0i * x0r) - (y1r * x1i - y1i * x1r);
+                        }
+                        acc[c][j] += 0.5 * s;
+                        continue;
+                    }
                     const KSV_CONST double* d = mats + 2 * (int64_t)q.dmat[j];
                     double s = 0;
 #pragma unroll
@@ -340,10 +358,8 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
 #pragma unroll
                     for (int j = 0; j < 3; j++) {
                         if (j >= m.nrec) break;
-                        double v = acc[c][j];
-#pragma unroll
-                        for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
-                        if (lane == 0) red[m.slot[j]][wave] += v;
+                        const double v = wave_sum_last(acc[c][j]);
+                        if (lane == 63) red[m.slot[j]][wave] += v;
                     }
                 }
                 o += span - 1;
@@ -609,7 +625,7 @@ extern "C" int ksv_adjoint(ksv_state* s, const ksv_op* ops, size_t nops, const d
     AdjPlan plan;
     {
         std::string err;
-        rc = plan_adjoint(n, atile, ops, nops, mats, nmats, params, nrec, plan, err);
+        rc = plan_adjoint(n, atile, ops, nops, mats, nmats, params, nrec, plan, err, true);
         if (rc) return adj_fail(rc, err);
     }
     const uint64_t ntiles = 1ull << (n - plan.k);

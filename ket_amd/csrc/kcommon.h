@@ -96,4 +96,26 @@ __device__ __forceinline__ const KSV_CONST T* to_const(const T* p) {
     return (const KSV_CONST T*)(p);
 }
 
+// Sum of v over the 64 lanes of a wave, valid in lane 63, by DPP moves (VALU
+// data paths between lanes) instead of LDS-crossbar permutes: within quads,
+// half rows and rows, then row 0 into row 1, row 2 into row 3 (row_bcast15)
+// and rows 0..1 into rows 2..3 (row_bcast31).  Lanes a DPP move does not
+// write, or whose source lies outside the wave, take 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double wave_sum_last(double v) {
+    v += dpp_f64<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141, 0xF>(v);   // row_half_mirror
+    v += dpp_f64<0x140, 0xF>(v);   // row_mirror
+    v += dpp_f64<0x142, 0xA>(v);   // row_bcast15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xC>(v);   // row_bcast31 into rows 2 and 3
+    return v;
+}
+
 }  // namespace ksv

@@ -294,7 +294,8 @@ struct AOp {
     int32_t mat;       // AK_DENSE: complex offset of G^dag in the pass matrices
     int8_t t[4];
     int8_t ins[8];     // ascending local positions of targets and local controls
-    int32_t dmat[3];   // complex offset of each record's dG (2^k x 2^k)
+    int32_t dmat[3];   // complex offset of each record's dG (2^k x 2^k), or -1 - P for a
+                       // rotation about Pauli P (0 X, 1 Y, 2 Z; one qubit): dG G^dag = (-i/2) P
     int16_t slot[3];   // pass-local slot of each record
     uint8_t flags;     // AOP_DIAG: G^dag and every dG are diagonal
     uint8_t run;       // AK_DENSE: ops of the chain this op heads (0: inside a chain)
@@ -348,9 +349,12 @@ int invert_tape(const ksv_op* ops, size_t nops, const double* mats, size_t nmats
                 std::vector<double>& imats, std::string& err);
 
 // adjplan.cpp: the inverse of `ops` split into tile passes of K qubits.
-// params[r].op names the tape op of record r (already validated).
+// params[r].op names the tape op of record r (already validated).  With
+// `pauli`, records of one-qubit Pauli rotations carry no derivative matrix
+// (AOp::dmat); a plan whose matrices are refilled for other parameter sets
+// (adjoint_plan_mats) is made without.
 int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mats, size_t nmats,
-                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err);
+                 const ksv_param* params, size_t nrec, AdjPlan& plan, std::string& err, bool pauli = false);
 
 // Batched copies (ksv_shots_run): a copy of nsys qubits fits one LDS tile, so one
 // LDS pass over the ntot-qubit state runs the whole gate segment on every copy,

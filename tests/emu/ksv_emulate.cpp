@@ -609,7 +609,7 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
                            int* stats) {
     AdjPlan plan;
     std::string err;
-    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err);
+    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err, true);
     if (rc) {
         g_msg = err;
         return rc;
@@ -778,6 +778,18 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
                         CHECK(m.nrec == 0 || m.k <= 2, "records on a wide op");
                         for (int j = 0; j < m.nrec; j++) {
                             CHECK(m.slot[j] >= 0 && m.slot[j] < s.slot_count, "bad slot");
+                            if (m.dmat[j] < 0) {
+                                // rotation about Pauli P: Re <y| (-i/2) P |x>, x before G^dag
+                                CHECK(D == 2 && m.dmat[j] >= -3, "Pauli record on a wrong op");
+                                const int pk = -1 - m.dmat[j];
+                                const cd h(0, -0.5);
+                                cd px[2];
+                                if (pk == 0) { px[0] = x[1]; px[1] = x[0]; }
+                                else if (pk == 1) { px[0] = cd(0, -1) * x[1]; px[1] = cd(0, 1) * x[0]; }
+                                else { px[0] = x[0]; px[1] = -x[1]; }
+                                red[m.slot[j]] += (std::conj(y[0]) * h * px[0] + std::conj(y[1]) * h * px[1]).real();
+                                continue;
+                            }
                             CHECK(m.dmat[j] >= 0 && 2 * ((size_t)m.dmat[j] + (size_t)D * D) <= plan.mats.size(), "derivative out of range");
                             const cd* d = reinterpret_cast<const cd*>(plan.mats.data()) + m.dmat[j];
                             double acc = 0;
@@ -836,7 +848,7 @@ extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, co
                                 const ksv_param* params, size_t nrec, int* stats) {
     AdjPlan plan;
     std::string err;
-    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err);
+    int rc = plan_adjoint(n, K, ops, nops, mats, nmats, params, nrec, plan, err, true);
     if (rc) {
         g_msg = err;
         return rc;
