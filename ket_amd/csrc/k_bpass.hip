@@ -386,12 +386,14 @@ static hipError_t launch_bpass_t(const Program& p, const Step& s, void* psi, hip
     a.swz = s.swz;
     const dim3 grid((unsigned)(1ull << (p.n - s.k)));
     const bool wide = p.bthreads == 512;
-    // FP64 passes of 3 or 4 dense blocks: their MFMA work is about the tile's HBM
+    // FP64 passes of 3 or 4 dense blocks (diagonal blocks not counted): their MFMA work is about the tile's HBM
     // time, which the pipelined walk overlaps (synthetic 30-qubit passes: 7.5 -> 7.1 ms
     // at 3 blocks, 9.8 -> 8.8 ms at 4).  With fewer blocks the one-tile kernel streams
     // faster; with more, the fragments leave room for one workgroup per CU only, and
     // FP32 passes gain nothing.
-    if (p.bpipe && sizeof(T) == 8 && s.bq == 4 && s.k == 12 && s.op_count >= 3 && s.op_count <= 4) {
+    int ndense = 0;   // diagonal blocks are phases, cheap next to the tile's HBM time
+    for (int i = 0; i < s.op_count; i++) ndense += (p.brecs[s.op_begin + i].flags & BREC_DIAG) ? 0 : 1;
+    if (p.bpipe && sizeof(T) == 8 && s.bq == 4 && s.k == 12 && ndense >= 3 && ndense <= 4) {
         bool done = false;
         hipError_t e = launch_pipe<T, 4, 12, 512>(a, p.n, st, &done);
         if (e != hipSuccess || done) return e;
