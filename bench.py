@@ -653,7 +653,11 @@ def main():
     achieved = per_launch / (avg_ms * 1e-3) / 1e9
     traffic = (traffic_from_profiles(f"k_bpass<double,{args.bq or 4},{args.btile or 12},{args.bthreads or 256}>@{n}q")
                if info["blk_passes"] == info["passes"] else None)
-    kname = "ksv::k_bpass" if info["blk_passes"] else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass"
+    # fused-block passes run as k_bpass (one tile per workgroup) or, for FP64 passes of
+    # 3-4 dense blocks, as the pipelined tile walk k_bpass_pipe; the roofline covers
+    # every pass launch of the schedule
+    kname = ("ksv::k_bpass + ksv::k_bpass_pipe (every fused-block pass launch)" if info["blk_passes"]
+             else "ksv::k_rpass" if info["reg_passes"] else "ksv::k_pass")
     hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
     if info["blk_passes"] == info["passes"] and info["blocks"]:
         # a dense 2^b-qubit block is a complex 2^b x 2^b by 2^b x 2^(n-b) product:
@@ -678,7 +682,7 @@ def main():
     # 2 x 2^n x 16 per launch over the HIP-event launch time); the fused blocks' MFMA
     # work is reported beside it
     roofline = {"bound": "hbm", **hbm, "traffic": traffic,
-                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel and workload, "
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of these kernels and workload, "
                                    "calibrated by a plain copy (profiles/hbm_traffic_k_bpass_random30.md)")
                 if traffic else None,
                 "kernel": kname, "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4),
