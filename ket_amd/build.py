@@ -27,6 +27,12 @@ FLAGS = [
 ]
 
 
+# Per-source flags.  k_bpass.hip: MFMA accumulators in VGPRs, so the block
+# epilogue reads them without v_accvgpr_read moves (VALU work that does not
+# overlap the FP64 MFMA work).
+SOURCE_FLAGS = {"k_bpass.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def build(verbose=False):
     """Compile every source and link libketsv.so; raises on failure."""
     os.makedirs(LIBDIR, exist_ok=True)
@@ -37,7 +43,8 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmd = [HIPCC] + FLAGS + ["-I", os.path.join(ROOT, "include")] + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = ([HIPCC] + FLAGS + SOURCE_FLAGS.get(src, []) + ["-I", os.path.join(ROOT, "include")] + lang +
+               ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
