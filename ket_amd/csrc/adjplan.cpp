@@ -164,6 +164,51 @@ void make_chains(const std::vector<AItem>& it, const std::vector<int>& chosen, u
     }
 }
 
+// Chains that may share one LDS round trip: adjacent chains of `order` whose
+// ops all act on one qubit each (an X whose controls lie in the tile, then
+// uncontrolled one-qubit ops on its target, or uncontrolled one-qubit ops
+// alone), on distinct targets, at most AGRP_MAX of them.  Running them one
+// after the other on the 2^k elements of each group over their targets is the
+// same arithmetic as running them in turn over the whole tile.  grp[x] = chains
+// of the group a chain head x starts (0 elsewhere, 1 for a chain left alone).
+void make_groups(const std::vector<AItem>& it, const std::vector<int>& order, const std::vector<int>& run,
+                 uint64_t T, std::vector<int>& grp, std::vector<int>& tgt) {
+    grp.assign(order.size(), 0);
+    tgt.assign(order.size(), -1);
+    auto one_qubit = [&](size_t x) {   // target of the chain at head x, or -1
+        const AItem& h = it[order[x]];
+        if (h.tg.size() != 1) return -1;
+        if (h.kind == AK_X) return x_head(h, T) ? h.tg[0] : -1;
+        if (h.kind != AK_DENSE || !h.ctl.empty()) return -1;
+        for (int c = 1; c < run[x]; c++)
+            if (!plain_1q(it[order[x + c]], h.tg[0])) return -1;
+        return h.tg[0];
+    };
+    size_t x = 0;
+    while (x < order.size()) {
+        const int t0 = one_qubit(x);
+        tgt[x] = t0;
+        if (t0 < 0) {
+            grp[x] = 1;
+            x += (size_t)std::max(run[x], 1);
+            continue;
+        }
+        uint64_t used = 1ull << t0;
+        size_t y = x + (size_t)std::max(run[x], 1);
+        int nch = 1;
+        while (nch < AGRP_MAX && y < order.size()) {
+            const int t = one_qubit(y);
+            if (t < 0 || ((used >> t) & 1)) break;
+            tgt[y] = t;
+            used |= 1ull << t;
+            nch++;
+            y += (size_t)std::max(run[y], 1);
+        }
+        grp[x] = nch;
+        x = y;
+    }
+}
+
 // Ops of `rem` that a pass over tile T takes (see the file comment), within
 // ADJ_SLOTS parameter records.  Scans at most `window` ops when chosen == null.
 int take(const std::vector<AItem>& it, const std::vector<int>& rem, uint64_t T, uint64_t all, size_t window,
@@ -314,7 +359,7 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
     const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
     // the tile always holds the lowest qubits: 128-byte runs of amplitudes
     const uint64_t forced = (1ull << std::min(K, 3)) - 1;
-    std::vector<int> rem(items.size()), chosen, rest, order, run;
+    std::vector<int> rem(items.size()), chosen, rest, order, run, grp, tgt;
     std::iota(rem.begin(), rem.end(), 0);
     while (!rem.empty()) {
         const AItem& first = items[rem[0]];
@@ -342,6 +387,7 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
         rest.clear();
         take(items, rem, T, all, 0, &chosen, &rest);
         make_chains(items, chosen, T, order, run);
+        make_groups(items, order, run, T, grp, tgt);
         AdjStep s;
         s.k = K;
         int pos[64];
@@ -355,6 +401,30 @@ int plan_adjoint(int n, int K, const ksv_op* ops, size_t nops, const double* mat
         s.op_begin = (int)plan.aops.size();
         s.slot_begin = (int)plan.slot_rec.size();
         for (size_t x = 0; x < order.size(); x++) {
+            if (grp[x] > 1) {
+                // group header: the targets of its chains, in order, and how many
+                // records follow
+                AOp g;
+                std::memset(&g, 0, sizeof g);
+                g.kind = AK_GRP;
+                g.k = grp[x];
+                std::vector<int> ins;
+                int members = 0;
+                size_t y = x;
+                for (int c = 0; c < grp[x]; c++) {
+                    g.t[c] = (int8_t)pos[tgt[y]];
+                    ins.push_back(pos[tgt[y]]);
+                    members += std::max(run[y], 1);
+                    y += (size_t)std::max(run[y], 1);
+                }
+                std::sort(ins.begin(), ins.end());
+                g.nins = (int32_t)ins.size();
+                for (size_t b = 0; b < ins.size(); b++) g.ins[b] = (int8_t)ins[b];
+                g.run = (uint8_t)members;
+                plan.aops.push_back(g);
+                plan.aop_src.push_back(-1);
+                plan.aop_rec.push_back({-1, -1, -1});
+            }
             const AItem& it = items[order[x]];
             AOp a;
             std::memset(&a, 0, sizeof a);
@@ -438,6 +508,7 @@ int adjoint_plan_mats(const AdjPlan& plan, const ksv_op* ops, size_t nops, const
     std::vector<cd> m, dm;
     for (size_t x = 0; x < plan.aops.size(); x++) {
         const AOp& a = plan.aops[x];
+        if (a.kind == AK_GRP) continue;   // a group header carries no matrices
         const int src = plan.aop_src[x];
         if (src < 0 || (size_t)src >= nops) {
             err = "adjoint plan and tape differ";

@@ -22,11 +22,20 @@ namespace ksv {
 
 // Local index of element e of an op: zero bits inserted at the op's
 // ascending positions, then its local control bits set.
+// One and two insertions (plain one-qubit chains, X heads with one control)
+// take a uniform branch of their own: the unrolled general form costs a select
+// per possible insertion.
 __device__ __forceinline__ uint32_t aop_elem(uint32_t e, const AOp& op) {
     uint32_t l = e;
+    if (op.nins == 1) {
+        l = ins0(l, op.ins[0]);
+    } else if (op.nins == 2) {
+        l = ins0(ins0(l, op.ins[0]), op.ins[1]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 5; i++)
-        if (i < op.nins) l = ins0(l, op.ins[i]);
+        for (int i = 0; i < 5; i++)
+            if (i < op.nins) l = ins0(l, op.ins[i]);
+    }
     return l | op.lctl;
 }
 
@@ -245,6 +254,205 @@ __device__ __forceinline__ void adj_chain_g(V* P, V* L, const AOp& op, const KSV
     adj_chain<V, KK, 1, CMAX, NT>(P, L, op, opw, run, mats, ne, tid, acc, xctl);
 }
 
+// One member of a group (AK_GRP) on the group's sub-target S: an X swaps each
+// pair whose control bits are set; a one-qubit op applies G^dag to phi and
+// lambda and adds its records' terms to red (one wave sum per record).
+template <int M, int S, int G>
+__device__ __forceinline__ void grp_member(double (&xr)[G][1 << M], double (&xi)[G][1 << M],
+                                           double (&yr)[G][1 << M], double (&yi)[G][1 << M],
+                                           const uint32_t (&id)[G][1 << M], const AOp& m,
+                                           const KSV_CONST double* mats, double* red, int nw, uint32_t lane,
+                                           uint32_t wave) {
+    constexpr int D = 1 << M;
+    constexpr int B = 1 << (M - 1 - S);   // the pair partner's element bit
+    if (m.kind == AK_X) {
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                if (r & B) continue;
+                if ((id[g][r] & m.lctl) == m.lctl) {
+                    double t;
+                    t = xr[g][r]; xr[g][r] = xr[g][r | B]; xr[g][r | B] = t;
+                    t = xi[g][r]; xi[g][r] = xi[g][r | B]; xi[g][r | B] = t;
+                    t = yr[g][r]; yr[g][r] = yr[g][r | B]; yr[g][r | B] = t;
+                    t = yi[g][r]; yi[g][r] = yi[g][r | B]; yi[g][r | B] = t;
+                }
+            }
+        return;
+    }
+    const KSV_CONST double* u = mats + 2 * (int64_t)m.mat;
+    const bool diag = (m.flags & AOP_DIAG) != 0;
+    double ar[G][D], ai[G][D], br[G][D], bi[G][D];
+    if (diag) {
+        const double d0r = u[0], d0i = u[1], d1r = u[6], d1i = u[7];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                const double mr = (r & B) ? d1r : d0r, mi = (r & B) ? d1i : d0i;
+                ar[g][r] = fma(mr, xr[g][r], -mi * xi[g][r]);
+                ai[g][r] = fma(mr, xi[g][r], mi * xr[g][r]);
+                br[g][r] = fma(mr, yr[g][r], -mi * yi[g][r]);
+                bi[g][r] = fma(mr, yi[g][r], mi * yr[g][r]);
+            }
+    } else {
+        const double u00r = u[0], u00i = u[1], u01r = u[2], u01i = u[3];
+        const double u10r = u[4], u10i = u[5], u11r = u[6], u11i = u[7];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                if (r & B) continue;
+                const int r1 = r | B;
+                const double x0r = xr[g][r], x0i = xi[g][r], x1r = xr[g][r1], x1i = xi[g][r1];
+                const double y0r = yr[g][r], y0i = yi[g][r], y1r = yr[g][r1], y1i = yi[g][r1];
+                ar[g][r] = fma(-u01i, x1i, fma(u01r, x1r, fma(-u00i, x0i, fma(u00r, x0r, 0.0))));
+                ai[g][r] = fma(u01i, x1r, fma(u01r, x1i, fma(u00i, x0r, fma(u00r, x0i, 0.0))));
+                ar[g][r1] = fma(-u11i, x1i, fma(u11r, x1r, fma(-u10i, x0i, fma(u10r, x0r, 0.0))));
+                ai[g][r1] = fma(u11i, x1r, fma(u11r, x1i, fma(u10i, x0r, fma(u10r, x0i, 0.0))));
+                br[g][r] = fma(-u01i, y1i, fma(u01r, y1r, fma(-u00i, y0i, fma(u00r, y0r, 0.0))));
+                bi[g][r] = fma(u01i, y1r, fma(u01r, y1i, fma(u00i, y0r, fma(u00r, y0i, 0.0))));
+                br[g][r1] = fma(-u11i, y1i, fma(u11r, y1r, fma(-u10i, y0i, fma(u10r, y0r, 0.0))));
+                bi[g][r1] = fma(u11i, y1r, fma(u11r, y1i, fma(u10i, y0r, fma(u10r, y0i, 0.0))));
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (j >= m.nrec) break;
+        asm volatile("" ::: "memory");
+        double sacc = 0;
+        if (m.dmat[j] < 0) {
+            // rotation about Pauli P: Re <lambda| (-i/2) P |phi>, phi before G^dag
+            const int pk = -1 - m.dmat[j];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < D; r++) {
+                    if (r & B) continue;
+                    const int r1 = r | B;
+                    const double x0r = xr[g][r], x0i = xi[g][r], x1r = xr[g][r1], x1i = xi[g][r1];
+                    const double y0r = yr[g][r], y0i = yi[g][r], y1r = yr[g][r1], y1i = yi[g][r1];
+                    if (pk == 0)        // X
+                        sacc += (y0r * x1i - y0i * x1r) + (y1r * x0i - y1i * x0r);
+                    else if (pk == 1)   // Y
+                        sacc += (y1r * x0r + y1i * x0i) - (y0r * x1r + y0i * x1i);
+                    else                // Z
+                        sacc += (y0r * x0i - y0i * x0r) - (y1r * x1i - y1i * x1r);
+                }
+            sacc *= 0.5;
+        } else {
+            // Re sum_r conj(lambda_r) (dG G^dag phi)_r, lambda before its own G^dag
+            const KSV_CONST double* d = mats + 2 * (int64_t)m.dmat[j];
+            const double d00r = d[0], d00i = d[1], d01r = d[2], d01i = d[3];
+            const double d10r = d[4], d10i = d[5], d11r = d[6], d11i = d[7];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < D; r++) {
+                    if (r & B) continue;
+                    const int r1 = r | B;
+                    const double a0r = ar[g][r], a0i = ai[g][r], a1r = ar[g][r1], a1i = ai[g][r1];
+                    double t0r, t0i, t1r, t1i;
+                    if (diag) {
+                        t0r = fma(d00r, a0r, -d00i * a0i);
+                        t0i = fma(d00r, a0i, d00i * a0r);
+                        t1r = fma(d11r, a1r, -d11i * a1i);
+                        t1i = fma(d11r, a1i, d11i * a1r);
+                    } else {
+                        t0r = fma(-d01i, a1i, fma(d01r, a1r, fma(-d00i, a0i, fma(d00r, a0r, 0.0))));
+                        t0i = fma(d01i, a1r, fma(d01r, a1i, fma(d00i, a0r, fma(d00r, a0i, 0.0))));
+                        t1r = fma(-d11i, a1i, fma(d11r, a1r, fma(-d10i, a0i, fma(d10r, a0r, 0.0))));
+                        t1i = fma(d11i, a1r, fma(d11r, a1i, fma(d10i, a0r, fma(d10r, a0i, 0.0))));
+                    }
+                    sacc = fma(yi[g][r], t0i, fma(yr[g][r], t0r, sacc));
+                    sacc = fma(yi[g][r1], t1i, fma(yr[g][r1], t1r, sacc));
+                }
+        }
+        const double v = wave_sum_last(sacc);
+        if (lane == 63) red[m.slot[j] * nw + wave] += v;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            xr[g][r] = ar[g][r]; xi[g][r] = ai[g][r];
+            yr[g][r] = br[g][r]; yi[g][r] = bi[g][r];
+        }
+}
+
+// A group (AK_GRP): each thread takes G groups of the 2^M elements over the
+// group's targets, runs the group's `nmem` member records on them in order in
+// registers, and writes them back: one LDS round trip for up to AGRP_MAX chains.
+template <typename V, int M, int G, int NT>
+__device__ __forceinline__ void adj_group(V* P, V* L, const AOp& grp, const KSV_CONST uint64_t* memw, int nmem,
+                                          const KSV_CONST double* mats, uint32_t ne, uint32_t tid, double* red,
+                                          uint32_t lane, uint32_t wave) {
+    constexpr int D = 1 << M;
+    // every lane runs every iteration (the members' wave sums need the whole
+    // wave); lanes past the tile's groups hold zeros and store nothing
+    const uint32_t iters = (ne + G * NT - 1) / (G * NT);
+    for (uint32_t it = 0; it < iters; it++) {
+        const uint32_t e0 = tid + it * G * NT;
+        uint32_t id[G][D];
+        bool ok[G];
+        double xr[G][D], xi[G][D], yr[G][D], yi[G][D];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            ok[g] = e0 + g * NT < ne;
+            const uint32_t l = aop_elem(e0 + g * NT, grp);
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                uint32_t q = l;
+#pragma unroll
+                for (int b = 0; b < M; b++)
+                    if ((r >> (M - 1 - b)) & 1) q |= 1u << grp.t[b];
+                id[g][r] = q;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                V x, y;
+                x.x = x.y = y.x = y.y = 0.0;
+                if (ok[g]) {
+                    x = P[id[g][r]];
+                    y = L[id[g][r]];
+                }
+                xr[g][r] = x.x; xi[g][r] = x.y;
+                yr[g][r] = y.x; yi[g][r] = y.y;
+            }
+        for (int c = 0; c < nmem; c++) {
+            // one member's fields in scalar registers at a time
+            asm volatile("" ::: "memory");
+            AOp m;
+            {
+                uint64_t w[sizeof(AOp) / 8];
+#pragma unroll
+                for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = memw[(sizeof(AOp) / 8) * c + i];
+                __builtin_memcpy(&m, w, sizeof m);
+            }
+            const int sub = m.t[0] == grp.t[0] ? 0 : (m.t[0] == grp.t[1] ? 1 : 2);
+            if (sub == 0) grp_member<M, 0, G>(xr, xi, yr, yi, id, m, mats, red, NT / 64, lane, wave);
+            else if (sub == 1) grp_member<M, 1, G>(xr, xi, yr, yi, id, m, mats, red, NT / 64, lane, wave);
+            else if constexpr (M > 2) grp_member<M, (M > 2 ? 2 : 0), G>(xr, xi, yr, yi, id, m, mats, red, NT / 64, lane, wave);
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                V x, y;
+                x.x = xr[g][r]; x.y = xi[g][r];
+                y.x = yr[g][r]; y.y = yi[g][r];
+                if (ok[g]) {
+                    P[id[g][r]] = x;
+                    L[id[g][r]] = y;
+                }
+            }
+    }
+}
+
 // One backward pass: the workgroup walks tiles blockIdx.x, +gridDim.x, ...;
 // for each it stages phi and lambda in LDS, applies the pass's ops to both
 // and accumulates the records' inner products in red[slot][wave]; at the
@@ -297,6 +505,19 @@ __global__ __launch_bounds__(NT) void k_adjpass(AdjPassArgs a) {
 #pragma unroll
                 for (int i = 0; i < (int)(sizeof(AOp) / 8); i++) w[i] = ops[(sizeof(AOp) / 8) * o + i];
                 __builtin_memcpy(&op, w, sizeof op);
+            }
+            if (op.kind == AK_GRP) {
+                const KSV_CONST uint64_t* memw = ops + (sizeof(AOp) / 8) * (o + 1);
+                const uint32_t ne = tsize >> op.nins;
+                if (op.k == 2) {
+                    if (ne >= 2 * NT) adj_group<V, 2, 2, NT>(P, L, op, memw, op.run, mats, ne, tid, &red[0][0], lane, wave);
+                    else adj_group<V, 2, 1, NT>(P, L, op, memw, op.run, mats, ne, tid, &red[0][0], lane, wave);
+                } else {
+                    adj_group<V, 3, 1, NT>(P, L, op, memw, op.run, mats, ne, tid, &red[0][0], lane, wave);
+                }
+                o += op.run;
+                __syncthreads();
+                continue;
             }
             if ((base & op.gctl) != op.gctl) {   // a chain shares its head's controls
                 if (op.run > 1) o += op.run - 1;

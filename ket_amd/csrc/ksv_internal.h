@@ -282,6 +282,10 @@ enum AKind : int32_t {
     AK_DENSE = 0,   // 2^k x 2^k (k <= 3) G^dag on local bits t[0] (MSB) .. t[k-1]
     AK_X = 1,       // X on local bit t[0] (CNOT, Toffoli; self-inverse)
     AK_SWAP = 2,    // exchange local bits t[0], t[1] (SWAP, CSWAP; self-inverse)
+    AK_GRP = 3,     // the next `run` records (one-qubit chains on distinct targets
+                    // t[0] .. t[k-1], k = 2 or 3: X heads with local controls and
+                    // uncontrolled one-qubit ops) run in one LDS round trip, each
+                    // thread holding the 2^k elements of its groups over those targets
 };
 
 struct AOp {
@@ -307,6 +311,7 @@ constexpr uint8_t AOP_DIAG = 1;
 // each element group while it sits in registers.
 constexpr int ACHAIN_1Q = 4;        // longest chain of 1-qubit ops
 constexpr int ACHAIN_2Q = 2;        // ... of 2-qubit ops
+constexpr int AGRP_MAX = 3;         // one-qubit chains sharing one LDS round trip
 
 constexpr int ADJ_SLOTS = 128;      // parameter records per pass
 constexpr int ADJ_DENSE_MAX = 3;    // widest dense op inside a pass

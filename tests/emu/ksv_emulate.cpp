@@ -674,6 +674,103 @@ extern "C" int emu_adjoint(int n, int K, const ksv_op* ops, size_t nops, const d
                 uint32_t xctl = 0;
                 int xpos = -1;
                 const AOp& h = plan.aops[s.op_begin + o];
+                if (h.kind == AK_GRP) {
+                    // one-qubit chains on distinct targets, run in turn on the 2^M
+                    // elements of each group over those targets
+                    const int M = h.k;
+                    CHECK((M == 2 || M == 3) && h.nins == M && h.gctl == 0 && h.lctl == 0 && h.run >= M &&
+                              o + h.run < s.op_count,
+                          "bad group header");
+                    for (int i = 0; i < M; i++) {
+                        CHECK(h.ins[i] >= 0 && h.ins[i] < k && (i == 0 || h.ins[i] > h.ins[i - 1]), "bad group bits");
+                        CHECK(h.t[i] >= 0 && h.t[i] < k, "group target outside the tile");
+                    }
+                    const int D = 1 << M;
+                    for (uint32_t e = 0; e < (ts >> M); e++) {
+                        uint32_t l = e;
+                        for (int i = 0; i < M; i++) l = (uint32_t)ins0(l, h.ins[i]);
+                        uint32_t id[8];
+                        cd x[8], y[8], a[8], b[8];
+                        for (int r = 0; r < D; r++) {
+                            uint32_t g = l;
+                            for (int bb = 0; bb < M; bb++)
+                                if ((r >> (M - 1 - bb)) & 1) g |= 1u << h.t[bb];
+                            CHECK(g < ts, "group element outside the tile");
+                            id[r] = g;
+                            x[r] = P[g];
+                            y[r] = L[g];
+                        }
+                        for (int c = 1; c <= h.run; c++) {
+                            const AOp& m = plan.aops[s.op_begin + o + c];
+                            int sub = -1;
+                            for (int i = 0; i < M; i++)
+                                if (h.t[i] == m.t[0]) sub = i;
+                            CHECK(sub >= 0 && m.k == 1 && m.gctl == 0, "group member off the group's targets");
+                            const int B = 1 << (M - 1 - sub);
+                            if (m.kind == AK_X) {
+                                CHECK((m.lctl & (1u << m.t[0])) == 0, "X control on its target");
+                                for (int r = 0; r < D; r++)
+                                    if (!(r & B) && (id[r] & m.lctl) == m.lctl) {
+                                        std::swap(x[r], x[r | B]);
+                                        std::swap(y[r], y[r | B]);
+                                    }
+                                continue;
+                            }
+                            CHECK(m.kind == AK_DENSE && m.lctl == 0, "group member kind");
+                            const bool diag = (m.flags & AOP_DIAG) != 0;
+                            CHECK(m.mat >= 0 && 2 * ((size_t)m.mat + 4) <= plan.mats.size(), "matrix out of range");
+                            const cd* u = reinterpret_cast<const cd*>(plan.mats.data()) + m.mat;
+                            if (diag) CHECK(u[1] == cd(0, 0) && u[2] == cd(0, 0), "diagonal flag on a dense matrix");
+                            for (int r = 0; r < D; r++) {
+                                if (r & B) continue;
+                                const int r1 = r | B;
+                                a[r] = u[0] * x[r] + u[1] * x[r1];
+                                a[r1] = u[2] * x[r] + u[3] * x[r1];
+                                b[r] = u[0] * y[r] + u[1] * y[r1];
+                                b[r1] = u[2] * y[r] + u[3] * y[r1];
+                            }
+                            for (int j = 0; j < m.nrec; j++) {
+                                CHECK(m.slot[j] >= 0 && m.slot[j] < s.slot_count, "bad slot");
+                                double acc = 0;
+                                if (m.dmat[j] < 0) {
+                                    CHECK(m.dmat[j] >= -3, "Pauli record on a wrong op");
+                                    const int pk = -1 - m.dmat[j];
+                                    const cd hh(0, -0.5);
+                                    for (int r = 0; r < D; r++) {
+                                        if (r & B) continue;
+                                        const int r1 = r | B;
+                                        cd px0, px1;
+                                        if (pk == 0) { px0 = x[r1]; px1 = x[r]; }
+                                        else if (pk == 1) { px0 = cd(0, -1) * x[r1]; px1 = cd(0, 1) * x[r]; }
+                                        else { px0 = x[r]; px1 = -x[r1]; }
+                                        acc += (std::conj(y[r]) * hh * px0 + std::conj(y[r1]) * hh * px1).real();
+                                    }
+                                } else {
+                                    CHECK(2 * ((size_t)m.dmat[j] + 4) <= plan.mats.size(), "derivative out of range");
+                                    const cd* d = reinterpret_cast<const cd*>(plan.mats.data()) + m.dmat[j];
+                                    if (diag) CHECK(d[1] == cd(0, 0) && d[2] == cd(0, 0), "diagonal flag on a dense derivative");
+                                    for (int r = 0; r < D; r++) {
+                                        if (r & B) continue;
+                                        const int r1 = r | B;
+                                        const cd t0 = d[0] * a[r] + d[1] * a[r1], t1 = d[2] * a[r] + d[3] * a[r1];
+                                        acc += (std::conj(y[r]) * t0 + std::conj(y[r1]) * t1).real();
+                                    }
+                                }
+                                red[m.slot[j]] += acc;
+                            }
+                            for (int r = 0; r < D; r++) {
+                                x[r] = a[r];
+                                y[r] = b[r];
+                            }
+                        }
+                        for (int r = 0; r < D; r++) {
+                            P[id[r]] = x[r];
+                            L[id[r]] = y[r];
+                        }
+                    }
+                    o += h.run;
+                    continue;
+                }
                 if (h.kind == AK_X && h.run > 1) {
                     CHECK(h.k == 1 && h.gctl == 0 && h.lctl != 0 && h.run <= 1 + ACHAIN_1Q && o + h.run <= s.op_count,
                           "bad X-headed chain");
@@ -863,7 +960,8 @@ extern "C" int emu_adjoint_plan(int n, int K, const ksv_op* ops, size_t nops, co
             for (int o = 0; o < s.op_count; o++) {
                 const AOp& op = plan.aops[s.op_begin + o];
                 stats[3]++;
-                o += op.kind == AK_DENSE ? op.run - 1 : 0;
+                // a group or a chain (dense or X-headed) is one step
+                o += op.kind == AK_GRP ? op.run : (op.run > 1 ? op.run - 1 : 0);
             }
         }
     }
