@@ -423,9 +423,8 @@ __device__ __forceinline__ void adj_group(V* P, V* L, const AOp& grp, const KSV_
                 xr[g][r] = x.x; xi[g][r] = x.y;
                 yr[g][r] = y.x; yi[g][r] = y.y;
             }
+#pragma unroll 2
         for (int c = 0; c < nmem; c++) {
-            // one member's fields in scalar registers at a time
-            asm volatile("" ::: "memory");
             AOp m;
             {
                 uint64_t w[sizeof(AOp) / 8];
