@@ -294,7 +294,7 @@ static void program_blob(const ksv_state* s, const Program& p, std::vector<char>
         }
         std::vector<double> fr(total, 0.0);
         for (size_t i = 0; i < p.brecs.size(); i++)
-            block_fragments(p.bmats.data() + p.bmat_off[i], bqs[i], (p.brecs[i].flags & BREC_DIAG) != 0,
+            block_fragments(p.bmats.data() + p.bmat_off[i], bqs[i], p.brecs[i].flags,
                             fr.data() + p.brecs[i].frag);
         if (s->prec == KSV_F64) {
             off[3] = put(fr.data(), 8 * total);
@@ -459,7 +459,7 @@ int ksv_program_set_matrix(ksv_program* p, size_t op, const double* mat, size_t 
         const int R = 1 << p->blk_bq[b];
         const size_t nf = (size_t)(2 * R / 16) * (2 * R / 4) * 64;
         std::vector<double> fr(nf);
-        block_fragments(p->bmats.data() + p->bmat_off[b], p->blk_bq[b], (p->brecs[b].flags & BREC_DIAG) != 0, fr.data());
+        block_fragments(p->bmats.data() + p->bmat_off[b], p->blk_bq[b], p->brecs[b].flags, fr.data());
         if (f64) {
             HIPCHK(hipMemcpy((double*)p->d_frags + p->brecs[b].frag, fr.data(), 8 * nf, hipMemcpyHostToDevice));
         } else {

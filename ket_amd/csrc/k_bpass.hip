@@ -113,8 +113,10 @@ struct TileMap {
     }
 };
 
-// One block of a pass on the tile in LDS (record w, fragments af).
-template <typename T, int BQ, int K, int NT>
+// One block of a pass on the tile in LDS (record w, fragments af).  MONO: the
+// build has the monomial-block path (the pipelined walk leaves it out: its
+// registers would cost that kernel its second workgroup per CU).
+template <typename T, int BQ, int K, int NT, bool MONO = true>
 __device__ __forceinline__ void bpass_block(typename BVec2<T>::type* lds, const KSV_CONST uint64_t* w,
                                             const T (&af)[3 * ((1 << BQ) / 16) * ((1 << BQ) / 4)], uint32_t lane,
                                             uint32_t wave) {
@@ -182,6 +184,37 @@ __device__ __forceinline__ void bpass_block(typename BVec2<T>::type* lds, const 
             }
         }
         return;
+    }
+    if constexpr (BQ == 4 && MONO) {
+    if (w[7] & BREC_MONO) {
+        // one entry per column: row r moves to row pi(r) (nibble r of w[1]) times
+        // U[pi(r)][r].  The wave reads all rows of its columns before writing any,
+        // and columns belong to one wave, so no write can overtake a read.
+        const uint64_t perm = w[1];
+        uint32_t wa[R / 4];
+#pragma unroll
+        for (int t = 0; t < R / 4; t++) {
+            const uint32_t pr = (uint32_t)(perm >> (4 * (4 * t + rl))) & 15u;
+            wa[t] = ((pr & 1) ? ar[0] : 0u) ^ ((pr & 2) ? ar[1] : 0u) ^ ((pr & 4) ? ar[2] : 0u) ^
+                    ((pr & 8) ? ar[3] : 0u);
+        }
+#pragma unroll
+        for (int m = 0; m < NBW; m++) {
+            const uint32_t col = col_of(m);
+            V x[R / 4];
+#pragma unroll
+            for (int t = 0; t < R / 4; t++) x[t] = lds[ra[t] ^ col];
+#pragma unroll
+            for (int t = 0; t < R / 4; t++) {
+                const T pr = af[2 * t], pi = af[2 * t + 1];
+                V y;
+                y.x = x[t].x * pr - x[t].y * pi;
+                y.y = x[t].x * pi + x[t].y * pr;
+                lds[wa[t] ^ col] = y;
+            }
+        }
+        return;
+    }
     }
     // the slices own disjoint columns: slice m + 1 is read from LDS while
     // slice m runs on MFMA
@@ -331,7 +364,7 @@ __global__ __launch_bounds__(NT) void k_bpass_pipe(BPassArgs a, uint64_t ntiles)
             for (int f = 0; f < NL; f++) af[f] = flds[(blk * NL + f) * 64 + lane];
 #pragma unroll
             for (int f = NL; f < NF; f++) af[f] = af[f - NL] + af[f - NL / 2];   // Ur + Ui, as the host forms it
-            bpass_block<T, BQ, K, NT>(lds, w, af, lane, wave);
+            bpass_block<T, BQ, K, NT, false>(lds, w, af, lane, wave);
             lds_barrier();
         }
         const uint64_t base = TM::base(a, t);
@@ -391,9 +424,13 @@ static hipError_t launch_bpass_t(const Program& p, const Step& s, void* psi, hip
     // at 3 blocks, 9.8 -> 8.8 ms at 4).  With fewer blocks the one-tile kernel streams
     // faster; with more, the fragments leave room for one workgroup per CU only, and
     // FP32 passes gain nothing.
-    int ndense = 0;   // diagonal blocks are phases, cheap next to the tile's HBM time
-    for (int i = 0; i < s.op_count; i++) ndense += (p.brecs[s.op_begin + i].flags & BREC_DIAG) ? 0 : 1;
-    if (p.bpipe && sizeof(T) == 8 && s.bq == 4 && s.k == 12 && ndense >= 3 && ndense <= 4) {
+    int ndense = 0, nmono = 0;   // diagonal and monomial blocks are phases, cheap next to the tile's HBM time
+    for (int i = 0; i < s.op_count; i++) {
+        const int64_t f = p.brecs[s.op_begin + i].flags;
+        ndense += (f & (BREC_DIAG | BREC_MONO)) ? 0 : 1;
+        nmono += (f & BREC_MONO) ? 1 : 0;
+    }
+    if (p.bpipe && sizeof(T) == 8 && s.bq == 4 && s.k == 12 && ndense >= 3 && ndense <= 4 && nmono == 0) {
         bool done = false;
         hipError_t e = launch_pipe<T, 4, 12, 512>(a, p.n, st, &done);
         if (e != hipSuccess || done) return e;

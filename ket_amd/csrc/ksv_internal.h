@@ -117,9 +117,14 @@ struct BRec {
     uint16_t ac[16];   // ... of column bit i, i < K - BQ
     int64_t frag;      // offset (elements) of the block's A fragments: [mb][kk][lane]
     int64_t flags;     // BREC_DIAG: every absorbed op is diagonal; the fragments hold the
-                       // diagonal and the block is a phase per row instead of a GEMM
+                       // diagonal and the block is a phase per row instead of a GEMM.
+                       // BREC_MONO (4-qubit blocks): every absorbed op is diagonal or a
+                       // permutation, so U has one entry per column: row r goes to row
+                       // pi(r) times U[pi(r)][r]; pi(r) is nibble r of ar[4..7] (unused
+                       // row words of a 4-qubit block) and the fragments hold the phases
 };
 constexpr int64_t BREC_DIAG = 1;
+constexpr int64_t BREC_MONO = 2;
 static_assert(sizeof(BRec) == 64, "BRec layout");
 
 struct BPassArgs {
@@ -262,7 +267,7 @@ void schedule_blocks(int n, int btile, int bq, std::vector<LOp>& lops, Program& 
 // size of the real embedding's fragments): Ur, Ui and Ur + Ui of the
 // 2^bq x 2^bq matrix u (row-major), each [mb][kk][lane]; for a diagonal
 // block, the diagonal (fragments 2t / 2t + 1).
-void block_fragments(const cd* u, int bq, bool diag, double* out);
+void block_fragments(const cd* u, int bq, int64_t flags, double* out);
 // Lower and schedule a tape into `p` (host side only).  kernel: KSV_OPT_PASS_KERNEL
 // value; tile / rtile / btile: LDS, register-round and fused-block tile widths;
 // bq: fused-block width (0 = defaults).

@@ -824,6 +824,7 @@ struct FBlock {
     uint64_t touch = 0;
     std::vector<int> ops;     // lowered ops absorbed, in application order
     bool diag = true;         // every absorbed op is diagonal (by kind, whatever its angles)
+    bool mono = true;         // every absorbed op is diagonal or a permutation
 };
 
 // Apply a logical op to a small dense vector over qubits Q (bit i = Q[i]).
@@ -969,6 +970,7 @@ static void fuse_blocks(int n, int bq, const std::vector<LOp>& lops, std::vector
         f.touch |= o.touch;
         f.ops.push_back((int)i);
         f.diag = f.diag && o.cls == L_DIAG;
+        f.mono = f.mono && (o.cls == L_DIAG || o.cls == L_PERMX || o.cls == L_SWAP);
         for (int q : S) last[q] = best;
     }
 }
@@ -1052,15 +1054,53 @@ static void fuse_blocks_front(int n, int bq, const std::vector<LOp>& lops, std::
             f.touch |= lops[i].touch;
             f.ops.push_back(i);
             f.diag = f.diag && lops[i].cls == L_DIAG;
+            f.mono = f.mono && (lops[i].cls == L_DIAG || lops[i].cls == L_PERMX || lops[i].cls == L_SWAP);
             done[i] = 1;
         }
         blocks.push_back(f);
     }
 }
 
-void block_fragments(const cd* u, int bq, bool diag, double* out) {
+// Row of the one entry of column c of a monomial block, or -1.
+static int mono_row(const cd* u, int R, int c) {
+    int row = -1;
+    for (int r = 0; r < R; r++)
+        if (u[(size_t)r * R + c] != cd(0, 0)) {
+            if (row >= 0) return -1;
+            row = r;
+        }
+    return row;
+}
+
+// pi of a monomial 4-qubit block as 16 nibbles (nibble r = pi(r)), or false.
+static bool mono_perm(const cd* u, uint64_t& word) {
+    word = 0;
+    uint32_t seen = 0;
+    for (int c = 0; c < 16; c++) {
+        const int r = mono_row(u, 16, c);
+        if (r < 0 || ((seen >> r) & 1)) return false;
+        seen |= 1u << r;
+        word |= (uint64_t)r << (4 * c);
+    }
+    return true;
+}
+
+void block_fragments(const cd* u, int bq, int64_t flags, double* out) {
     const int R = 1 << bq, MB = 2 * R / 16, KS = 2 * R / 4;
-    if (diag) {
+    if (flags & BREC_MONO) {
+        // fragment 2t / 2t + 1, lane l: Re / Im of U[pi(r)][r], r = 4t + l / 16 (the
+        // rows a lane reads in the kernel), the phase of row r on its way to pi(r)
+        std::fill(out, out + (size_t)MB * KS * 64, 0.0);
+        for (int t = 0; t < R / 4; t++)
+            for (int l = 0; l < 64; l++) {
+                const int r = 4 * t + l / 16;
+                const int pr = std::max(mono_row(u, R, r), 0);
+                out[(size_t)(2 * t) * 64 + l] = u[(size_t)pr * R + r].real();
+                out[(size_t)(2 * t + 1) * 64 + l] = u[(size_t)pr * R + r].imag();
+            }
+        return;
+    }
+    if (flags & BREC_DIAG) {
         // fragment 2t / 2t + 1, lane l: Re / Im of U[r][r], r = 4t + l / 16 (the rows a
         // lane reads in the kernel); the remaining fragments are unused
         std::fill(out, out + (size_t)MB * KS * 64, 0.0);
@@ -1171,6 +1211,11 @@ static void block_pass(int n, int K, int bq, uint64_t T, std::vector<FBlock>& bl
         for (int i = 0; i < K - bq; i++) r.ac[i] = (uint16_t)swz_addr(st.swz, 1u << colpos[b][i]);
         r.frag = (int64_t)prog.brecs.size() * nfrag;
         r.flags = f.diag ? BREC_DIAG : 0;
+        uint64_t perm = 0;
+        if (!f.diag && f.mono && bq == 4 && mono_perm(f.u.data(), perm)) {
+            r.flags = BREC_MONO;
+            for (int i = 0; i < 4; i++) r.ar[4 + i] = (uint16_t)(perm >> (16 * i));
+        }
         prog.bmat_off.push_back((int64_t)prog.bmats.size());
         prog.bmats.insert(prog.bmats.end(), f.u.begin(), f.u.end());
         for (int id : f.ops)

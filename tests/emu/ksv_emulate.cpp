@@ -341,8 +341,8 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
     const size_t nf = (size_t)MB * KS * 64;
     for (int i = 0; i < s.op_count; i++) {
         fr[i].resize(nf);
-        block_fragments(p.bmats.data() + p.bmat_off[s.op_begin + i], bq,
-                        (p.brecs[s.op_begin + i].flags & BREC_DIAG) != 0, fr[i].data());
+        block_fragments(p.bmats.data() + p.bmat_off[s.op_begin + i], bq, p.brecs[s.op_begin + i].flags,
+                        fr[i].data());
     }
     for (uint64_t b = 0; b < (N >> K); b++) {
         uint64_t base = b;
@@ -395,6 +395,45 @@ static int emu_blk(const Program& p, const Step& s, cd* psi, int n) {
                     }
                 for (uint32_t a = 0; a < (1u << K); a++)
                     CHECK(rd[a] == 1 && wr[a] == 1, "diagonal block LDS slot not read and written exactly once");
+                lds.swap(out);
+                continue;
+            }
+            if (r.flags & BREC_MONO) {
+                // one entry per column: lane l reads rows 4t + l/16 of its column and
+                // writes row pi(row) (nibble row of ar[4..7]) times U[pi(row)][row]
+                CHECK(bq == 4, "monomial flag on a 5-qubit block");
+                uint64_t perm = 0;
+                for (int i = 0; i < 4; i++) perm |= (uint64_t)r.ar[4 + i] << (16 * i);
+                const cd* u = p.bmats.data() + p.bmat_off[s.op_begin + bi];
+                for (int wv = 0; wv < NW; wv++)
+                    for (int m = 0; m < NBW; m++) {
+                        const uint32_t nb = (uint32_t)wv + (uint32_t)NW * (uint32_t)m;
+                        for (int l = 0; l < 64; l++) {
+                            const uint32_t rl = l >> 4, cl = l & 15;
+                            uint32_t col = 0;
+                            for (int bb = 0; bb < 4; bb++)
+                                if ((cl >> bb) & 1) col ^= r.ac[bb];
+                            for (int bb = 0; bb < K - bq - 4; bb++)
+                                if ((nb >> bb) & 1) col ^= r.ac[4 + bb];
+                            for (int t = 0; t < R / 4; t++) {
+                                const uint32_t row = 4u * (uint32_t)t + rl;
+                                const uint32_t pr = (uint32_t)(perm >> (4 * row)) & 15u;
+                                uint32_t ra = 0, wa = 0;
+                                for (int bb = 0; bb < bq; bb++) {
+                                    if ((row >> bb) & 1) ra ^= r.ar[bb];
+                                    if ((pr >> bb) & 1) wa ^= r.ar[bb];
+                                }
+                                const cd ph(af[(size_t)(2 * t) * 64 + l], af[(size_t)(2 * t + 1) * 64 + l]);
+                                CHECK(ph == u[(size_t)pr * R + row], "monomial phase is not U[pi(r)][r]");
+                                CHECK((ra ^ col) < (1u << K) && (wa ^ col) < (1u << K), "monomial block address out of range");
+                                rd[ra ^ col]++;
+                                wr[wa ^ col]++;
+                                out[wa ^ col] = lds[ra ^ col] * ph;
+                            }
+                        }
+                    }
+                for (uint32_t a = 0; a < (1u << K); a++)
+                    CHECK(rd[a] == 1 && wr[a] == 1, "monomial block LDS slot not read and written exactly once");
                 lds.swap(out);
                 continue;
             }
@@ -561,6 +600,21 @@ int emu_run_updated(int n, int prune, const ksv_op* ops, size_t nops, const doub
         stats[4] = blks;
         stats[5] = (int64_t)p.brecs.size();
     }
+    return 0;
+}
+
+// Fused-block plan of a tape in exact mode: kinds[0] dense, [1] diagonal, [2] monomial blocks.
+int emu_block_kinds(int n, const ksv_op* ops, size_t nops, const double* mats, size_t nmats, int btile, int bq,
+                    int64_t* kinds) {
+    Program p;
+    std::string err;
+    int rc = plan_program(p, n, KSV_F64, ops, nops, mats, nmats, 3, 0, 0, true, false, err, btile, bq);
+    if (rc) {
+        g_msg = err;
+        return rc;
+    }
+    kinds[0] = kinds[1] = kinds[2] = 0;
+    for (const BRec& r : p.brecs) kinds[(r.flags & BREC_DIAG) ? 1 : ((r.flags & BREC_MONO) ? 2 : 0)]++;
     return 0;
 }
 

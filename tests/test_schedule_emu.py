@@ -180,6 +180,33 @@ def test_fused_block_random_layers(bq, btile):
     assert np.max(np.abs(got - ref)) < TOL
 
 
+@pytest.mark.parametrize("bq,btile", [(4, 12), (4, 11)])
+def test_fused_monomial_blocks(bq, btile):
+    """Blocks of permutations and phases only (cnot, x, swap, cz, s, t after a
+    layer of rotations, and the brick circuit's trailing cnots) move each row to
+    its image times its phase instead of running a GEMM; the emulated kernel
+    equals the dense oracle and the plan holds such blocks."""
+    n = 14
+    calls = workloads.random_layer_calls(n, 3, seed=31)
+    for q in range(0, n - 1, 2):
+        calls += [("cnot", (q, q + 1)), ("x", (q,)), ("cz", (q + 1, (q + 3) % n))]
+    calls += [("swap", (0, 5)), ("s", (3,)), ("t", (7,)), ("cnot", (9, 2)), ("y", (11,))]
+    ref = dense_ref(n, calls, prune=False)
+    got, stats = run_emu(n, calls, prune=False, kernel="mfma", btile=btile, bq=bq)
+    assert stats[4] > 0
+    assert np.max(np.abs(got - ref)) < TOL
+    ops, mats = build(n, calls).tape()
+    ops = np.ascontiguousarray(ops, dtype=OP_DTYPE)
+    mats = np.ascontiguousarray(mats, dtype=np.float64)
+    kinds = np.zeros(3, dtype=np.int64)
+    L = emu()
+    L.emu_block_kinds.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    assert L.emu_block_kinds(n, ops.ctypes.data, len(ops), mats.ctypes.data, len(mats), btile, bq,
+                             kinds.ctypes.data) == 0
+    assert kinds[2] > 0, kinds
+
+
 def test_fused_blocks_fall_back_under_pruning():
     n = 13
     calls = random_calls(n, 80, seed=99)
