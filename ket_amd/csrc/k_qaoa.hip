@@ -75,8 +75,12 @@ __device__ __forceinline__ uint32_t qidx(uint32_t t, uint32_t j) {
 }
 
 template <typename T, int LW, int LR>
-__device__ __forceinline__ void qexch(T (&re)[QR], T (&im)[QR], typename QV2<T>::type* lds, uint32_t tid) {
+__device__ __forceinline__ void qexch(T (&re)[QR], T (&im)[QR], typename QV2<T>::type* lds, uint32_t tid_in) {
     using V = typename QV2<T>::type;
+    // the LDS addresses are formed in every exchange: shared between the
+    // exchanges of psi and lambda, they would hold 48 registers across the mixer
+    uint32_t tid = tid_in;
+    asm volatile("" : "+v"(tid));
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < QR; j++) {
@@ -140,7 +144,10 @@ __device__ __forceinline__ void qmix_round(T (&pr)[QR], T (&pi)[QR], T (&lr)[QR]
 #undef QSLOT
 }
 
-template <typename T>
+// One kernel per mode (MODE = a QMode): register allocation is per kernel, and
+// the backward modes hold phi and lambda, so a single kernel for every mode
+// would give the forward ones the backward ones' registers.
+template <typename T, int MODE>
 __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     using V = typename QV2<T>::type;
     __shared__ V lds[1 << 12];
@@ -155,7 +162,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     const int layer = a.layer;
     V* psi = reinterpret_cast<V*>(a.psi) + q.off;
     V* lam = reinterpret_cast<V*>(a.lam) + q.off;
-    const bool passb = a.mode == QM_FWD_B || a.mode == QM_BWD_B;
+    const bool passb = MODE == QM_FWD_B || MODE == QM_BWD_B;
 
     // physical index of tile index x: pass A tiles are qubits 0..11 (tile t
     // on qubits 12..n-1); pass B tiles are qubits 0..L-1 on tile bits 0..L-1
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     __shared__ V phs[256];
     int cut_hh = 0;
     uint32_t a01[12];                      // a0 in the low 16 bits, a1 in the high 16 bits
-    const bool tab = a.mode == QM_FWD_A || a.mode == QM_ENERGY || a.mode == QM_BWD_A;
+    const bool tab = MODE == QM_FWD_A || MODE == QM_ENERGY || MODE == QM_BWD_A;
     if (tab) {
         const uint32_t th = (uint32_t)(t << 12);      // the tile's high bits
         const uint32_t hm = (uint32_t)(((1ull << q.n) - 1) & ~0xFFFull);
@@ -186,10 +193,10 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
             const uint32_t a0 = (uint32_t)__popc(adj[u] & hm & th), a1 = (uint32_t)__popc(adj[u] & hm & ~th);
             a01[u] = a0 | (a1 << 16);
         }
-        if (a.mode != QM_ENERGY) {
+        if (MODE != QM_ENERGY) {
             // this layer's phases (k_qphs); the backward pass takes their conjugates
             const double* ph = a.phs + ((size_t)q.gbase + a.layer) * QPMAX * 2;
-            const double sg = a.mode == QM_BWD_A ? -1.0 : 1.0;
+            const double sg = MODE == QM_BWD_A ? -1.0 : 1.0;
             for (int c = (int)tid; c <= q.nedges; c += QNT) {
                 V v;
                 v.x = (T)ph[2 * c];
@@ -207,7 +214,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     };
 
     T pr[QR], pi[QR], lr[QR], li[QR];
-    if (a.mode == QM_INIT) {
+    if (MODE == QM_INIT) {
         const T amp = (T)(1.0 / sqrt((double)(1ull << q.n)));
 #pragma unroll
         for (int j = 0; j < QR; j++) {
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
         }
         return;
     }
-    const bool bwd = a.mode == QM_BWD_A || a.mode == QM_BWD_B;
+    const bool bwd = MODE == QM_BWD_A || MODE == QM_BWD_B;
 #pragma unroll
     for (int j = 0; j < QR; j++) {
         const uint64_t g = phys(qidx<1>(tid, j));
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
             li[j] = u.y;
         }
     }
-    if (a.mode == QM_ENERGY) {
+    if (MODE == QM_ENERGY) {
         double e = 0;
 #pragma unroll
         for (int j = 0; j < QR; j++) {
@@ -262,7 +269,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
     double gb = 0, gg = 0;
 
     // forward pass A: cost phase first
-    if (a.mode == QM_FWD_A) {
+    if (MODE == QM_FWD_A) {
 #pragma unroll
         for (int j = 0; j < QR; j++) {
             const V phv = phs[cut_of(qidx<1>(tid, j))];
@@ -288,7 +295,7 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
         if (bwd) qexch<T, 3, 1>(lr, li, lds, tid);
     }
     // backward pass A: the mixer is undone, now dE/dgamma and the inverse cost phase
-    if (a.mode == QM_BWD_A) {
+    if (MODE == QM_BWD_A) {
 #pragma unroll
         for (int j = 0; j < QR; j++) {
             const int cut = cut_of(qidx<1>(tid, j));
@@ -305,9 +312,13 @@ __global__ __launch_bounds__(QNT) void k_qaoa(QArgs a) {
             li[j] = __builtin_fma(cr, xi, ci * xr);
         }
     }
+    // the store addresses are formed again here: kept from the loads, they would
+    // hold 32 registers across the whole mixer
+    uint32_t tid_st = tid;
+    asm volatile("" : "+v"(tid_st));
 #pragma unroll
     for (int j = 0; j < QR; j++) {
-        const uint64_t g = phys(qidx<1>(tid, j));
+        const uint64_t g = phys(qidx<1>(tid_st, j));
         V v;
         v.x = pr[j];
         v.y = pi[j];
@@ -462,8 +473,20 @@ static int qlaunch(ksv_batch* b, int mode, int layer, double* part) {
     a.layer = layer;
     a.p = b->p;
     a.mode = mode;
-    if (b->prec == KSV_F64) hipLaunchKernelGGL((k_qaoa<double>), dim3((unsigned)b->nwork), dim3(QNT), 0, b->st, a);
-    else hipLaunchKernelGGL((k_qaoa<float>), dim3((unsigned)b->nwork), dim3(QNT), 0, b->st, a);
+    const dim3 grid((unsigned)b->nwork), blk(QNT);
+#define QL(M)                                                                         \
+    if (b->prec == KSV_F64) hipLaunchKernelGGL((k_qaoa<double, M>), grid, blk, 0, b->st, a); \
+    else hipLaunchKernelGGL((k_qaoa<float, M>), grid, blk, 0, b->st, a);
+    switch (mode) {
+    case QM_INIT: QL(QM_INIT) break;
+    case QM_FWD_A: QL(QM_FWD_A) break;
+    case QM_FWD_B: QL(QM_FWD_B) break;
+    case QM_ENERGY: QL(QM_ENERGY) break;
+    case QM_BWD_B: QL(QM_BWD_B) break;
+    case QM_BWD_A: QL(QM_BWD_A) break;
+    default: return qfail(KSV_ERR_RANGE, "bad QAOA kernel mode");
+    }
+#undef QL
     QHIP(hipGetLastError());
     return KSV_OK;
 }
