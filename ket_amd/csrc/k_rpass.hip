@@ -38,6 +38,7 @@
 // state from one register set to the other.)
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "kcommon.h"
 
@@ -367,6 +368,41 @@ __device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, 
     }
 }
 
+// The same exchange through a real array of half the size: real parts, then
+// imaginary parts (two more barriers).  Builds without the dense path use it,
+// so that their workgroups take 16 KiB of LDS at 11-qubit tiles and three
+// waves fit on a SIMD.
+template <typename T, int TB, typename P>
+__device__ __forceinline__ void r_exch_half(RREGS_ARGS, T* lds, P xw, uint32_t tid) {
+    uint64_t xs[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) xs[i] = rec_word(xw, i);
+    auto x = [&](int i) -> uint32_t { return (uint32_t)(xs[i >> 2] >> (16 * (i & 3))) & 0xFFFF; };
+    uint32_t wb = 0, rb = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        const uint32_t bit = (tid >> i) & 1;
+        wb ^= bit ? x(i) : 0u;
+        rb ^= bit ? x(16 + i) : 0u;
+    }
+    const uint32_t w0 = x(12), w1 = x(13), w2 = x(14), w3 = x(15);
+    const uint32_t r0 = x(28), r1 = x(29), r2 = x(30), r3 = x(31);
+    auto wa = [&](int j) { return wb ^ ((j & 1) ? w0 : 0) ^ ((j & 2) ? w1 : 0) ^ ((j & 4) ? w2 : 0) ^ ((j & 8) ? w3 : 0); };
+    auto ra = [&](int j) { return rb ^ ((j & 1) ? r0 : 0) ^ ((j & 2) ? r1 : 0) ^ ((j & 4) ? r2 : 0) ^ ((j & 8) ? r3 : 0); };
+    __syncthreads();        // the previous LDS use of every thread is finished
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) lds[wa(j)] = re[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) re[j] = lds[ra(j)];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) lds[wa(j)] = im[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RREGS; j++) im[j] = lds[ra(j)];
+}
+
 #define RSLOT_CASES(SLOT, FN) \
     switch (SLOT) {           \
     case 0: FN(0); break;     \
@@ -421,13 +457,15 @@ __device__ __forceinline__ uint64_t slot_offset(int j, uint32_t rmap) {
 // op stream (2888 against 3367 ms for the 30-qubit random circuit in 11-qubit
 // register rounds).
 template <typename T, int TB, bool PR, bool DN, bool STG>
-__global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
+__device__ __forceinline__ void rpass_body(const RPassArgs& a) {
     using V = typename Vec2<T>::type;
     constexpr int K = TB + RSLOTS;
     constexpr int NT = 1 << TB;
     constexpr int C = NT / 8;                 // op records per staged chunk: 16 bytes per thread
     static_assert(C <= 64 && sizeof(ROp) == 128, "one wave's ballot covers a chunk");
-    __shared__ V lds[1 << K];
+    // the dense path stages complex amplitudes; without it the exchange moves
+    // real and imaginary parts in turn through half the space
+    __shared__ typename std::conditional<DN || STG, V, T>::type lds[1 << K];
     __shared__ uint4 stage[STG ? 2 : 1][STG ? NT : 1];   // two chunks of op records
     const uint32_t tid = threadIdx.x;
     V* psi = reinterpret_cast<V*>(a.psi);
@@ -481,15 +519,18 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
         const uint64_t gctl = h3;
         const auto pay = rec_pay(w, (T*)nullptr);
         if (kind == R_EXCH) {
-            r_exch<T, TB>(re, im, lds, w + 4, tid);
+            if constexpr (DN || STG) r_exch<T, TB>(re, im, lds, w + 4, tid);
+            else r_exch_half<T, TB>(re, im, lds, w + 4, tid);
             return;
         }
+        if constexpr (DN || STG) {
         if (DN && kind == R_DENSE) {
             // every thread reaches the barrier inside, whatever gctl says
             const bool on = (base & gctl) == gctl;
             const bool lp = on && ((tid & tctl) == tctl);
             r_dense<T, TB, PR>(re, im, lds, kq, locw, rctl, cmats + 2 * mat, tid, true, lp);
             return;
+        }
         }
         if ((base & gctl) != gctl) return;
         const bool lp = (tid & tctl) == tctl;
@@ -582,8 +623,28 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
     }
 }
 
+template <typename T, int TB, bool PR, bool DN, bool STG>
+__global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
+    rpass_body<T, TB, PR, DN, STG>(a);
+}
+
+// FP64 rounds under ket's pruning without the dense path (the library's default
+// mode on circuits of one- and two-qubit gates): held to 168 registers, three
+// waves per SIMD instead of two (the half-size exchange leaves LDS for them).
+template <int TB>
+__global__ __launch_bounds__(1 << TB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_rpass_prune3(RPassArgs a) {
+    rpass_body<double, TB, true, false, false>(a);
+}
+
 template <typename T, bool PR, bool DN, bool STG>
 static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
+    if constexpr (sizeof(T) == 8 && PR && !DN && !STG) {
+        const dim3 grid((unsigned)a.ntiles);
+        if (tb == 7) hipLaunchKernelGGL((k_rpass_prune3<7>), grid, dim3(1 << 7), 0, st, a);
+        else if (tb == 8) hipLaunchKernelGGL((k_rpass_prune3<8>), grid, dim3(1 << 8), 0, st, a);
+        else hipLaunchKernelGGL((k_rpass_prune3<9>), grid, dim3(1 << 9), 0, st, a);
+        return;
+    }
     const dim3 grid((unsigned)a.ntiles);
     if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR, DN, STG>), grid, dim3(1 << 7), 0, st, a);
     else if (tb == 8) hipLaunchKernelGGL((k_rpass<T, 8, PR, DN, STG>), grid, dim3(1 << 8), 0, st, a);
