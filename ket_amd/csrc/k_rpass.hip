@@ -102,6 +102,32 @@ __device__ __forceinline__ void prune2(T& r, T& i) {
     }
 }
 
+// ket's accumulate test, deferred to the end of an op.  max(|re|, |im|) >= PRUNE_T
+// implies norm2 >= 1e-14 as ket rounds it (fl(T * T) >= 1e-14 and the rounded sum
+// of two non-negative squares is at least the larger one), so only amplitudes
+// below PRUNE_T are candidates: an op tests each output with one max and one
+// compare, and runs the exact test (prune_all) over the amplitudes it wrote only
+// where some lane of the wave holds a candidate.
+constexpr double PRUNE_T = 1.0000001e-7;
+
+template <typename T>
+__device__ __forceinline__ bool pcand(T r, T i) {
+    return (double)__builtin_fmax(__builtin_fabs(r), __builtin_fabs(i)) < PRUNE_T;
+}
+
+// the op wrote register j when j has every bit of rctl (and, with a thread-bit
+// control, when this thread is selected)
+template <typename T, bool PR, bool SEL>
+__device__ __forceinline__ void prune_all(T (&re)[RREGS], T (&im)[RREGS], bool cand, uint32_t rctl, bool lp) {
+    if (PR && cand && (!SEL || lp)) {
+#pragma unroll
+        for (int j = 0; j < RREGS; j++) {
+            if (((uint32_t)j & rctl) != rctl) continue;
+            prune2<T, PR>(re[j], im[j]);
+        }
+    }
+}
+
 // (ore, oim) = a * x0 + b * x1.  PR (ket's per-gate pruning): ket's
 // add(mul(a, x0), mul(b, x1)), every product and sum rounded on its own
 // (src/statevector.ts:61-62), so results and prune decisions are ket's bit
@@ -144,6 +170,7 @@ __device__ __forceinline__ void dmul(T& yr, T& yi, T dx, T dy, T xr, T xi) {
 template <typename T, int S, bool PR, bool SEL, bool RC, typename P>
 __device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp, uint32_t form) {
     const CM<T> m0 = ldm(m, 0), m1 = ldm(m, 1), m2 = ldm(m, 2), m3 = ldm(m, 3);
+    bool cand = false;
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
         if ((j >> S) & 1) continue;
@@ -165,8 +192,7 @@ __device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp, ui
             mac2<T, PR>(y0r, y0i, m0, x0r, x0i, m1, x1r, x1i);
             mac2<T, PR>(y1r, y1i, m2, x0r, x0i, m3, x1r, x1i);
         }
-        prune2<T, PR>(y0r, y0i);
-        prune2<T, PR>(y1r, y1i);
+        if (PR) cand = cand | pcand(y0r, y0i) | pcand(y1r, y1i);
         if (SEL) {
             y0r = lp ? y0r : x0r; y0i = lp ? y0i : x0i;
             y1r = lp ? y1r : x1r; y1i = lp ? y1i : x1i;
@@ -174,6 +200,7 @@ __device__ __forceinline__ void r_m1(RREGS_ARGS, P m, uint32_t rctl, bool lp, ui
         re[j] = y0r; im[j] = y0i;
         re[j1] = y1r; im[j1] = y1i;
     }
+    prune_all<T, PR, SEL>(re, im, cand, RC ? rctl : 0u, lp);
 }
 
 template <typename T, int S, bool SEL>
@@ -216,17 +243,19 @@ __device__ __forceinline__ void r_diag1(RREGS_ARGS, uint32_t rctl, P pay, bool h
     CM<T> dt;
     dt.x = hi ? d1.x : d0.x;
     dt.y = hi ? d1.y : d0.y;
+    bool cand = false;
 #pragma unroll
     for (int j = 0; j < RREGS; j++) {
         if (((uint32_t)j & rctl) != rctl) continue;
         const CM<T> d = S >= 0 ? (((j >> (S < 0 ? 0 : S)) & 1) ? d1 : d0) : dt;
         T yr, yi;
         dmul<T, PR>(yr, yi, d.x, d.y, re[j], im[j]);
-        prune2<T, PR>(yr, yi);
+        if (PR) cand = cand | pcand(yr, yi);
         if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
         re[j] = yr;
         im[j] = yi;
     }
+    prune_all<T, PR, SEL>(re, im, cand, rctl, lp);
 }
 
 // Diagonal over k <= 6 qubits.  Row bit (k-1-i) of qubit i comes from a
@@ -254,6 +283,7 @@ __device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, ui
             rl |= rb;
         }
     }
+    bool cand = false;
     if (k <= 2) {
         const CM<T> d0 = ldm(pay, 0), d1 = ldm(pay, 1), d2 = ldm(pay, 2), d3 = ldm(pay, 3);
 #pragma unroll
@@ -264,7 +294,7 @@ __device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, ui
             const T dy = (row & 2) ? ((row & 1) ? d3.y : d2.y) : ((row & 1) ? d1.y : d0.y);
             T yr, yi;
             dmul<T, PR>(yr, yi, dx, dy, re[j], im[j]);
-            prune2<T, PR>(yr, yi);
+            if (PR) cand = cand | pcand(yr, yi);
             if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
             re[j] = yr;
             im[j] = yi;
@@ -277,12 +307,13 @@ __device__ __forceinline__ void r_diag(RREGS_ARGS, uint32_t k, uint64_t locw, ui
             const T dx = wide[2 * row], dy = wide[2 * row + 1];
             T yr, yi;
             dmul<T, PR>(yr, yi, dx, dy, re[j], im[j]);
-            prune2<T, PR>(yr, yi);
+            if (PR) cand = cand | pcand(yr, yi);
             if (SEL) { yr = lp ? yr : re[j]; yi = lp ? yi : im[j]; }
             re[j] = yr;
             im[j] = yi;
         }
     }
+    prune_all<T, PR, SEL>(re, im, cand, rctl, lp);
 }
 
 // Dense 2^k x 2^k (k = 2..4) on slots ts[0] (MSB) .. ts[k-1].  The thread's
@@ -639,11 +670,16 @@ __global__ __launch_bounds__(1 << TB) __attribute__((amdgpu_waves_per_eu(3, 8)))
 template <typename T, bool PR, bool DN, bool STG>
 static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
     if constexpr (sizeof(T) == 8 && PR && !DN && !STG) {
+        // 13-qubit rounds spill too much at 168 registers: they keep two waves
         const dim3 grid((unsigned)a.ntiles);
-        if (tb == 7) hipLaunchKernelGGL((k_rpass_prune3<7>), grid, dim3(1 << 7), 0, st, a);
-        else if (tb == 8) hipLaunchKernelGGL((k_rpass_prune3<8>), grid, dim3(1 << 8), 0, st, a);
-        else hipLaunchKernelGGL((k_rpass_prune3<9>), grid, dim3(1 << 9), 0, st, a);
-        return;
+        if (tb == 7) {
+            hipLaunchKernelGGL((k_rpass_prune3<7>), grid, dim3(1 << 7), 0, st, a);
+            return;
+        }
+        if (tb == 8) {
+            hipLaunchKernelGGL((k_rpass_prune3<8>), grid, dim3(1 << 8), 0, st, a);
+            return;
+        }
     }
     const dim3 grid((unsigned)a.ntiles);
     if (tb == 7) hipLaunchKernelGGL((k_rpass<T, 7, PR, DN, STG>), grid, dim3(1 << 7), 0, st, a);
