@@ -401,8 +401,8 @@ __device__ __forceinline__ void r_exch(RREGS_ARGS, typename Vec2<T>::type* lds, 
 
 // The same exchange through a real array of half the size: real parts, then
 // imaginary parts (two more barriers).  Builds without the dense path use it,
-// so that their workgroups take 16 KiB of LDS at 11-qubit tiles and three
-// waves fit on a SIMD.
+// so that their workgroups take 16 KiB of LDS (plus staged records) at
+// 11-qubit tiles and three waves fit on a SIMD.
 template <typename T, int TB, typename P>
 __device__ __forceinline__ void r_exch_half(RREGS_ARGS, T* lds, P xw, uint32_t tid) {
     uint64_t xs[8];
@@ -496,7 +496,7 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a) {
     static_assert(C <= 64 && sizeof(ROp) == 128, "one wave's ballot covers a chunk");
     // the dense path stages complex amplitudes; without it the exchange moves
     // real and imaginary parts in turn through half the space
-    __shared__ typename std::conditional<DN || STG, V, T>::type lds[1 << K];
+    __shared__ typename std::conditional<DN, V, T>::type lds[1 << K];
     __shared__ uint4 stage[STG ? 2 : 1][STG ? NT : 1];   // two chunks of op records
     const uint32_t tid = threadIdx.x;
     V* psi = reinterpret_cast<V*>(a.psi);
@@ -550,12 +550,12 @@ __device__ __forceinline__ void rpass_body(const RPassArgs& a) {
         const uint64_t gctl = h3;
         const auto pay = rec_pay(w, (T*)nullptr);
         if (kind == R_EXCH) {
-            if constexpr (DN || STG) r_exch<T, TB>(re, im, lds, w + 4, tid);
+            if constexpr (DN) r_exch<T, TB>(re, im, lds, w + 4, tid);
             else r_exch_half<T, TB>(re, im, lds, w + 4, tid);
             return;
         }
-        if constexpr (DN || STG) {
-        if (DN && kind == R_DENSE) {
+        if constexpr (DN) {
+        if (kind == R_DENSE) {
             // every thread reaches the barrier inside, whatever gctl says
             const bool on = (base & gctl) == gctl;
             const bool lp = on && ((tid & tctl) == tctl);
@@ -660,24 +660,25 @@ __global__ __launch_bounds__(1 << TB) void k_rpass(RPassArgs a) {
 }
 
 // FP64 rounds under ket's pruning without the dense path (the library's default
-// mode on circuits of one- and two-qubit gates): held to 168 registers, three
-// waves per SIMD instead of two (the half-size exchange leaves LDS for them).
-template <int TB>
+// mode on circuits of one- and two-qubit gates, and the copies of batched
+// trajectories): held to 168 registers, three waves per SIMD instead of two (the
+// half-size exchange leaves LDS for them).
+template <int TB, bool STG>
 __global__ __launch_bounds__(1 << TB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_rpass_prune3(RPassArgs a) {
-    rpass_body<double, TB, true, false, false>(a);
+    rpass_body<double, TB, true, false, STG>(a);
 }
 
 template <typename T, bool PR, bool DN, bool STG>
 static void launch_tb(const RPassArgs& a, int tb, hipStream_t st) {
-    if constexpr (sizeof(T) == 8 && PR && !DN && !STG) {
+    if constexpr (sizeof(T) == 8 && PR && !DN) {
         // 13-qubit rounds spill too much at 168 registers: they keep two waves
         const dim3 grid((unsigned)a.ntiles);
         if (tb == 7) {
-            hipLaunchKernelGGL((k_rpass_prune3<7>), grid, dim3(1 << 7), 0, st, a);
+            hipLaunchKernelGGL((k_rpass_prune3<7, STG>), grid, dim3(1 << 7), 0, st, a);
             return;
         }
         if (tb == 8) {
-            hipLaunchKernelGGL((k_rpass_prune3<8>), grid, dim3(1 << 8), 0, st, a);
+            hipLaunchKernelGGL((k_rpass_prune3<8, STG>), grid, dim3(1 << 8), 0, st, a);
             return;
         }
     }

@@ -515,3 +515,22 @@ def test_openmp_build_is_bit_identical(prune):
     b = O.zero_state(n)
     O.apply_tape_omp(n, b, ops, mats, prune=prune)
     assert np.array_equal(a, b)
+
+
+def test_prune_candidate_bound():
+    """The register-round kernels test ket's isNegligible (norm2 < 1e-14, products
+    and sum rounded on their own, src/complex.ts:24-27) exactly only for outputs
+    with max(|re|, |im|) < 1.0000001e-7: at or above that bound the rounded norm is
+    at least 1e-14 (fl(T*T) >= 1e-14, and fl(a + b) >= a for b >= 0), checked here
+    on the bound itself and on random pairs at and just above it; just below it
+    both outcomes occur, so the exact test is needed there."""
+    T = 1.0000001e-7
+    assert T * T >= 1e-14
+    rng = np.random.default_rng(3)
+    big = T * (1 + rng.uniform(0, 1e-6, 200000))
+    small = rng.uniform(0, 1, 200000) * big
+    for re, im in ((big, small), (small, big), (big, np.zeros_like(big)), (-big, -small)):
+        n2 = re * re + im * im
+        assert np.all(n2 >= 1e-14)
+    # below the bound: candidates on both sides of ket's test
+    assert 8e-8 * 8e-8 + 8e-8 * 8e-8 >= 1e-14 and 5e-8 * 5e-8 + 5e-8 * 5e-8 < 1e-14
